@@ -1,0 +1,159 @@
+"""Oracle callables for the named built-in forms of the GPU path (test infrastructure only).
+
+In the reference a user writes ``p.deriv``, ``p.linear``, ``p.initial`` and
+``p.observe`` as MATLAB function handles (e.g. Examples/SDE_1D/Kubo.m:16-17,
+Examples/SPDE_2D/Soliton.m:16-18, Examples/SPDE_3D/GPE.m:24-25).  The GPU
+path offers a library of such handles selected by name through ``p['gpu']``
+(precedent: Applications/Quantum/qpreferences.m:48-60 picks built-in derivs by
+flag).  This module turns the same ``gpu`` dictionary into FP64 callables with
+the reference's calling conventions, so the oracle integrates exactly the
+equation the GPU integrates.
+
+Forms (``gpu`` keys):
+
+deriv    'zero'                        D = 0
+         'kubo'   c                    D = i c w0 a                       (Kubo.m:17)
+         'gp'     g, u, s              D = g |a|^2 a + u a + s (w0 + i w1)   [noises >= 2]
+                                       D = g |a|^2 a + u a + s w0            [noises == 1]
+linear   l0, l1[d], l2[d]              L = l0 + sum_d l1[d] D_d + l2[d] D_d^2  (xpropfactor.m:150-156)
+initial  'const'  c                    a = c
+         'sech'   amp, width, eps      a = amp sech(x/width) + eps (v0 + i v1)/sqrt2
+         'gauss'  amp, width           a = amp exp(-(x^2+y^2)/(2 width^2))
+         'phase'  amp                  a = amp exp(2 pi i u0),  u0 = uniform(0,1)
+         'coherent' alpha              a = alpha + (v0 + i v1)/2  (Wigner vacuum)
+observe  'a', 'abs2', 'a_ave', 'abs2_ave', 'abs2_int', 'abs4_ave', 'corr'
+"""
+import numpy as np
+
+DERIVS = ("zero", "kubo", "gp")
+INITIALS = ("const", "sech", "gauss", "phase", "coherent")
+OBSERVES = ("a", "abs2", "a_ave", "abs2_ave", "abs2_int", "abs4_ave", "corr")
+
+
+def _space_axes(p):
+    return tuple(range(1, p.dimensions))
+
+
+def make_deriv(g):
+    kind = g.get("deriv", "zero")
+    if kind == "zero":
+        return lambda a, w, p: 0 * a
+    if kind == "kubo":
+        c = float(g.get("c", 1.0))
+        return lambda a, w, p: 1j * c * w[0:1] * a
+    if kind == "gp":
+        gg = complex(g.get("g", 0.0))
+        u = complex(g.get("u", 0.0))
+        s = float(g.get("s", 0.0))
+
+        def deriv(a, w, p):
+            d = gg * (np.conj(a) * a) * a + u * a
+            if s != 0.0 and p.noises >= 2:
+                d = d + s * (w[0:1] + 1j * w[1:2])
+            elif s != 0.0 and p.noises == 1:
+                d = d + s * w[0:1]
+            return d
+        return deriv
+    raise ValueError("unknown drift form %r" % (kind,))
+
+
+def make_linear(g):
+    lin = g.get("linear", None)
+    if lin is None:
+        return None
+    l0 = complex(lin.get("l0", 0.0))
+    l1 = lin.get("l1", 0.0)
+    l2 = lin.get("l2", 0.0)
+
+    def linear(p):
+        nd = p.dimensions
+        c1 = np.broadcast_to(np.asarray(l1, complex), (max(1, nd - 1),))
+        c2 = np.broadcast_to(np.asarray(l2, complex), (max(1, nd - 1),))
+        L = l0 + 0j
+        for d in range(1, nd):
+            L = L + c1[d - 1] * p.D[d] + c2[d - 1] * p.D[d] ** 2
+        return L
+    return linear
+
+
+def make_initial(g):
+    ini = g.get("initial", "const")
+    if ini == "const":
+        c = complex(g.get("c", 1.0))
+        return lambda *args: c + 0 * args[0]
+    if ini == "sech":
+        amp = float(g.get("amp", 1.0))
+        wd = float(g.get("width", 1.0))
+        eps = float(g.get("eps", 0.0))
+
+        def initial(v, p):
+            a = amp / np.cosh(p.x / wd) + 0j
+            if eps != 0.0:
+                a = a + eps * (v[0:1] + 1j * v[1:2]) / np.sqrt(2.0)
+            return a
+        return initial
+    if ini == "gauss":
+        amp = float(g.get("amp", 1.0))
+        wd = float(g.get("width", 1.0))
+
+        def initial(v, p):
+            r2 = p.x ** 2
+            if p.dimensions > 2:
+                r2 = r2 + p.y ** 2
+            return amp * np.exp(-0.5 * r2 / wd ** 2) + 0j
+        return initial
+    if ini == "phase":
+        amp = float(g.get("amp", 1.0))
+
+        def initial(*args):
+            p = args[-1]
+            u = args[-2] * p.dv                                  # uniform(0,1): xrandom.m:199 scales by 1/dv
+            return amp * np.exp(2j * np.pi * u[0:1])
+        return initial
+    if ini == "coherent":
+        alpha = complex(g.get("alpha", 1.0))
+
+        def initial(v, p):
+            return alpha + (v[0:1] + 1j * v[1:2]) / 2.0
+        return initial
+    raise ValueError("unknown initial form %r" % (ini,))
+
+
+def make_observe(name):
+    if name == "a":
+        return lambda a, p: a[0:1]
+    if name == "abs2":
+        return lambda a, p: (np.conj(a) * a)[0:1]
+    if name == "a_ave":
+        return lambda a, p: np.mean(a[0:1], axis=_space_axes(p)) if p.dimensions > 1 else a[0:1]
+    if name == "abs2_ave":
+        return lambda a, p: (np.mean((np.conj(a) * a)[0:1], axis=_space_axes(p))
+                             if p.dimensions > 1 else (np.conj(a) * a)[0:1])
+    if name == "abs4_ave":
+        return lambda a, p: (np.mean((np.abs(a) ** 4)[0:1], axis=_space_axes(p))
+                             if p.dimensions > 1 else (np.abs(a) ** 4)[0:1])
+    if name == "abs2_int":
+        def f(a, p):
+            r = (np.conj(a) * a)[0:1]
+            for d in range(p.dimensions - 1, 0, -1):
+                r = np.sum(r, axis=d) * p.dx[d]
+            return r
+        return f
+    if name == "corr":
+        def f(a, p):
+            ax = _space_axes(p)
+            A = np.fft.fftn(a[0:1], axes=ax)
+            return np.fft.ifftn(np.abs(A) ** 2, axes=ax) / p.nspace
+        return f
+    raise ValueError("unknown observe form %r" % (name,))
+
+
+def apply_forms(p_in):
+    """Return a copy of the reference-style ``p`` with callables from ``p['gpu']``."""
+    p = dict(p_in)
+    g = p["gpu"]
+    p["deriv"] = make_deriv(g)
+    p["linear"] = make_linear(g)
+    p["initial"] = make_initial(g)
+    p["observe"] = [make_observe(o) for o in g.get("observe", ["a"])]
+    return p

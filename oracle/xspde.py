@@ -1,0 +1,635 @@
+"""FP64 restatement of xSPDE's ensemble integration path (test infrastructure only).
+
+Array convention: a field cell is ``a[field, x, (y,) traj]`` (the reference's
+``p.d.ca = [fields, space..., ens]``, xpreferences.m:145).  Noise and random
+cells are ``w[comp, x, (y,) traj]`` (xpreferences.m:153-158).  Only one field
+cell is supported (``p.fieldcells = 1``), which covers every example on the
+path.
+
+Functions and the reference code they restate:
+
+====================  ===========================================  =======================
+function              reference                                     pinned by
+====================  ===========================================  =======================
+xpreferences          Simulator/xpreferences.m:42-219               Batchtest rows (below)
+                      Applications/SPDE/spdpreferences.m:16-72
+                      Applications/SPDE/xgrid.m:18-48
+xpropfactor           Applications/SPDE/xpropfactor.m:140-175       linear-only exactness,
+                                                                    Batchtest rows
+xprop / xtransform    xprop.m:88-117, xtransform.m:196-245          transforms brute force,
+                                                                    Batchtest rows
+xsetbound             xsetbound.m:92-121 (homogeneous values)       Batchtest #21 (NN)
+RK4                   Applications/SDE/RK4.m:7-42                   Batchtest #4, #8, #14,
+                                                                    #16, #21
+MP                    Applications/SDE/MP.m:53-70                   order test (dt vs dt/2),
+                                                                    Kubo closed form
+xnoise / xrandom      Simulator/xnoise.m:133-145, xrandom.m:185-200 moments, KS test;
+                                                                    MATLAB stream itself:
+                                                                    parity unpinned
+xpath                 Simulator/xpath.m:16-86                       Batchtest rows, checks
+                                                                    pairing test
+xdata                 Simulator/xdata.m:129-192 (mean branch)       Batchtest rows
+xensemble             Simulator/xensemble.m:12-108                  Batchtest #21 (order
+                                                                    extrapolation)
+xsim statistics       Simulator/xsim.m:64-188                       Batchtest rows,
+                                                                    synthetic sigma test
+xcompress             Applications/SPDE/xcompress.m:18-69           Batchtest #16
+Ave / Int / D1        Applications/SPDE/Ave.m, Int.m, D1.m          Batchtest #14, #16, #21
+xnft                  Applications/SPDE/xnft.m                      Batchtest #16
+====================  ===========================================  =======================
+
+Built-in drift/initial/observe *forms* (the named forms the GPU path
+implements) are in ``oracle.forms``; they are user-level functions in the
+reference's terms (``p.deriv``, ``p.initial``, ``p.observe``).
+"""
+import math
+import time as _time
+
+import numpy as np
+
+from . import philox
+from .transforms import transform
+
+
+class P(dict):
+    """Parameter structure with attribute access (the reference's struct ``p``)."""
+
+    def __getattr__(self, k):
+        try:
+            return self[k]
+        except KeyError as e:
+            raise AttributeError(k) from e
+
+    def __setattr__(self, k, v):
+        self[k] = v
+
+    def copy(self):
+        return P(dict.copy(self))
+
+
+def _listify(x, n=None, default=None):
+    if x is None:
+        x = default
+    if callable(x) or not isinstance(x, (list, tuple)):
+        x = [x]
+    x = list(x)
+    if n is not None:
+        while len(x) < n:
+            x.append(x[-1] if x else default)
+    return x
+
+
+# ----------------------------------------------------------------- preferences
+def xpreferences(p_in):
+    """Defaults, lattice, scales and grids (xpreferences.m, spdpreferences.m, xgrid.m)."""
+    p = P(p_in)
+    nd = max(1, int(p.get("dimensions", 1)))
+    p.dimensions = nd
+    p.fields = int(p.get("fields", 1))
+    noises = p.get("noises", None)
+    p.noises = p.fields if noises is None else int(noises)
+    if p.get("inrandoms", None) is None and p.get("urandoms", None) is None:
+        p.inrandoms = p.noises                                          # xpreferences.m:74-76
+    else:
+        p.inrandoms = int(p.get("inrandoms", 0) or 0)
+    p.urandoms = int(p.get("urandoms", 0) or 0)
+    p.ensembles = list(p.get("ensembles", [1, 1, 1])) + [1] * 3
+    p.ensembles = [int(e) for e in p.ensembles[:3]]
+    p.ranges = np.asarray(_listify(p.get("ranges"), nd, 10.0), float)[:nd]
+    st = p.get("steps", None)
+    st = [1] * nd if st is None else _listify(st)
+    p.steps = np.asarray(st + [1] * (nd - len(st)), int)[:nd]
+    pts = p.get("points", None)
+    pts = [51] + [35] * (nd - 1) if pts is None else _listify(pts)
+    pts = list(pts) + [1] * (nd - len(pts))
+    pts = np.asarray(pts[:nd], int)
+    pts[1:] = 1 + (pts[1:] - 1) * p.steps[1:]                    # xpreferences.m:137
+    p.points = pts
+    og = p.get("origins", None)
+    og = [0.0] + list(-p.ranges[1:] / 2) if og is None else _listify(og)
+    p.origins = np.asarray(list(og) + list(-p.ranges[len(og):] / 2), float)[:nd]
+    ch = p.get("checks", None)
+    ch = [1] + [0] * (nd - 1) if ch is None else _listify(ch)
+    p.checks = np.asarray(list(ch) + [0] * (nd - len(ch)), int)[:nd]
+    if p.checks[1:].any():
+        raise NotImplementedError("spatial checks are outside the integration path")
+    p.iterations = int(p.get("iterations", 4))
+    p.order = int(p.get("order", 0))
+    p.seed = int(p.get("seed", 0))
+    p.relerr = int(p.get("relerr", 1))
+    p.tol = float(p.get("tol", 1e-20))
+    meth = p.get("method", None)
+    if meth is None:
+        meth = "RK4" if np.prod(p.ensembles) == 1 else "MP"            # xpreferences.m:101-105
+    p.method = meth if isinstance(meth, str) else meth.__name__
+    p.ipsteps = int(p.get("ipsteps", 2))                               # xpreferences.m:213-218
+    dx = p.ranges / np.maximum(1, pts - 1)
+    dx = np.where(pts == 1, 1.0, dx)
+    p.dx = dx
+    p.dv = float(np.prod(dx[1:]))                                      # xpreferences.m:140
+    p.noisefactor = 1.0 / p.dv
+    p.knoisefactor = float(np.prod(pts[1:])) / p.dv
+    p.dk = 2 * np.pi / (pts * dx)
+    p.dk[0] = p.dk[0] / p.steps[0]
+    p.kfact = dx / math.sqrt(2 * np.pi)
+    p.space = tuple(int(n) for n in pts[1:])
+    p.nspace = int(np.prod(pts[1:])) if nd > 1 else 1
+    p.dt = float(p.ranges[0] / max(1, pts[0] - 1))                     # xpreferences.m:174
+    p.nscale = p.steps[0] * (1 + p.checks[0]) / p.dt                   # xpreferences.m:175
+    p.nchk = 1 + int(p.checks.sum())
+    p.fbsw = 0
+    E = p.ensembles[0]
+    p.d_ca = (p.fields,) + p.space + (E,)
+    # spdpreferences.m:16-72 and xgrid.m:18-48
+    bnd = p.get("boundaries", None)
+    p.btypes = []
+    p.setboundaries = 0
+    sh = (1,) + p.space + (1,)
+    p.xs, p.ks = [], []
+    for d in range(1, nd):
+        n = int(pts[d])
+        p0 = np.arange(n)
+        xc = p.origins[d] + p0 * dx[d]
+        kc = (p0 - n // 2) * p.dk[d]
+        dkp = np.pi / ((n - 1) * dx[d]) if n > 1 else 0.0
+        b = [0, 0]
+        if bnd is not None and len(bnd) > d - 1 and bnd[d - 1] is not None:
+            b = list(bnd[d - 1])
+        t = int(b[0]) + 2 * int(b[1])
+        if t == -3:
+            kcp = p0 * dkp
+        elif t == -1:
+            kcp = np.concatenate([[0.0], np.arange(1, n) - 0.5]) * dkp
+        elif t == 0:
+            kcp = np.fft.ifftshift(kc)
+        elif t == 1:
+            kcp = np.concatenate([np.arange(1, n) - 0.5, [0.0]]) * dkp
+        elif t == 3:
+            kcp = p0 * dkp
+        else:
+            raise ValueError("Invalid boundary type: [%d,%d]" % (b[0], b[1]))
+        if t != 0:
+            p.setboundaries = 1
+        p.btypes.append((t, b))
+        shape = [1] * (nd + 1)
+        shape[d] = n
+        p.xs.append(np.broadcast_to(xc.reshape(shape), sh).copy())
+        p.ks.append(np.broadcast_to(kcp.reshape(shape), sh).copy())
+    names = ["x", "y", "z"]
+    for i, (xg, kg) in enumerate(zip(p.xs, p.ks)):
+        p[names[i]] = xg
+        p["k" + names[i]] = kg
+        p["D" + names[i]] = 1j * kg
+    p.D = [None] + [1j * kg for kg in p.ks]
+    p.xc = [p.origins[0] + p.dx[0] * np.arange(pts[0])] + \
+           [p.origins[d] + np.arange(pts[d]) * dx[d] for d in range(1, nd)]
+    p.kc = [None] + [(np.arange(pts[d]) - pts[d] // 2) * p.dk[d] for d in range(1, nd)]
+    p.deriv = p.get("deriv", None) or (lambda a, w, p: 0 * a)
+    p.linear = p.get("linear", None)
+    p.initial = p.get("initial", None) or (lambda v, p: 0)
+    obs = p.get("observe", None)
+    p.observe = _listify(obs) if obs is not None else [lambda a, p: a]
+    nobs = len(p.observe)
+    p.transforms = [np.asarray(_listify(t, nd, 0))[:nd] for t in
+                    _listify(p.get("transforms", None), nobs, 0)]
+    outs = p.get("output", None)
+    p.output = _listify(outs, nobs, None) if outs is not None else [None] * nobs
+    p.output = [o if o is not None else (lambda av, p, _n=n: av[_n])
+                for n, o in enumerate(p.output)]
+    p.compare = _listify(p.get("compare", None), len(p.output), None)
+    ax = p.get("axes", None)
+    p.axes = _listify(ax, len(p.output), None) if ax is not None else [None] * len(p.output)
+    p.cutoff = float(p.get("cutoff", -1e20))
+    p.t = float(p.origins[0])
+    p.dtr = 0.0
+    return p
+
+
+# ------------------------------------------------------------------ propagator
+def xpropfactor(p):
+    """propagator = exp(L * dtr / ipsteps) (xpropfactor.m:143-156)."""
+    if p.linear is None or (not callable(p.linear) and p.linear == 0):
+        return None
+    L = p.linear(p)
+    return np.exp(np.asarray(L) * (p.dtr / p.ipsteps))
+
+
+def xsetbound(a, p):
+    """Homogeneous boundary values: Dirichlet ends set to the default boundval 0
+    (xsetbound.m:100-116; xboundfun default zeros, spdpreferences.m:84-93)."""
+    if not p.setboundaries:
+        return a
+    a = a.copy()
+    for d, (t, b) in enumerate(p.btypes, start=1):
+        if b[0] == 1:
+            idx = [slice(None)] * a.ndim
+            idx[d] = 0
+            a[tuple(idx)] = 0
+        if b[1] == 1:
+            idx = [slice(None)] * a.ndim
+            idx[d] = -1
+            a[tuple(idx)] = 0
+    return a
+
+
+def xtransform(a, p, inverse):
+    """Per space dimension spectral transform of xtransform.m:208-245."""
+    for d, (t, _) in enumerate(p.btypes, start=1):
+        if a.shape[d] > 1:
+            a = transform(a, d, t, inverse)
+    return a
+
+
+def xprop(a, p, prop):
+    """One interaction-picture propagation (xprop.m:88-117), homogeneous patches."""
+    if prop is not None:
+        a = xtransform(np.asarray(a, complex), p, 0)
+        a = prop * a
+        a = xtransform(a, p, 1)
+    if p.setboundaries:
+        a = xsetbound(a, p)
+    return a
+
+
+# --------------------------------------------------------------------- methods
+def RK4(a, w, p, prop):
+    """xSPDE fourth-order interaction-picture Runge-Kutta (RK4.m:7-42)."""
+    dt = 0.5 * p.dtr
+    t = p.t
+    if p.setboundaries:
+        a = xsetbound(a, p)
+    a1 = a + p.deriv(a, w, p) * dt
+    a1 = xprop(a1, p, prop)
+    a = xprop(a, p, prop)
+    p.t = t + dt
+    a2 = a + p.deriv(a1, w, p) * dt
+    a2 = xsetbound(a2, p)
+    a3 = a + p.deriv(a2, w, p) * 2 * dt
+    a3 = xsetbound(a3, p)
+    a = (a1 + 2 * a2 + a3 - a) / 3
+    p.t = t + 2 * dt
+    a = xprop(a, p, prop)
+    a3 = xprop(a3, p, prop)
+    a = a + p.deriv(a3, w, p) * dt / 3
+    a = xsetbound(a, p)
+    p.t = t
+    return a
+
+
+def MP(a, w, p, prop):
+    """Interaction-picture midpoint with p.iterations fixed-point passes (MP.m:53-70)."""
+    dt = p.dtr / 2.0
+    t = p.t
+    a = xprop(a, p, prop)
+    p.t = t + dt
+    a0 = a
+    for _ in range(p.iterations):
+        a = a0 + p.deriv(a, w, p) * dt
+        a = xsetbound(a, p)
+    a = 2 * a - a0
+    p.t = t + 2 * dt
+    a = xprop(a, p, prop)
+    p.t = t
+    return a
+
+
+METHODS = {"RK4": RK4, "MP": MP}
+
+
+# ----------------------------------------------------------------------- noise
+def _site_major_to_cells(x, comps, space, E):
+    """[E, site*comps+comp] -> [comp, x, (y,), E] with site = ix + nx*iy."""
+    x = x.reshape((E,) + tuple(reversed(space)) + (comps,))
+    perm = (len(space) + 1,) + tuple(range(len(space), 0, -1)) + (0,)
+    return np.ascontiguousarray(np.transpose(x, perm))
+
+
+def noise_draw(p, nsp, q, seq=0, inst=0):
+    """One xnoise Gaussian draw (xnoise.m:137-139): N(0,1)*sqrt(nscale*noisefactor)."""
+    E = p.ensembles[0]
+    ns = max(1, p.nspace)
+    n = philox.normals(nsp, philox.TAG_NOISE, q, np.arange(E), ns * p.noises, seq, inst)
+    return _site_major_to_cells(n, p.noises, p.space, E) * math.sqrt(p.nscale * p.noisefactor)
+
+
+def xrandom(p, nsp, seq=0, inst=0):
+    """Initial randoms (xrandom.m:187-200): Gaussians of variance 1/dv, then uniforms."""
+    E = p.ensembles[0]
+    ns = max(1, p.nspace)
+    v = []
+    if p.inrandoms:
+        n = philox.normals(nsp, philox.TAG_GAUSS_INIT, 0, np.arange(E), ns * p.inrandoms, seq, inst)
+        v.append(_site_major_to_cells(n, p.inrandoms, p.space, E) * math.sqrt(p.noisefactor))
+    if p.urandoms:
+        u = philox.uniforms(nsp, philox.TAG_UNIFORM_INIT, 0, np.arange(E), ns * p.urandoms, seq, inst)
+        v.append(_site_major_to_cells(u, p.urandoms, p.space, E) * p.noisefactor)
+    if not v:
+        v = [np.zeros((1,) + p.space + (E,))]
+    return v
+
+
+# ------------------------------------------------------------------------ data
+def xnft(a, trans, p):
+    """k-space observable transform (xnft.m): fft along flagged dims times kfact."""
+    for d in range(1, p.dimensions):
+        if trans[d] > 0 and a.shape[d] > 1:
+            a = np.fft.fft(a, axis=d) * p.kfact[d]
+    return a
+
+
+def xdata(a, n, p):
+    """Ensemble mean of observable n (xdata.m:133-192, mean branch): [lines, nsp_o]."""
+    if np.any(p.transforms[n][1:]):
+        a = xnft(a, p.transforms[n], p)
+    o = np.asarray(p.observe[n](a, p))
+    E = a.shape[-1]
+    if o.ndim == 0 or o.shape[-1] != E:
+        o = np.broadcast_to(o, tuple(np.shape(o)) + (E,)) if o.ndim == 0 else \
+            np.broadcast_to(o[..., None], o.shape + (E,))
+    lines = o.shape[0] if o.ndim > 1 else 1
+    o = o.reshape(lines, -1, E)
+    return o.mean(axis=-1)
+
+
+# ------------------------------------------------------------------------ path
+def xpath(a, nc, p, prop, nsp, seq=0, inst=0, noise_fn=None):
+    """Time loop over points x steps x checks (xpath.m:29-86).
+
+    ``noise_fn(q)`` overrides the counter stream: it must return draw q.
+    Returns the final field and av[o] of shape [lines, points, nsp_o].
+    """
+    npts = int(p.points[0])
+    nsteps = int(p.steps[0])
+    chk = int(p.checks[0])
+    method = METHODS[p.method]
+    av = [None] * len(p.observe)
+    q = 0
+    draw = noise_fn or (lambda qq: noise_draw(p, nsp, qq, seq, inst))
+    for n in range(npts):
+        for step in range(nsteps):
+            if n > 0 or step > 0 or npts == 1:
+                w = None
+                for c in range(chk + 1):
+                    wn = draw(q)
+                    q += 1
+                    if w is not None and chk and nc == 2:
+                        wn = (wn + w) / 2                                 # xnoise.m:140-141
+                    w = wn
+                    if nc == 1 or c == 1:
+                        a = method(a, w, p, prop)
+                        w = None
+                        p.t = p.t + p.dtr
+            if step == 0:
+                for o in range(len(p.observe)):
+                    d = xdata(a, o, p)
+                    if av[o] is None:
+                        av[o] = np.zeros((d.shape[0], npts, d.shape[1]), dtype=complex)
+                    av[o][:, n, :] = d
+    return a, av
+
+
+def initial_field(p, v):
+    a = np.asarray(p.initial(*v, p), dtype=complex)
+    return np.broadcast_to(a, p.d_ca).copy() if a.shape != p.d_ca else a.copy()
+
+
+# -------------------------------------------------------------------- ensemble
+def run_path(p, j, nc, npar=1, seq=0, inst=0, noise_fn=None, a0=None):
+    """One (sub-ensemble j, check nc) call of p.path as xensemble.m:26-79 sets it up."""
+    p = p.copy()
+    nsp = p.seed + npar + (j - 1) * p.ensembles[2]                      # xensemble.m:26
+    if p.checks[0] and nc != 2:
+        p.dtr = p.dt / (2 * p.steps[0])                                  # xensemble.m:41-45
+    else:
+        p.dtr = p.dt / p.steps[0]
+    prop = xpropfactor(p)
+    v = xrandom(p, nsp, seq, inst)
+    p.t = float(p.origins[0])
+    a = initial_field(p, v) if a0 is None else np.array(a0, dtype=complex)
+    return xpath(a, nc, p, prop, nsp, seq, inst, noise_fn)
+
+
+def xensemble(npar, p, path_fn=None):
+    """Sub-ensemble and check loops with D, D^2 accumulation (xensemble.m:12-108).
+
+    ``path_fn(p, j, nc, npar)`` may replace the FP64 path (used to feed GPU
+    averages through the same bookkeeping).  Returns data[o] of shape
+    [ndata_o, m1].
+    """
+    e2, e3 = p.ensembles[1], p.ensembles[2]
+    ne = 1.0 / (e2 * e3) - 1
+    m = p.fbsw + p.nchk
+    m1 = max(3, m + 1)
+    data = [None] * len(p.output)
+    for j in range(1, e2 + 1):
+        D = {}
+        for nc in range(1, p.nchk + 1):
+            if path_fn is None:
+                _, av = run_path(p, j, nc, npar)
+            else:
+                av = path_fn(p, j, nc, npar)
+            for o, out in enumerate(p.output):
+                D[(nc, o)] = np.real(np.asarray(out(av, p))).reshape(-1)
+        for o in range(len(p.output)):
+            nes = 1 + ne
+            if p.checks[0] and p.order > 0:                              # xensemble.m:93-97
+                dl = (D[(1, o)] - D[(2, o)]) / (2.0 ** p.order - 1.0)
+                D[(2, o)] = D[(1, o)]
+                D[(1, o)] = D[(1, o)] + dl
+            if data[o] is None:
+                data[o] = np.zeros((D[(1, o)].size, m1))
+            for n1 in range(1, m + 1):
+                data[o][:, n1 - 1] += D[(n1, o)] * nes
+            data[o][:, m1 - 1] += D[(1, o)] ** 2 * nes
+    return data
+
+
+# ------------------------------------------------------------------ statistics
+def _grid_for_output(p, n, shape, kspace):
+    """Broadcast grids p.t, p.x, ... for compare functions (xgrid.m:49-56)."""
+    q = p.copy()
+    nd = p.dimensions
+    full = len(shape) - 2 == nd - 1
+    axes = [p.xc[0]]
+    for d in range(1, nd):
+        if kspace[d]:
+            axes.append(np.fft.fftshift(np.fft.fftfreq(p.points[d], 1.0 / p.points[d])) * p.dk[d])
+        else:
+            axes.append(p.xc[d])
+    grids = []
+    for i, ax in enumerate(axes):
+        sh = [1] * (nd + 1)
+        sh[i + 1] = len(ax)
+        grids.append(ax.reshape(sh))
+    q.t = grids[0]
+    names = ["x", "y", "z"]
+    for i in range(1, nd):
+        q[names[i - 1]] = grids[i]
+        q["k" + names[i - 1]] = grids[i]
+    q.r = grids
+    return q, full
+
+
+def xcompress_idx(axes_o, shape):
+    """Index lists per axis for xcompress.m:36-68 (0 all, -1 midpoint, -2 last)."""
+    idx = []
+    for d, npt in enumerate(shape[1:]):
+        ind = 0 if axes_o is None or d >= len(axes_o) else axes_o[d]
+        if np.isscalar(ind):
+            if ind == 0:
+                sel = np.arange(npt)
+            elif ind == -1:
+                sel = np.array([npt // 2])
+            elif ind == -2:
+                sel = np.array([npt - 1])
+            else:
+                sel = np.array([int(ind) - 1])
+        else:
+            sel = np.asarray(ind, int) - 1
+        idx.append(sel)
+    return idx
+
+
+def xsim_stats(p, data, elapsed=0.0):
+    """Final sampling/step/difference errors of xsim.m:64-188 (no bins/scatters).
+
+    ``data[o]`` is [ndata_o, m1] as returned by xensemble summed over npar.
+    Returns (error vector, reduced per-output data list).
+    """
+    ne = p.ensembles[1] * p.ensembles[2]
+    n1 = ne - 1
+    nc1 = p.nchk + p.fbsw
+    sx = max(3, 1 + nc1)
+    nout = len(p.output)
+    er = np.zeros((nout, max(6, nc1 + 4)))
+    em = np.zeros_like(er)
+    outd = []
+    for o in range(nout):
+        d = np.array(data[o], dtype=float)
+        shape = p._avshape[o]                                           # (lines, nt, *space_o)
+        kspace = p.transforms[o] if len(shape) - 2 == p.dimensions - 1 else np.zeros(p.dimensions)
+        dd = d.reshape(shape + (d.shape[1],))
+        for ax in range(1, p.dimensions):
+            if kspace[ax] and len(shape) - 2 == p.dimensions - 1:
+                dd = np.fft.fftshift(dd, axes=1 + ax)                  # xsim.m:77-81
+        idx = xcompress_idx(p.axes[o], shape)
+        dd = dd[np.ix_(*([np.arange(shape[0])] + idx + [np.arange(d.shape[1])]))]
+        cshape = dd.shape[:-1]
+        d = dd.reshape(-1, d.shape[1])
+        mx = np.max(np.abs(d[:, 0])) ** 2
+        if not (p.relerr and mx > p.tol):
+            mx = 1.0
+        if ne > 1:
+            d[:, sx - 1] = np.abs(np.sqrt((d[:, sx - 1] - d[:, 0] ** 2).astype(complex) / n1))
+        else:
+            d[:, sx - 1] = 0
+        for nc in range(2, nc1 + 1):
+            d[:, nc - 1] = np.abs(d[:, nc - 1] - d[:, 0])
+            er[o, nc - 1] = np.mean(d[:, nc - 1] ** 2)
+            em[o, nc - 1] = math.sqrt(np.max(d[:, nc - 1] ** 2))
+        esum = np.sum(d[:, 1:sx] ** 2, axis=1) + 1e-20
+        ept = (esum > p.tol) & (np.abs(d[:, 0]) > p.cutoff)
+        cmp = p.compare[o] if o < len(p.compare) else None
+        if cmp is not None:
+            q, full = _grid_for_output(p, o, shape, kspace)
+            c = np.asarray(cmp(q), dtype=float)
+            if full:
+                c = np.broadcast_to(c, shape)
+            else:
+                c = np.broadcast_to(c, (shape[0], shape[1]) + (1,) * (p.dimensions - 1))
+                c = c.reshape(shape[0], shape[1], *([1] * (len(shape) - 2)))
+                c = np.broadcast_to(c, shape)
+            c = c[np.ix_(*([np.arange(shape[0])] + idx))]
+            dif0 = c.reshape(-1)
+            ept = ept & (np.abs(dif0) > p.cutoff)
+            mx = np.max(np.abs(dif0)) ** 2
+            if not (p.relerr and mx > p.tol):
+                mx = 1.0
+            dif = (d[:, 0] - dif0) ** 2
+            em[o, sx] = math.sqrt(np.max(dif) / mx)
+            er[o, sx] = np.mean(dif) / mx
+            if ne > 1:
+                er[o, sx + 1] = np.sum(ept * (dif / esum))
+            er[o, sx + 2] = np.sum(ept)
+        er[o, 1:p.nchk] = er[o, 1:p.nchk] / mx
+        er[o, sx - 1] = np.mean(d[:, sx - 1] ** 2) / mx
+        em[o, sx - 1] = math.sqrt(np.max(d[:, sx - 1] ** 2) / mx)
+        seg = er[o, 1:sx + 1]
+        er[o, 0] = np.sum(seg) / (1e-99 + np.sum(seg > 0))
+        er[o, :sx + 1] = np.sqrt(er[o, :sx + 1])
+        outd.append(d.reshape(cshape + (d.shape[1],)))
+    error = np.zeros(max(6, sx + 3))
+    for c in range(2, sx + 2):
+        pts = 1e-99 + np.sum(er[:, c - 1] > p.tol)
+        error[c - 1] = math.sqrt(np.sum(er[:, c - 1] ** 2) / pts)
+    error[0] = math.sqrt(np.sum(error[1:sx + 1] ** 2) / (1e-99 + np.sum(error[1:sx + 1] > p.tol)))
+    error[sx + 1:sx + 3] = np.sum(er[:, sx + 1:sx + 3], axis=0)
+    error[sx + 1] = error[sx + 1] / (1e-20 + error[sx + 2])
+    error[sx + 2] = elapsed
+    return error, outd
+
+
+def xsim(p_in, path_fn=None):
+    """[error, data, p] = xsim(p) for one sequence element (xsim.m:28-188)."""
+    t0 = _time.time()
+    p = xpreferences(p_in)
+    data = None
+    for npar in range(1, p.ensembles[2] + 1):                         # parfor == serial, xsim.rst:275
+        dp = xensemble(npar, p, path_fn)
+        data = dp if data is None else [x + y for x, y in zip(data, dp)]
+    # shapes of the averages [lines, points, space_o] per output, from one probe
+    if "_avshape" not in p:
+        p._avshape = _probe_shapes(p)
+    err, outd = xsim_stats(p, data, _time.time() - t0)
+    return err, outd, p
+
+
+def _probe_shapes(p):
+    """Output shapes from observing a zero field (xpreferences.m:262-343)."""
+    q = p.copy()
+    q.ensembles = [1, 1, 1]
+    a = np.zeros((p.fields,) + p.space + (1,), complex)
+    av = []
+    for o in range(len(p.observe)):
+        d = xdata(a, o, q)
+        av.append(np.zeros((d.shape[0], int(p.points[0]), d.shape[1]), complex))
+    shapes = []
+    for out in p.output:
+        r = np.asarray(out(av, p))
+        lines, nt = r.shape[0], r.shape[1]
+        nsp = int(np.prod(r.shape[2:]))
+        if nsp == p.nspace and p.dimensions > 1:
+            shapes.append((lines, nt) + p.space)
+        else:
+            shapes.append((lines, nt))
+    return shapes
+
+
+# ------------------------------------------------------- observable helpers
+def Ave(o, p):
+    """Lattice average over space axes (Ave.m, default weights)."""
+    return np.mean(o, axis=tuple(range(1, p.dimensions)), keepdims=False)
+
+
+def Int(o, p):
+    """Lattice integral sum * dx over space axes (Int.m, no limits)."""
+    r = o
+    for d in range(p.dimensions - 1, 0, -1):
+        r = np.sum(r, axis=d) * p.dx[d]
+    return r
+
+
+def D1(a, d, p):
+    """Central first derivative along space axis d (D1.m), Neumann ends = boundval 0."""
+    ax = d - 1
+    r = (np.roll(a, -1, axis=ax) - np.roll(a, 1, axis=ax)) / (2 * p.dx[d - 1])
+    t, b = p.btypes[d - 2]
+    if b[0] == -1:
+        idx = [slice(None)] * a.ndim
+        idx[ax] = 0
+        r[tuple(idx)] = 0
+    if b[1] == -1:
+        idx = [slice(None)] * a.ndim
+        idx[ax] = -1
+        r[tuple(idx)] = 0
+    return r

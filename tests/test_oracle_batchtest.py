@@ -1,0 +1,86 @@
+"""Pins the oracle's RK4IP/MP path, propagators, checks, order extrapolation and
+xsim statistics on the deterministic (RNG-independent) rows of the reference's
+regression harness, Examples/Batchtest.m:67-75 at its +-0.1% tolerance (:81),
+and on exact heat-equation solutions per boundary type (Boundaries2.m:47-69)."""
+import numpy as np
+import pytest
+
+from oracle import xspde as X
+
+
+def sech(x):
+    return 1 / np.cosh(x)
+
+
+NLS = dict(deriv=lambda a, w, p: 1j * a * (np.conj(a) * a),
+           linear=lambda p: 0.5j * (p.Dx ** 2 - 1.0))
+
+
+def soliton():                      # Examples/SPDE_2D/Soliton.m:12-22, Batchtest #4
+    return dict(dimensions=2, checks=0, order=2, initial=lambda v, p: sech(p.x),
+                compare=[lambda p: sech(p.x)], **NLS)
+
+
+def characteristic():               # Examples/SPDE_2D/Characteristic.m:12-23, Batchtest #8
+    return dict(dimensions=2, initial=lambda w, p: sech(2 * (p.x + 2.5)),
+                linear=lambda p: -p.Dx, points=[51, 35],
+                observe=[lambda a, p: a, lambda a, p: a ** 2],
+                compare=[None, lambda p: sech(2 * (p.t - 2.5)) ** 2], axes=[None, [0, 18]])
+
+
+def solitonav():                    # Examples/SPDE_2D/SolitonAv.m:12-22, Batchtest #14
+    return dict(dimensions=2, initial=lambda v, p: sech(p.x),
+                observe=[lambda a, p: X.Ave(np.real(a), p), lambda a, p: X.Int(np.real(a), p)],
+                compare=[lambda p: np.pi / 10 + 0 * p.t, lambda p: np.pi + 0 * p.t], **NLS)
+
+
+def gaussianav2d():                 # Examples/SPDE_3D/GaussianAv2D.m:12-37, Batchtest #16
+    def g2(p):
+        sig = 1 + (p.t / 10) ** 2
+        return sig ** -1 * np.exp(-(p.x ** 2 + p.y ** 2) / sig)
+    ii = lambda a, p: a * np.conj(a)
+    return dict(dimensions=3, initial=lambda v, p: np.exp(-0.5 * (p.x ** 2 + p.y ** 2)),
+                deriv=lambda a, w, p: np.zeros(a.shape),
+                observe=[ii, lambda a, p: X.Ave(ii(a, p), p), lambda a, p: X.Int(ii(a, p), p), ii],
+                linear=lambda p: 1j * 0.05 * (p.Dx ** 2 + p.Dy ** 2),
+                transforms=[0, 0, 0, [0, 1, 1]], compare=[g2, None, None, lambda r: 1],
+                axes=[None, None, None, [0, -1, -1]])
+
+
+def solitonderivn():                # Examples/SPDE_2D/SolitonDerivN.m:8-23, Batchtest #21 (DCT-I)
+    return dict(dimensions=2, points=[101, 101], method="RK4", order=4, ranges=[10, 15],
+                initial=lambda v, p: sech(p.x),
+                observe=[lambda a, p: a * np.conj(a),
+                         lambda a, p: X.Int(np.abs(X.D1(a, 2, p)) ** 2, p)],
+                boundaries=[[-1, -1]], **NLS)
+
+
+ROWS = [(soliton, 5.047219e-03), (characteristic, 9.654854e-05), (solitonav, 1.410144e-02),
+        (gaussianav2d, 4.813746e-07), (solitonderivn, 2.134464e-06)]
+
+
+@pytest.mark.parametrize("make,expected", ROWS, ids=[r[0].__name__ for r in ROWS])
+def test_batchtest_known_answer(make, expected):
+    err = X.xsim(make())[0]
+    assert expected / 1.001 < err[0] < expected * 1.001
+
+
+HEAT = {  # boundary [lower, upper] -> (initial, exact) on x in [0, pi]  (Boundaries2.m:47-69)
+    "DD": ([1, 1], lambda x, t: 4 * np.sin(x) * np.exp(-t) + np.sin(2 * x) * np.exp(-4 * t)),
+    "NN": ([-1, -1], lambda x, t: 5 + 4 * np.cos(x) * np.exp(-t) + np.cos(2 * x) * np.exp(-4 * t)),
+    "DN": ([1, -1], lambda x, t: 4 * np.sin(x / 2) * np.exp(-t / 4) + np.sin(3 * x / 2) * np.exp(-9 * t / 4)),
+    "ND": ([-1, 1], lambda x, t: 4 * np.cos(x / 2) * np.exp(-t / 4) + np.cos(3 * x / 2) * np.exp(-9 * t / 4)),
+}
+
+
+@pytest.mark.parametrize("name", list(HEAT))
+@pytest.mark.parametrize("method", ["MP", "RK4"])
+def test_heat_equation_exact_per_boundary(name, method):
+    """Linear-only interaction picture is exact (tutorial.rst:120): the spectral
+    propagator of each boundary type must reproduce the heat equation's modes."""
+    b, exact = HEAT[name]
+    p = dict(dimensions=2, points=[11, 51], ranges=[4, np.pi], origins=[0, 0], method=method,
+             boundaries=[b], initial=lambda v, p: exact(p.x, 0.0), linear=lambda p: p.Dx ** 2,
+             compare=[lambda p: exact(p.x, p.t)])
+    err = X.xsim(p)[0]
+    assert err[0] < 1e-12
