@@ -1,0 +1,165 @@
+/*
+ * xspde_hip.h - C ABI of the MI355X (gfx950) ensemble integration path of
+ * xSPDE (peterddrummond/xspde_matlab, xSPDE 4.21).
+ *
+ * The library replaces, for a library of named built-in drift/linear/
+ * initial/observe forms, the reference's path plug-in slot
+ *
+ *     [a,av,raw] = p.path(a,nc,p)          Simulator/xpath.m:1,
+ *                                          called at Simulator/xensemble.m:79
+ *
+ * together with the pieces that slot calls on the hot loop:
+ *     p.method  RK4 / MP                   Applications/SDE/RK4.m:1, MP.m:1
+ *     p.prop    xprop -> xtransform        Applications/SPDE/xprop.m:1, xtransform.m:1
+ *     p.noisegen xnoise, p.randomgen       Simulator/xnoise.m:1, xrandom.m:1
+ *     xdata (ensemble means)               Simulator/xdata.m:1
+ * and offers a batched entry that runs all ensembles(2) x checks paths of
+ * one parallel ensemble (xensemble.m:25-80) in one launch.
+ *
+ * Conventions: plain pointers and sizes, no exceptions across the ABI; every
+ * entry returns an xsh_status and leaves a message for xsh_last_error()
+ * (thread-local), the analogue of the reference's error() aborts
+ * (e.g. xtransform.m:225-226, spdpreferences.m:67-68).  Host buffers are
+ * MATLAB column-major, interleaved complex double (mxGetComplexDoubles).
+ */
+#ifndef XSPDE_HIP_H
+#define XSPDE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XSH_ABI_VERSION 1
+
+typedef enum {
+  XSH_OK = 0,
+  XSH_EINVAL = 1,       /* invalid argument or descriptor field        */
+  XSH_EUNSUPPORTED = 2, /* valid xSPDE input, not a built-in GPU form  */
+  XSH_ENOMEM = 3,       /* device allocation refused                   */
+  XSH_EDEVICE = 4,      /* HIP runtime error / no device               */
+  XSH_ESTATE = 5        /* call out of order (e.g. nothing run yet)    */
+} xsh_status;
+
+enum { XSH_FP32 = 0, XSH_FP64 = 1 };
+enum { XSH_RK4 = 0, XSH_MP = 1 };
+enum { XSH_DERIV_ZERO = 0, XSH_DERIV_KUBO = 1, XSH_DERIV_GP = 2 };
+enum {
+  XSH_INIT_CONST = 0,
+  XSH_INIT_SECH = 1,
+  XSH_INIT_GAUSS = 2,
+  XSH_INIT_PHASE = 3,
+  XSH_INIT_COHERENT = 4
+};
+enum {
+  XSH_OBS_A = 0,        /* a(x)             per site   */
+  XSH_OBS_ABS2 = 1,     /* |a(x)|^2         per site   */
+  XSH_OBS_A_AVE = 2,    /* lattice mean a              */
+  XSH_OBS_ABS2_AVE = 3, /* lattice mean |a|^2          */
+  XSH_OBS_ABS2_INT = 4, /* sum |a|^2 dV                */
+  XSH_OBS_ABS4_AVE = 5, /* lattice mean |a|^4          */
+  XSH_OBS_CORR = 6      /* (1/Ns) sum_x a*(x) a(x+r), per r (periodic) */
+};
+#define XSH_MAX_OBS 8
+
+/* Problem descriptor: the subset of the reference's struct p that the path
+ * reads (xpreferences.m:42-219, spdpreferences.m:10-72).  Lattice sizes are
+ * the refined sizes points{1}(2:nd) = 1+(points-1)*steps(2:nd)
+ * (xpreferences.m:137). */
+typedef struct {
+  int struct_size;      /* sizeof(xsh_desc)                                  */
+  int precision;        /* XSH_FP32 | XSH_FP64 (field arithmetic)            */
+  int dimensions;       /* 1 = SDE, 2 = t+x, 3 = t+x+y  (p.dimensions)       */
+  int points;           /* p.points{1}(1): stored time points                */
+  int nx, ny;           /* lattice points in x, y (1 if absent)              */
+  int steps;            /* p.steps(1)                                        */
+  int checks;           /* p.checks(1): 0 or 1                               */
+  int method;           /* XSH_RK4 | XSH_MP  (p.method)                      */
+  int iterations;       /* p.iterations (MP)                                 */
+  int ensembles[3];     /* p.ensembles                                       */
+  int seed;             /* p.seed                                            */
+  int noises;           /* real noise components per site (p.noises)        */
+  int inrandoms;        /* Gaussian initial randoms per site (p.inrandoms)   */
+  int urandoms;         /* uniform initial randoms per site (p.urandoms)     */
+  int boundaries[2][2]; /* [dim][lower,upper]: 0 periodic, -1 Neumann, 1 Dirichlet */
+  double ranges[3];     /* p.ranges                                          */
+  double origins[3];    /* p.origins                                         */
+  int deriv;            /* XSH_DERIV_*                                       */
+  double deriv_par[6];  /* KUBO: c.  GP: g.re g.im u.re u.im s               */
+  double lin_l0[2];     /* L = l0 + sum_d l1[d] D_d + l2[d] D_d^2 (complex)  */
+  double lin_l1[2][2];
+  double lin_l2[2][2];
+  int has_linear;       /* 0: propagator is 1 (xpropfactor.m:144,154)        */
+  int initial;          /* XSH_INIT_*                                        */
+  double init_par[6];   /* CONST: c.re c.im; SECH: amp width eps; GAUSS: amp width;
+                           PHASE: amp; COHERENT: alpha.re alpha.im            */
+  int nobserve;
+  int observe[XSH_MAX_OBS];
+  int seq;              /* sequence index (random stream key)                */
+  int inst;             /* instance id (random stream counter)               */
+} xsh_desc;
+
+typedef struct xsh_handle xsh_handle;
+
+typedef struct {
+  int nsites;               /* lattice sites per trajectory                     */
+  int nchk;                 /* 1 + checks                                       */
+  long long avlen;          /* complex values of av per (j, nc) path            */
+  long long obs_offset[XSH_MAX_OBS]; /* complex offset of observable o in av     */
+  int obs_sites[XSH_MAX_OBS];        /* sites per time point of observable o      */
+  long long traj_steps;     /* trajectory-steps executed by xsh_run_ensemble    */
+  int threads, traj_per_block; /* launch shape of the path kernel              */
+} xsh_info;
+
+/* Library version (XSH_ABI_VERSION). */
+int xsh_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* xsh_last_error(void);
+
+/* Validate a descriptor, derive the lattice (xpreferences.m:133-175,
+ * spdpreferences.m:16-72), build propagator and twiddle tables
+ * (xpropfactor.m:143-175) and allocate device buffers.
+ * Replaces the per-(j,nc) setup of xensemble.m:41-53. */
+int xsh_create(const xsh_desc* desc, xsh_handle** out);
+
+/* Release device memory and the handle. */
+void xsh_destroy(xsh_handle* h);
+
+/* Sizes of the outputs of a handle. */
+int xsh_query(const xsh_handle* h, xsh_info* info);
+
+/* One path, the contract of p.path (xpath.m:1, called xensemble.m:79):
+ * integrates ensembles(1) trajectories of sub-ensemble j (1-based), check nc
+ * (1 fine, 2 coarse) of parallel ensemble npar (1-based), from a0
+ * ([1, nx, ny, ensembles(1)] complex, or NULL to use the built-in initial
+ * form, xensemble.m:53-60), and writes the final fields a_out (same shape,
+ * may be NULL) and the averages av_out ([avlen] complex, xpath.m:79-83). */
+int xsh_run_path(xsh_handle* h, int npar, int j, int nc, const double* a0, double* a_out,
+                 double* av_out);
+
+/* All ensembles(2) x (1+checks) paths of parallel ensemble npar in one
+ * launch (xensemble.m:25-80), asynchronously on the handle's stream.  The
+ * averages stay in device memory until xsh_read_averages. */
+int xsh_run_ensemble(xsh_handle* h, int npar);
+
+/* Wait for the handle's stream; returns the last kernel time in *ms if non-NULL. */
+int xsh_synchronize(xsh_handle* h, double* ms);
+
+/* Copy the averages of the last xsh_run_ensemble:
+ * [ensembles(2)][nchk][avlen] complex (interleaved doubles). */
+int xsh_read_averages(xsh_handle* h, double* av_out);
+
+/* Test hooks (no reference counterpart; used by the parity tests). */
+/* Standard normals of one draw: out[t][L], L < count, for trajectories
+ * traj0 .. traj0+ntraj-1, in the working precision of the handle. */
+int xsh_probe_normals(xsh_handle* h, int nsp, int tag, int draw, int traj0, int ntraj, int count,
+                      double* out);
+/* Apply the handle's interaction-picture propagator for check nc to nplanes
+ * planes ([nplanes][nx*ny] complex, in/out on the host):
+ * a -> T^-1 (P . T a) with the handle's boundary transforms. */
+int xsh_probe_propagate(xsh_handle* h, int nc, int nplanes, double* planes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XSPDE_HIP_H */

@@ -16,7 +16,7 @@ deriv    'zero'                        D = 0
          'gp'     g, u, s              D = g |a|^2 a + u a + s (w0 + i w1)   [noises >= 2]
                                        D = g |a|^2 a + u a + s w0            [noises == 1]
 linear   l0, l1[d], l2[d]              L = l0 + sum_d l1[d] D_d + l2[d] D_d^2  (xpropfactor.m:150-156)
-initial  'const'  c                    a = c
+initial  'const'  c0                   a = c0
          'sech'   amp, width, eps      a = amp sech(x/width) + eps (v0 + i v1)/sqrt2
          'gauss'  amp, width           a = amp exp(-(x^2+y^2)/(2 width^2))
          'phase'  amp                  a = amp exp(2 pi i u0),  u0 = uniform(0,1)
@@ -79,7 +79,7 @@ def make_linear(g):
 def make_initial(g):
     ini = g.get("initial", "const")
     if ini == "const":
-        c = complex(g.get("c", 1.0))
+        c = complex(g.get("c0", 1.0))
         return lambda *args: c + 0 * args[0]
     if ini == "sech":
         amp = float(g.get("amp", 1.0))

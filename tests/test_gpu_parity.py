@@ -1,0 +1,136 @@
+"""GPU parity of the HIP path against the FP64 oracle, through the C ABI.
+
+Same seeded counter stream on both sides, so trajectories are compared
+directly.  Tolerances: FP64 GPU vs FP64 oracle 1e-9 relative (transcendental
+rounding of the noise transform only); FP32 GPU vs FP64 oracle as stated per
+test (single-precision rounding, grows with the number of steps).
+"""
+import numpy as np
+import pytest
+
+from oracle import philox
+from oracle import xspde as X
+from xspde_matlab_amd import Problem
+from xspde_matlab_amd.xsim import preferences
+
+from _problems import cgl2d, kubo, nls1d, oracle_problem, shrink
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_path(p, j, nc, npar=1):
+    q = X.xpreferences(oracle_problem(p))
+    a, av = X.run_path(q, j, nc, npar)
+    return q, a, av
+
+
+def _cmp(gpu, ref, rtol, what):
+    gpu, ref = np.asarray(gpu), np.asarray(ref)
+    scale = max(np.abs(ref).max(), 1e-300)
+    err = np.abs(gpu - ref).max() / scale
+    print("%s: max |gpu-oracle|/max|oracle| = %.3e (tol %.1e)" % (what, err, rtol), flush=True)
+    assert err <= rtol, what
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
+def test_normals_match_oracle(prec, tol):
+    prob = Problem(shrink(kubo(prec), E=8, e2=1, points=3))
+    for tag, draw in [(philox.TAG_NOISE, 0), (philox.TAG_NOISE, 17), (philox.TAG_GAUSS_INIT, 0)]:
+        g = prob.handle.probe_normals(5, tag, draw, 3, 7, 37)
+        r = philox.normals(5, tag, draw, np.arange(3, 10), 37)
+        _cmp(g, r, tol, "normals %s tag %d draw %d" % (prec, tag, draw))
+    prob.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 3e-6)])
+@pytest.mark.parametrize("shape", [(32,), (64,), (128,), (256,), (512,), (1024,), (32, 32), (64, 64), (128, 128)])
+def test_propagator_matches_oracle(prec, tol, shape):
+    if prec == "fp64" and shape == (128, 128):
+        pytest.skip("fp64 128x128 planes exceed the LDS workspace of the fused kernel")
+    nd = len(shape) + 1
+    p = dict(dimensions=nd, points=[3] + list(shape), ranges=[1.0] + [10.0] * len(shape), ensembles=[2, 1, 1],
+             noises=0, gpu=dict(deriv="zero", linear=dict(l0=0.3 - 0.2j, l1=0.1 + 0.05j, l2=0.4 + 0.9j),
+                                initial="const", observe=["a"], precision=prec))
+    prob = Problem(p)
+    rng = np.random.default_rng(7)
+    planes = rng.standard_normal((3,) + shape[::-1]) + 1j * rng.standard_normal((3,) + shape[::-1])
+    got = prob.handle.probe_propagate(1, planes.reshape(3, -1)).reshape(planes.shape)
+    q = X.xpreferences(oracle_problem(p))
+    q.dtr = q.dt / 2
+    prop = X.xpropfactor(q)
+    a = np.moveaxis(planes, 0, -1)            # [y, x, plane] -> need [1, x, (y), plane]
+    a = np.transpose(a, tuple(range(len(shape) - 1, -1, -1)) + (len(shape),))[None]
+    ref = X.xprop(a, q, prop)[0]
+    ref = np.moveaxis(np.transpose(ref, tuple(range(len(shape) - 1, -1, -1)) + (len(shape),)), -1, 0)
+    _cmp(got, ref, tol, "propagate %s %s" % (prec, shape))
+    prob.close()
+
+
+def _path_case(p, tol, nc_list=(1, 2), j=2):
+    prob = Problem(p)
+    for nc in nc_list:
+        av_g, a_g = prob.xpath(nc, npar=1, j=j, want_final=True)
+        q, a_r, av_r = _oracle_path(p, j, nc)
+        # final fields: oracle [1, x, (y), E] -> [E, site] with site = ix + nx*iy
+        ar = np.moveaxis(a_r[0], -1, 0)
+        ar = ar.reshape(ar.shape[0], -1, order="F") if ar.ndim > 2 else ar.reshape(ar.shape[0], -1)
+        _cmp(a_g.reshape(ar.shape), ar, tol, "%s final field nc=%d" % (p["name"], nc))
+        for o, (g, r) in enumerate(zip(av_g, av_r)):
+            if r.shape[2] > 1 and len(q.space) == 2:
+                r = r.reshape(r.shape[0], r.shape[1], q.space[0], q.space[1]).reshape(
+                    r.shape[0], r.shape[1], -1, order="F")
+            _cmp(g, r, tol, "%s av[%d] nc=%d" % (p["name"], o, nc))
+    prob.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 2e-5)])
+@pytest.mark.parametrize("method", ["MP", "RK4"])
+def test_kubo_path(prec, tol, method):
+    p = shrink(kubo(prec), E=300, e2=2, points=11)
+    p["method"] = method
+    _path_case(p, tol)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 5e-5)])
+@pytest.mark.parametrize("nx", [64, 256])
+@pytest.mark.parametrize("method", ["RK4", "MP"])
+def test_nls1d_path(prec, tol, nx, method):
+    p = shrink(nls1d(prec, nx=nx), E=20, e2=2, points=4, steps=3)
+    p["method"] = method
+    _path_case(p, tol)
+
+
+@pytest.mark.parametrize("prec,tol,n", [("fp64", 1e-10, 32), ("fp64", 1e-10, 64), ("fp32", 5e-5, 32),
+                                        ("fp32", 5e-5, 64), ("fp32", 5e-5, 128)])
+@pytest.mark.parametrize("method", ["RK4", "MP"])
+def test_cgl2d_path(prec, tol, n, method):
+    p = shrink(cgl2d(prec, n=n), E=3, e2=2, points=3, steps=2)
+    p["method"] = method
+    _path_case(p, tol)
+
+
+def test_kubo_mean_within_error_bars():
+    """<Re a>(t) = exp(-t/2) (Kubo.m:19) within 4 sampling error bars, 4096 trajectories."""
+    from xspde_matlab_amd import xsim
+    p = kubo("fp64")
+    err, data, q = xsim(p)
+    t = np.linspace(0, 2, q["points"][0])
+    mean, step, sig = data[0][:, 0], data[0][:, 1], data[0][:, 2]
+    dev = np.abs(mean - np.exp(-t / 2))
+    print("kubo: max dev %.3e, max sigma %.3e, max step %.3e" % (dev.max(), sig.max(), step.max()))
+    assert np.all(dev <= 4 * sig + 2 * step + 1e-12)
+    assert np.all(np.abs(data[1][:, 0] - 1) <= 4 * data[1][:, 2] + 2 * data[1][:, 1] + 1e-6)
+
+
+def test_ensemble_matches_oracle_fp64():
+    """xsim on the GPU (batched launch of all j, nc) vs the oracle's xsim bookkeeping."""
+    from xspde_matlab_amd import xsim
+    p = shrink(nls1d("fp64", nx=64), E=8, e2=3, points=4, steps=2)
+    p["order"] = 1
+    err_g, data_g, q = xsim(p)
+    qo = X.xpreferences(oracle_problem(p))
+    dat = X.xensemble(1, qo)
+    for o in range(len(dat)):
+        ref = dat[o]
+        # oracle per-site data are C-ordered [points, x]; identical for 1D
+        _cmp(data_g[o][:, 0], ref[:, 0], 1e-10, "ensemble mean o=%d" % o)

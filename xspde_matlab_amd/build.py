@@ -1,0 +1,56 @@
+"""Build recipe of libxspde_hip.so (gfx950) - in-tree, no JIT cache.
+
+    python -m xspde_matlab_amd.build            # compile + link
+    python -m xspde_matlab_amd.build --resource # also print VGPR/LDS/scratch use
+
+Each translation unit is compiled by hipcc in parallel, then linked into
+``xspde_matlab_amd/libxspde_hip.so``.  Packed FP32 instructions are
+disabled (``-Xclang -target-feature -Xclang -packed-fp32-ops``).
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libxspde_hip.so")
+SOURCES = ["xsh_kernels.hip", "xsh_capi.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-munsafe-fp-atomics", "-Wno-unused-result"]
+
+
+def _compile(src, objdir, extra):
+    obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    out = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
+                    if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for %s:\n%s" % (src, out))
+    return obj, out
+
+
+def build(resource=False, verbose=False):
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if resource else []
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        results = list(ex.map(lambda s: _compile(s, objdir, extra), SOURCES))
+    if resource or verbose:
+        for _, out in results:
+            if out:
+                print(out)
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(resource="--resource" in sys.argv, verbose="-v" in sys.argv))

@@ -1,0 +1,50 @@
+"""The benchmark workloads as xSPDE project dictionaries (built-in GPU forms).
+
+Each is an xSPDE project dictionary (the reference's field names) with the
+built-in GPU forms in ``gpu``.  Defaults are the full workload sizes;
+``shrink`` makes smaller copies for tests.
+"""
+import math
+
+
+def kubo(precision="fp64", E=4096, e2=16, points=201):
+    # Kubo oscillator: da = i a xi dt, a(0)=1, MP, dt=0.01 to t=2 (Kubo.m:12-19)
+    return dict(name="Kubo", dimensions=1, points=[points], ranges=[2.0], steps=[1], checks=1,
+                noises=1, method="MP", ensembles=[E // e2, e2, 1],
+                gpu=dict(deriv="kubo", c=1.0, initial="const", c0=1.0, observe=["a", "abs2"],
+                         precision=precision))
+
+
+def nls1d(precision="fp32", E=4096, e2=16, points=101, nx=256, steps=10, gamma=0.1, eps=0.1):
+    # 1D stochastic NLS / GP: da = [i/2 d2a - i|a|^2 a - gamma a] dt + sqrt(gamma) dW
+    return dict(name="NLS1D", dimensions=2, points=[points, nx], ranges=[5.0, 20.0], steps=[steps, 1],
+                checks=1, noises=2, inrandoms=2, method="RK4", ensembles=[E // e2, e2, 1],
+                gpu=dict(deriv="gp", g=-1j, u=-gamma, s=math.sqrt(gamma / 2.0),
+                         linear=dict(l2=0.5j), initial="sech", amp=1.0, width=1.0, eps=eps,
+                         observe=["abs2", "a"], precision=precision))
+
+
+def cgl2d(precision="fp32", E=1024, e2=16, points=101, n=128, steps=20, length=20.0):
+    # 2D damped CGL: da = [(1+i) lap a + a - (1+0.5i)|a|^2 a] dt + 0.05 dW (complex)
+    return dict(name="CGL2D", dimensions=3, points=[points, n, n], ranges=[4.0, length, length],
+                steps=[steps, 1, 1], checks=1, noises=2, urandoms=1, method="RK4",
+                ensembles=[E // e2, e2, 1],
+                gpu=dict(deriv="gp", g=-(1 + 0.5j), u=0.0, s=0.05 / math.sqrt(2.0),
+                         linear=dict(l0=1.0, l2=1 + 1j), initial="phase", amp=0.1,
+                         observe=["abs2_ave", "corr"], precision=precision))
+
+
+def shrink(p, E=None, e2=None, points=None, steps=None):
+    p = dict(p)
+    p["gpu"] = dict(p["gpu"])
+    ens = list(p["ensembles"])
+    if E is not None:
+        ens[0] = E
+    if e2 is not None:
+        ens[1] = e2
+    p["ensembles"] = ens
+    if points is not None:
+        p["points"] = [points] + list(p["points"][1:])
+    if steps is not None:
+        p["steps"] = [steps] + list(p["steps"][1:])
+    return p

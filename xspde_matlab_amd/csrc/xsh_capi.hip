@@ -1,0 +1,442 @@
+// C ABI of the ensemble integration path (include/xspde_hip.h).
+//
+// Host side of the drop-in: descriptor validation, the lattice and scale
+// derivations of xpreferences.m:133-175 / spdpreferences.m:16-72, the
+// separable propagator tables of xpropfactor.m:143-156, and the launch of
+// whole-path kernels for one path (p.path, xensemble.m:79) or all
+// ensembles(2) x checks paths of one parallel ensemble (xensemble.m:25-80).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/xspde_hip.h"
+#include "xsh_launch.hpp"
+#include "xsh_path.hpp"
+
+using cd = std::complex<double>;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) return fail(XSH_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+}  // namespace
+
+struct xsh_handle {
+  xsh_desc d;
+  int nd, nx, ny, ns, E, e2, e3, nchk;
+  double dt, dtr[2], dx[2], dv, x0[2], nscale, noisefactor;
+  const xsh::KernelEntry* kern;
+  int groups, threads, tpb;
+  long long avlen;
+  long long obs_off[XSH_MAX_OBS];
+  int obs_ns[XSH_MAX_OBS];
+  size_t esize;
+  void* d_tables;   // px[2], py[2], twx, twy
+  size_t tab_off[6];
+  double* d_av;      // [e2*nchk][avlen] complex
+  double* d_path_av; // [avlen] complex (single path)
+  double* d_a;       // [E][ns] complex (single path in/out)
+  hipStream_t stream;
+  hipEvent_t ev0, ev1;
+  bool ran;
+};
+
+extern "C" int xsh_version(void) { return XSH_ABI_VERSION; }
+extern "C" const char* xsh_last_error(void) { return g_err.c_str(); }
+
+static int validate(const xsh_desc* d) {
+  if (!d) return fail(XSH_EINVAL, "descriptor is NULL");
+  if (d->struct_size != (int)sizeof(xsh_desc))
+    return fail(XSH_EINVAL, "descriptor size %d != %d (ABI mismatch)", d->struct_size, (int)sizeof(xsh_desc));
+  if (d->precision != XSH_FP32 && d->precision != XSH_FP64)
+    return fail(XSH_EINVAL, "precision must be XSH_FP32 or XSH_FP64");
+  if (d->dimensions < 1 || d->dimensions > 3) return fail(XSH_EUNSUPPORTED, "dimensions %d not on the GPU path (1..3)", d->dimensions);
+  if (d->points < 1) return fail(XSH_EINVAL, "points must be >= 1");
+  if (d->steps < 1) return fail(XSH_EINVAL, "steps must be >= 1");
+  if (d->checks != 0 && d->checks != 1) return fail(XSH_EINVAL, "checks(1) must be 0 or 1");
+  if (d->method != XSH_RK4 && d->method != XSH_MP) return fail(XSH_EUNSUPPORTED, "method %d is not RK4 or MP", d->method);
+  if (d->method == XSH_MP && d->iterations < 1) return fail(XSH_EINVAL, "iterations must be >= 1");
+  for (int i = 0; i < 3; ++i)
+    if (d->ensembles[i] < 1) return fail(XSH_EINVAL, "ensembles(%d) must be >= 1", i + 1);
+  if (d->noises < 0 || d->noises > 2) return fail(XSH_EUNSUPPORTED, "noises %d: built-in forms use 0..2", d->noises);
+  if (d->inrandoms < 0 || d->urandoms < 0) return fail(XSH_EINVAL, "negative random count");
+  if (d->deriv < XSH_DERIV_ZERO || d->deriv > XSH_DERIV_GP) return fail(XSH_EUNSUPPORTED, "unknown drift form %d", d->deriv);
+  if (d->deriv == XSH_DERIV_KUBO && d->noises < 1) return fail(XSH_EINVAL, "Kubo drift needs noises >= 1");
+  if (d->initial < XSH_INIT_CONST || d->initial > XSH_INIT_COHERENT) return fail(XSH_EUNSUPPORTED, "unknown initial form %d", d->initial);
+  if ((d->initial == XSH_INIT_COHERENT || (d->initial == XSH_INIT_SECH && d->init_par[2] != 0.0)) && d->inrandoms < 2)
+    return fail(XSH_EINVAL, "initial form needs inrandoms >= 2");
+  if (d->initial == XSH_INIT_PHASE && d->urandoms < 1) return fail(XSH_EINVAL, "phase initial form needs urandoms >= 1");
+  if (d->initial == XSH_INIT_SECH && !(d->init_par[1] > 0)) return fail(XSH_EINVAL, "sech width must be > 0");
+  if (d->initial == XSH_INIT_GAUSS && !(d->init_par[1] > 0)) return fail(XSH_EINVAL, "gauss width must be > 0");
+  if (d->nobserve < 1 || d->nobserve > XSH_MAX_OBS) return fail(XSH_EINVAL, "nobserve must be 1..%d", XSH_MAX_OBS);
+  for (int o = 0; o < d->nobserve; ++o)
+    if (d->observe[o] < XSH_OBS_A || d->observe[o] > XSH_OBS_CORR) return fail(XSH_EUNSUPPORTED, "unknown observe form %d", d->observe[o]);
+  const int sdim = d->dimensions - 1;
+  const int n[2] = {d->nx, d->ny};
+  for (int k = 0; k < 2; ++k) {
+    if (k < sdim) {
+      if (n[k] < 2) return fail(XSH_EINVAL, "lattice size %d must be >= 2", n[k]);
+      for (int b = 0; b < 2; ++b) {
+        const int t = d->boundaries[k][b];
+        if (t != 0 && t != 1 && t != -1) return fail(XSH_EINVAL, "Invalid boundary: [%d,%d]", d->boundaries[k][0], d->boundaries[k][1]);
+      }
+      if (d->boundaries[k][0] != 0 || d->boundaries[k][1] != 0)
+        return fail(XSH_EUNSUPPORTED, "non-periodic boundaries [%d,%d] are not on the GPU path yet",
+                    d->boundaries[k][0], d->boundaries[k][1]);
+      if (!is_pow2(n[k])) return fail(XSH_EUNSUPPORTED, "lattice size %d is not a power of two", n[k]);
+      if (!(d->ranges[k + 1] > 0)) return fail(XSH_EINVAL, "ranges(%d) must be > 0", k + 2);
+    } else if (n[k] != 1) {
+      return fail(XSH_EINVAL, "lattice size beyond dimensions must be 1");
+    }
+  }
+  if (!(d->ranges[0] > 0) && d->points > 1) return fail(XSH_EINVAL, "ranges(1) must be > 0");
+  return XSH_OK;
+}
+
+template <class R>
+static void pack(std::vector<char>& buf, size_t off, const std::vector<cd>& v) {
+  R* p = reinterpret_cast<R*>(buf.data() + off);
+  for (size_t i = 0; i < v.size(); ++i) {
+    p[2 * i] = (R)v[i].real();
+    p[2 * i + 1] = (R)v[i].imag();
+  }
+}
+
+extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
+  if (!out) return fail(XSH_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = validate(desc);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(XSH_EDEVICE, "no HIP device");
+  auto* h = new xsh_handle();
+  std::memset(h, 0, sizeof(*h));
+  h->d = *desc;
+  const xsh_desc& d = h->d;
+  h->nd = d.dimensions;
+  h->nx = d.dimensions > 1 ? d.nx : 1;
+  h->ny = d.dimensions > 2 ? d.ny : 1;
+  h->ns = h->nx * h->ny;
+  h->E = d.ensembles[0];
+  h->e2 = d.ensembles[1];
+  h->e3 = d.ensembles[2];
+  h->nchk = 1 + d.checks;
+  // xpreferences.m:138-141,174-175 ; xensemble.m:41-45
+  h->dt = d.points > 1 ? d.ranges[0] / (d.points - 1) : d.ranges[0];
+  h->dtr[0] = d.checks ? h->dt / (2.0 * d.steps) : h->dt / d.steps;
+  h->dtr[1] = h->dt / d.steps;
+  const int npt[2] = {h->nx, h->ny};
+  h->dv = 1.0;
+  for (int k = 0; k < 2; ++k) {
+    h->dx[k] = (k < d.dimensions - 1) ? d.ranges[k + 1] / (npt[k] - 1) : 1.0;
+    h->x0[k] = (k < d.dimensions - 1) ? d.origins[k + 1] : 0.0;
+    if (k < d.dimensions - 1) h->dv *= h->dx[k];
+  }
+  h->noisefactor = 1.0 / h->dv;
+  h->nscale = d.steps * (1 + d.checks) / h->dt;
+  h->esize = d.precision == XSH_FP64 ? 16 : 8;
+
+  if (d.dimensions > 1) {
+    h->kern = xsh::find_kernel(d.precision, h->nx, h->ny);
+    if (!h->kern) {
+      delete h;
+      return fail(XSH_EUNSUPPORTED, "no %s kernel for a %dx%d lattice", d.precision == XSH_FP64 ? "fp64" : "fp32",
+                  npt[0], npt[1]);
+    }
+    h->threads = h->kern->threads;
+    h->tpb = h->kern->tpb;
+  } else {
+    h->threads = 256;
+    h->tpb = 256;
+  }
+  h->groups = (h->E + h->tpb - 1) / h->tpb;
+
+  // averages layout: observable o -> points x sites_o complex values
+  long long off = 0;
+  for (int o = 0; o < d.nobserve; ++o) {
+    const int kind = d.observe[o];
+    const bool site = kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR;
+    h->obs_ns[o] = site ? h->ns : 1;
+    h->obs_off[o] = off;
+    off += (long long)d.points * h->obs_ns[o];
+  }
+  h->avlen = off;
+
+  // tables: per nc the separable propagator factors (xpropfactor.m:145-156):
+  //   P(kx,ky) = exp(dt l0) exp(dt (l1x i kx + l2x (i kx)^2)) exp(dt (l1y i ky + l2y (i ky)^2))
+  // with dt = dtr/ipsteps, ipsteps = 2, k in FFT order (spdpreferences.m:61-62),
+  // and the inverse-FFT 1/N folded in; then the twiddles W_N^m.
+  const cd l0(d.lin_l0[0], d.lin_l0[1]);
+  std::vector<std::vector<cd>> tabs(6);
+  for (int nci = 0; nci < 2; ++nci) {
+    const double dtp = h->dtr[nci] / 2.0;
+    for (int k = 0; k < 2; ++k) {
+      const int n = npt[k];
+      std::vector<cd> t(n);
+      const cd l1(d.lin_l1[k][0], d.lin_l1[k][1]), l2(d.lin_l2[k][0], d.lin_l2[k][1]);
+      const double dk = 2.0 * M_PI / (n * h->dx[k]);
+      for (int m = 0; m < n; ++m) {
+        const int f = m < (n + 1) / 2 ? m : m - n;  // ifftshift((-N/2..N/2-1)) order
+        const double kk = f * dk;
+        cd L = l1 * cd(0, kk) + l2 * cd(-kk * kk, 0);
+        if (k == 0) L += l0;
+        t[m] = (k < d.dimensions - 1) ? std::exp(L * dtp) / (double)n : std::exp(L * dtp);
+      }
+      tabs[2 * nci + k] = t;
+    }
+  }
+  for (int k = 0; k < 2; ++k) {
+    const int n = npt[k];
+    std::vector<cd> t(n);
+    for (int m = 0; m < n; ++m) t[m] = std::polar(1.0, -2.0 * M_PI * m / n);
+    tabs[4 + k] = t;
+  }
+  size_t total = 0;
+  for (int i = 0; i < 6; ++i) {
+    h->tab_off[i] = total;
+    total += ((tabs[i].size() * h->esize + 255) / 256) * 256;
+  }
+  std::vector<char> host(total);
+  for (int i = 0; i < 6; ++i) {
+    if (d.precision == XSH_FP64)
+      pack<double>(host, h->tab_off[i], tabs[i]);
+    else
+      pack<float>(host, h->tab_off[i], tabs[i]);
+  }
+  auto cleanup = [&](int code) {
+    xsh_destroy(h);
+    return code;
+  };
+  if (hipMalloc(&h->d_tables, total) != hipSuccess) return cleanup(fail(XSH_ENOMEM, "hipMalloc tables"));
+  if (hipMemcpy(h->d_tables, host.data(), total, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(XSH_EDEVICE, "copy tables"));
+  const size_t avbytes = (size_t)h->e2 * h->nchk * h->avlen * 16;
+  if (hipMalloc(&h->d_av, avbytes) != hipSuccess)
+    return cleanup(fail(XSH_ENOMEM, "hipMalloc averages (%zu bytes)", avbytes));
+  if (hipMalloc(&h->d_path_av, (size_t)h->avlen * 16) != hipSuccess) return cleanup(fail(XSH_ENOMEM, "hipMalloc path averages"));
+  if (hipMalloc(&h->d_a, (size_t)h->E * h->ns * 16) != hipSuccess) return cleanup(fail(XSH_ENOMEM, "hipMalloc fields"));
+  if (hipStreamCreate(&h->stream) != hipSuccess) return cleanup(fail(XSH_EDEVICE, "hipStreamCreate"));
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return cleanup(fail(XSH_EDEVICE, "hipEventCreate"));
+  *out = h;
+  return XSH_OK;
+}
+
+extern "C" void xsh_destroy(xsh_handle* h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->d_tables) hipFree(h->d_tables);
+  if (h->d_av) hipFree(h->d_av);
+  if (h->d_path_av) hipFree(h->d_path_av);
+  if (h->d_a) hipFree(h->d_a);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+extern "C" int xsh_query(const xsh_handle* h, xsh_info* info) {
+  if (!h || !info) return fail(XSH_EINVAL, "NULL argument");
+  std::memset(info, 0, sizeof(*info));
+  info->nsites = h->ns;
+  info->nchk = h->nchk;
+  info->avlen = h->avlen;
+  for (int o = 0; o < h->d.nobserve; ++o) {
+    info->obs_offset[o] = h->obs_off[o];
+    info->obs_sites[o] = h->obs_ns[o];
+  }
+  // xpath.m:29-34: points*steps - 1 steps per path; fine runs take (1+checks) per step
+  const long long per = (long long)h->d.points * h->d.steps - (h->d.points > 1 ? 1 : 0);
+  info->traj_steps = (long long)h->E * h->e2 * per * (h->d.checks ? 3 : 1);
+  info->threads = h->threads;
+  info->traj_per_block = h->tpb;
+  return XSH_OK;
+}
+
+static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
+  xsh::PathArgs A;
+  std::memset(&A, 0, sizeof(A));
+  const xsh_desc& d = h->d;
+  A.nx = h->nx;
+  A.ny = h->ny;
+  A.ns = h->ns;
+  A.x0 = h->x0[0];
+  A.dx = h->dx[0];
+  A.y0 = h->x0[1];
+  A.dy = h->dx[1];
+  A.dv = h->dv;
+  A.npts = d.points;
+  A.steps = d.steps;
+  A.checks = d.checks;
+  A.method = d.method;
+  A.iterations = d.iterations;
+  A.hdt[0] = h->dtr[0] / 2.0;
+  A.hdt[1] = h->dtr[1] / 2.0;
+  A.E = h->E;
+  A.e2 = h->e2;
+  A.e3 = h->e3;
+  A.npar = npar;
+  A.seed = d.seed;
+  A.seq = d.seq;
+  A.inst = d.inst;
+  A.groups = h->groups;
+  A.noises = d.noises;
+  A.inrandoms = d.inrandoms;
+  A.urandoms = d.urandoms;
+  A.noise_scale = std::sqrt(h->nscale * h->noisefactor);
+  A.init_scale = std::sqrt(h->noisefactor);
+  A.deriv = d.deriv;
+  for (int i = 0; i < 6; ++i) A.dp[i] = d.deriv_par[i];
+  A.initial = d.initial;
+  for (int i = 0; i < 6; ++i) A.ip[i] = d.init_par[i];
+  A.nobs = d.nobserve;
+  for (int o = 0; o < d.nobserve; ++o) {
+    A.obs[o] = d.observe[o];
+    A.obs_off[o] = h->obs_off[o];
+  }
+  A.avlen = h->avlen;
+  A.inv_E = 1.0 / h->E;
+  A.has_linear = d.has_linear;
+  char* t = static_cast<char*>(h->d_tables);
+  A.px[0] = t + h->tab_off[0];
+  A.py[0] = t + h->tab_off[1];
+  A.px[1] = t + h->tab_off[2];
+  A.py[1] = t + h->tab_off[3];
+  A.twx = t + h->tab_off[4];
+  A.twy = t + h->tab_off[5];
+  return A;
+}
+
+static int launch(xsh_handle* h, const xsh::PathArgs& A, int nblocks) {
+  hipError_t e;
+  if (h->nd == 1)
+    e = xsh::launch_sde_path(h->d.precision, A, nblocks, h->stream);
+  else
+    e = h->kern->launch(A, nblocks, h->stream);
+  if (e != hipSuccess) return fail(XSH_EDEVICE, "kernel launch: %s", hipGetErrorString(e));
+  return XSH_OK;
+}
+
+extern "C" int xsh_run_path(xsh_handle* h, int npar, int j, int nc, const double* a0, double* a_out,
+                            double* av_out) {
+  if (!h) return fail(XSH_EINVAL, "handle is NULL");
+  if (npar < 1 || npar > h->e3) return fail(XSH_EINVAL, "npar %d outside 1..%d", npar, h->e3);
+  if (j < 1 || j > h->e2) return fail(XSH_EINVAL, "j %d outside 1..%d", j, h->e2);
+  if (nc < 1 || nc > h->nchk) return fail(XSH_EINVAL, "nc %d outside 1..%d", nc, h->nchk);
+  if (!av_out) return fail(XSH_EINVAL, "av_out is NULL");
+  xsh::PathArgs A = make_args(h, npar);
+  A.j_only = j;
+  A.nc_only = nc;
+  A.av = h->d_path_av;
+  const size_t fbytes = (size_t)h->E * h->ns * 16;
+  HIPCHK(hipMemsetAsync(h->d_path_av, 0, (size_t)h->avlen * 16, h->stream));
+  if (a0) {
+    HIPCHK(hipMemcpyAsync(h->d_a, a0, fbytes, hipMemcpyHostToDevice, h->stream));
+    A.a0 = h->d_a;
+  }
+  A.a_out = a_out ? h->d_a : nullptr;
+  int rc = launch(h, A, h->groups);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(av_out, h->d_path_av, (size_t)h->avlen * 16, hipMemcpyDeviceToHost, h->stream));
+  if (a_out) HIPCHK(hipMemcpyAsync(a_out, h->d_a, fbytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return XSH_OK;
+}
+
+extern "C" int xsh_run_ensemble(xsh_handle* h, int npar) {
+  if (!h) return fail(XSH_EINVAL, "handle is NULL");
+  if (npar < 1 || npar > h->e3) return fail(XSH_EINVAL, "npar %d outside 1..%d", npar, h->e3);
+  xsh::PathArgs A = make_args(h, npar);
+  A.av = h->d_av;
+  HIPCHK(hipMemsetAsync(h->d_av, 0, (size_t)h->e2 * h->nchk * h->avlen * 16, h->stream));
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  int rc = launch(h, A, h->groups * h->e2 * h->nchk);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  h->ran = true;
+  return XSH_OK;
+}
+
+extern "C" int xsh_synchronize(xsh_handle* h, double* ms) {
+  if (!h) return fail(XSH_EINVAL, "handle is NULL");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (ms) {
+    float f = 0.f;
+    if (!h->ran) return fail(XSH_ESTATE, "nothing has run on this handle");
+    HIPCHK(hipEventElapsedTime(&f, h->ev0, h->ev1));
+    *ms = f;
+  }
+  return XSH_OK;
+}
+
+extern "C" int xsh_read_averages(xsh_handle* h, double* av_out) {
+  if (!h || !av_out) return fail(XSH_EINVAL, "NULL argument");
+  if (!h->ran) return fail(XSH_ESTATE, "xsh_run_ensemble has not run");
+  HIPCHK(hipMemcpyAsync(av_out, h->d_av, (size_t)h->e2 * h->nchk * h->avlen * 16, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return XSH_OK;
+}
+
+extern "C" int xsh_probe_normals(xsh_handle* h, int nsp, int tag, int draw, int traj0, int ntraj, int count,
+                                 double* out) {
+  if (!h || !out) return fail(XSH_EINVAL, "NULL argument");
+  if (ntraj < 1 || count < 1 || traj0 < 0) return fail(XSH_EINVAL, "bad probe sizes");
+  double* d = nullptr;
+  const size_t bytes = (size_t)ntraj * count * sizeof(double);
+  HIPCHK(hipMalloc(&d, bytes));
+  hipError_t e = xsh::launch_probe_normals(h->d.precision, (uint32_t)nsp, (uint32_t)tag, (uint32_t)h->d.seq,
+                                           (uint32_t)h->d.inst, (uint32_t)draw, traj0, ntraj, count, d, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(d);
+  if (e != hipSuccess) return fail(XSH_EDEVICE, "probe normals: %s", hipGetErrorString(e));
+  return XSH_OK;
+}
+
+extern "C" int xsh_probe_propagate(xsh_handle* h, int nc, int nplanes, double* planes) {
+  if (!h || !planes) return fail(XSH_EINVAL, "NULL argument");
+  if (h->nd < 2) return fail(XSH_EINVAL, "no lattice to propagate (dimensions = 1)");
+  if (nc < 1 || nc > 2 || nplanes < 1) return fail(XSH_EINVAL, "bad probe arguments");
+  xsh::PathArgs A = make_args(h, 1);
+  const size_t n = (size_t)nplanes * h->ns;
+  std::vector<char> buf(n * h->esize);
+  if (h->d.precision == XSH_FP64)
+    std::memcpy(buf.data(), planes, n * 16);
+  else
+    for (size_t i = 0; i < 2 * n; ++i) reinterpret_cast<float*>(buf.data())[i] = (float)planes[i];
+  void* d = nullptr;
+  HIPCHK(hipMalloc(&d, buf.size()));
+  hipError_t e = hipMemcpyAsync(d, buf.data(), buf.size(), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = h->kern->probe(A, nc - 1, nplanes, d, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(buf.data(), d, buf.size(), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(d);
+  if (e != hipSuccess) return fail(XSH_EDEVICE, "probe propagate: %s", hipGetErrorString(e));
+  if (h->d.precision == XSH_FP64)
+    std::memcpy(planes, buf.data(), n * 16);
+  else
+    for (size_t i = 0; i < 2 * n; ++i) planes[i] = reinterpret_cast<float*>(buf.data())[i];
+  return XSH_OK;
+}

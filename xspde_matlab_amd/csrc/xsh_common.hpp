@@ -1,0 +1,172 @@
+// Device building blocks of the xSPDE ensemble path on gfx950:
+// complex arithmetic, the Philox4x32-10 counter stream with Box-Muller
+// normals, and fully unrolled in-register DFTs of length <= 64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xsh {
+
+template <class R>
+struct cpx {
+  R x, y;
+};
+
+template <class R>
+__device__ __forceinline__ cpx<R> mk(R a, R b) { return cpx<R>{a, b}; }
+template <class R>
+__device__ __forceinline__ cpx<R> operator+(cpx<R> a, cpx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <class R>
+__device__ __forceinline__ cpx<R> operator-(cpx<R> a, cpx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <class R>
+__device__ __forceinline__ cpx<R> operator*(cpx<R> a, cpx<R> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <class R>
+__device__ __forceinline__ cpx<R> operator*(R s, cpx<R> a) { return {s * a.x, s * a.y}; }
+template <class R>
+__device__ __forceinline__ cpx<R> conjc(cpx<R> a) { return {a.x, -a.y}; }
+template <class R>
+__device__ __forceinline__ R norm2(cpx<R> a) { return a.x * a.x + a.y * a.y; }
+// multiply by -i (SIGN=-1) or +i (SIGN=+1)
+template <int SIGN, class R>
+__device__ __forceinline__ cpx<R> mul_i(cpx<R> a) {
+  return SIGN > 0 ? cpx<R>{-a.y, a.x} : cpx<R>{a.y, -a.x};
+}
+
+// cos/sin(2 pi m / 64): compile-time twiddles of the register DFTs
+#include "xsh_twiddle64.inc"
+
+// ------------------------------------------------------------------ DFT
+// In-register DFT of length N (power of two, N <= 64), natural order in and
+// out.  SIGN = -1 forward (exp(-2 pi i nk/N)), +1 inverse (no 1/N).
+template <int N>
+struct bitrev {
+  static constexpr int bits = N <= 1 ? 0 : 1 + bitrev<N / 2>::bits;
+  __device__ static constexpr int of(int i) {
+    int r = 0;
+    for (int b = 0; b < bits; ++b)
+      if (i & (1 << b)) r |= 1 << (bits - 1 - b);
+    return r;
+  }
+};
+template <>
+struct bitrev<1> {
+  static constexpr int bits = 0;
+  __device__ static constexpr int of(int) { return 0; }
+};
+
+template <class R, int N, int SIGN>
+__device__ __forceinline__ void dft(cpx<R> (&v)[N]) {
+  if constexpr (N == 1) {
+    return;
+  } else if constexpr (N == 2) {
+    cpx<R> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  } else if constexpr (N == 4) {
+    cpx<R> a0 = v[0] + v[2], a1 = v[0] - v[2];
+    cpx<R> b0 = v[1] + v[3], b1 = mul_i<SIGN>(v[1] - v[3]);
+    v[0] = a0 + b0;
+    v[2] = a0 - b0;
+    v[1] = a1 + b1;
+    v[3] = a1 - b1;
+  } else {
+    // radix-2 decimation in time over the bit-reversed order, radix-4 leaves
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      constexpr int dummy = 0;
+      (void)dummy;
+      const int j = bitrev<N>::of(i);
+      if (j > i) {
+        cpx<R> t = v[i];
+        v[i] = v[j];
+        v[j] = t;
+      }
+    }
+#pragma unroll
+    for (int len = 2; len <= N; len <<= 1) {
+      const int half = len >> 1;
+#pragma unroll
+      for (int i = 0; i < N; i += len) {
+#pragma unroll
+        for (int k = 0; k < half; ++k) {
+          cpx<R> a = v[i + k];
+          cpx<R> b = v[i + k + half];
+          if (k == 0) {
+          } else if (4 * k == len) {
+            b = mul_i<SIGN>(b);
+          } else {
+            const int m = k * (64 / len);
+            const cpx<R> w = {R(kCos64[m]), R(SIGN) * R(kSin64[m])};
+            b = b * w;
+          }
+          v[i + k] = a + b;
+          v[i + k + half] = a - b;
+        }
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Stream layout:
+//   key     = (nsp, tag | seq << 8)
+//   counter = (group, draw, traj, inst),  group = L / 4, lane = L % 4
+// where L = site * comps + comp.
+struct u4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u4 philox10(u4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c.x;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c.z;
+    c = u4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1,
+           (uint32_t)p0};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+enum : uint32_t { TAG_NOISE = 0, TAG_GAUSS_INIT = 1, TAG_UNIFORM_INIT = 2 };
+
+__device__ __forceinline__ uint32_t key1(uint32_t tag, uint32_t seq) {
+  return (tag & 0xFFu) | (seq << 8);
+}
+
+// Box-Muller on (x0,x1): u = (x + 1/2) 2^-32 in (0,1)
+__device__ __forceinline__ void box_muller(uint32_t x0, uint32_t x1, float& n0, float& n1) {
+  const float u0 = (float)x0 * 2.3283064365386963e-10f + 1.1641532182693481e-10f;
+  const float u1 = (float)x1 * 2.3283064365386963e-10f + 1.1641532182693481e-10f;
+  const float r = __fsqrt_rn(-2.0f * __logf(u0));
+  float s, c;
+  __sincosf(6.2831853071795865f * u1, &s, &c);
+  n0 = r * c;
+  n1 = r * s;
+}
+__device__ __forceinline__ void box_muller(uint32_t x0, uint32_t x1, double& n0, double& n1) {
+  const double u0 = ((double)x0 + 0.5) * 2.3283064365386963e-10;
+  const double u1 = ((double)x1 + 0.5) * 2.3283064365386963e-10;
+  const double r = sqrt(-2.0 * log(u0));
+  double s, c;
+  sincos(6.283185307179586476925 * u1, &s, &c);
+  n0 = r * c;
+  n1 = r * s;
+}
+
+template <class R>
+__device__ __forceinline__ void normals4(u4 c, uint32_t k0, uint32_t k1, R (&n)[4]) {
+  const u4 x = philox10(c, k0, k1);
+  box_muller(x.x, x.y, n[0], n[1]);
+  box_muller(x.z, x.w, n[2], n[3]);
+}
+
+template <class R>
+__device__ __forceinline__ R uniform01(uint32_t x) {
+  return (R)(((double)x + 0.5) * 2.3283064365386963e-10);
+}
+
+}  // namespace xsh

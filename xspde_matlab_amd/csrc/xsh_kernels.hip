@@ -1,0 +1,496 @@
+// Path kernels and their launch table.
+//
+//  sde_path    dimensions = 1 (no space): one trajectory per lane, the whole
+//              path in registers (xpath.m:29-86 with RK4.m / MP.m on a scalar
+//              field; xprop.m:104-106 reduces to a scalar factor).
+//  spde_path   dimensions = 2, 3 on periodic lattices: one workgroup per TPB
+//              trajectories, fields resident in VGPRs for the whole path.
+//  probes      normals and propagation of given planes (parity hooks).
+#include "xsh_path.hpp"
+#include "xsh_launch.hpp"
+
+namespace xsh {
+
+template <class R>
+__device__ __forceinline__ void path_identity(const PathArgs& A, int lb, int& pidx, int& nci, int& j,
+                                              int& group) {
+  group = lb % A.groups;
+  pidx = lb / A.groups;
+  if (A.j_only > 0) {
+    j = A.j_only;
+    nci = A.nc_only - 1;
+    pidx = 0;
+  } else {
+    nci = pidx / A.e2;
+    j = pidx % A.e2 + 1;
+  }
+}
+
+// noise of one draw for VEC=2 consecutive sites s, s+1 (s even):
+// L = site*noises + comp, group L/4, lane L%4
+template <class R>
+__device__ __forceinline__ void draw_pair(const PathArgs& A, uint32_t nsp, uint32_t q, uint32_t traj,
+                                          int s, cpx<R>& w0, cpx<R>& w1) {
+  const R sc = (R)A.noise_scale;
+  const uint32_t k1 = key1(TAG_NOISE, (uint32_t)A.seq);
+  if (A.noises >= 2) {
+    R n[4];
+    normals4<R>(u4{(uint32_t)(s * A.noises) >> 2, q, traj, (uint32_t)A.inst}, nsp, k1, n);
+    w0 = cpx<R>{sc * n[0], sc * n[1]};
+    w1 = cpx<R>{sc * n[2], sc * n[3]};
+  } else if (A.noises == 1) {
+    const u4 x = philox10(u4{(uint32_t)s >> 2, q, traj, (uint32_t)A.inst}, nsp, k1);
+    R a, b;
+    if ((s & 3) == 0)
+      box_muller(x.x, x.y, a, b);
+    else
+      box_muller(x.z, x.w, a, b);
+    w0 = cpx<R>{sc * a, R(0)};
+    w1 = cpx<R>{sc * b, R(0)};
+  } else {
+    w0 = w1 = cpx<R>{R(0), R(0)};
+  }
+}
+
+// one normal / uniform of the initial randoms (xrandom.m:187-200)
+template <class R>
+__device__ __forceinline__ R init_normal(const PathArgs& A, uint32_t nsp, uint32_t traj, int L) {
+  R n[4];
+  normals4<R>(u4{(uint32_t)L >> 2, 0u, traj, (uint32_t)A.inst}, nsp,
+              key1(TAG_GAUSS_INIT, (uint32_t)A.seq), n);
+  return n[L & 3];
+}
+template <class R>
+__device__ __forceinline__ R init_uniform(const PathArgs& A, uint32_t nsp, uint32_t traj, int L) {
+  const u4 x = philox10(u4{(uint32_t)L >> 2, 0u, traj, (uint32_t)A.inst}, nsp,
+                        key1(TAG_UNIFORM_INIT, (uint32_t)A.seq));
+  const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
+  return uniform01<R>(xs[L & 3]);
+}
+
+// built-in initial forms (user-level p.initial handles, xensemble.m:60)
+template <class R>
+__device__ __forceinline__ cpx<R> initial_value(const PathArgs& A, uint32_t nsp, uint32_t traj, int site,
+                                                int ix, int iy) {
+  const double x = A.x0 + ix * A.dx, y = A.y0 + iy * A.dy;
+  const R isc = (R)A.init_scale;
+  switch (A.initial) {
+    case XSH_INIT_SECH: {
+      cpx<R> a = {(R)(A.ip[0] / cosh(x / A.ip[1])), R(0)};
+      if (A.ip[2] != 0.0) {
+        const R e = (R)(A.ip[2] * 0.70710678118654752440);
+        const R v0 = isc * init_normal<R>(A, nsp, traj, site * A.inrandoms);
+        const R v1 = isc * init_normal<R>(A, nsp, traj, site * A.inrandoms + 1);
+        a.x += e * v0;
+        a.y += e * v1;
+      }
+      return a;
+    }
+    case XSH_INIT_GAUSS: {
+      double r2 = x * x + (A.ny > 1 ? y * y : 0.0);
+      return cpx<R>{(R)(A.ip[0] * exp(-0.5 * r2 / (A.ip[1] * A.ip[1]))), R(0)};
+    }
+    case XSH_INIT_PHASE: {
+      const R u = init_uniform<R>(A, nsp, traj, site * A.urandoms);
+      R s, c;
+      if constexpr (sizeof(R) == 4)
+        __sincosf(6.2831853071795865f * u, &s, &c);
+      else
+        sincos(6.283185307179586476925 * u, &s, &c);
+      return cpx<R>{(R)A.ip[0] * c, (R)A.ip[0] * s};
+    }
+    case XSH_INIT_COHERENT: {
+      const R v0 = isc * init_normal<R>(A, nsp, traj, site * A.inrandoms);
+      const R v1 = isc * init_normal<R>(A, nsp, traj, site * A.inrandoms + 1);
+      return cpx<R>{(R)A.ip[0] + R(0.5) * v0, (R)A.ip[1] + R(0.5) * v1};
+    }
+    default:
+      return cpx<R>{(R)A.ip[0], (R)A.ip[1]};
+  }
+}
+
+// ===================================================================== SDE
+template <class R>
+__global__ __launch_bounds__(256) void sde_path(const PathArgs A) {
+  using C = cpx<R>;
+  __shared__ double red[8];
+  int pidx, nci, j, group;
+  path_identity<R>(A, blockIdx.x, pidx, nci, j, group);
+  const int traj = group * 256 + threadIdx.x;
+  const bool active = traj < A.E;
+  const uint32_t nsp = (uint32_t)(A.seed + A.npar + (j - 1) * A.e3);
+  const R hdt = (R)A.hdt[nci];
+  Drift<R> D;
+  D.init(A);
+  const C P = A.has_linear ? static_cast<const C*>(A.px[nci])[0] : C{R(1), R(0)};
+  double* av = A.av + 2 * (long long)pidx * A.avlen;
+
+  C a;
+  if (A.a0) {
+    const long long g = active ? traj : 0;
+    a = C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]};
+  } else
+    a = initial_value<R>(A, nsp, (uint32_t)traj, 0, 0, 0);
+
+  auto draw = [&](uint32_t q) -> C {
+    const R sc = (R)A.noise_scale;
+    if (A.noises <= 0) return C{R(0), R(0)};
+    R n[4];
+    normals4<R>(u4{0u, q, (uint32_t)traj, (uint32_t)A.inst}, nsp, key1(TAG_NOISE, (uint32_t)A.seq), n);
+    return C{sc * n[0], A.noises >= 2 ? sc * n[1] : R(0)};
+  };
+  auto step = [&](C xi) {
+    if (A.method == XSH_RK4) {
+      C a1 = a + hdt * D(a, xi);
+      a1 = P * a1;
+      a = P * a;
+      C a2 = a + hdt * D(a1, xi);
+      C a3 = a + (R(2) * hdt) * D(a2, xi);
+      a = R(1.0 / 3.0) * (a1 + R(2) * a2 + a3 - a);
+      a = P * a;
+      a3 = P * a3;
+      a = a + (hdt / R(3)) * D(a3, xi);
+    } else {
+      a = P * a;
+      const C a0 = a;
+      for (int it = 0; it < A.iterations; ++it) a = a0 + hdt * D(a, xi);
+      a = R(2) * a - a0;
+      a = P * a;
+    }
+  };
+  auto observe = [&](int n) {
+    for (int o = 0; o < A.nobs; ++o) {
+      double re = 0.0, im = 0.0;
+      if (active) {
+        switch (A.obs[o]) {
+          case XSH_OBS_A:
+          case XSH_OBS_A_AVE:
+            re = (double)a.x;
+            im = (double)a.y;
+            break;
+          case XSH_OBS_ABS4_AVE: {
+            const double m = (double)norm2(a);
+            re = m * m;
+            break;
+          }
+          default:
+            re = (double)norm2(a);
+        }
+      }
+      block_sum2<256>(re, im, red);
+      if (threadIdx.x == 0) atomic_add_c(av + 2 * (A.obs_off[o] + n), re * A.inv_E, im * A.inv_E);
+    }
+  };
+
+  uint32_t q = 0;
+  for (int n = 0; n < A.npts; ++n) {
+    for (int st = 0; st < A.steps; ++st) {
+      if (n > 0 || st > 0 || A.npts == 1) {
+        if (A.checks && nci == 1) {
+          const C w0 = draw(q), w1 = draw(q + 1);
+          q += 2;
+          step(R(0.5) * (w0 + w1));
+        } else {
+          for (int c = 0; c <= A.checks; ++c) step(draw(q++));
+        }
+      }
+      if (st == 0) observe(n);
+    }
+  }
+  if (A.a_out && active) {
+    A.a_out[2 * (long long)traj] = (double)a.x;
+    A.a_out[2 * (long long)traj + 1] = (double)a.y;
+  }
+}
+
+// ==================================================================== SPDE
+template <class R, int NX, int NY, int NT, int TPB>
+__global__ __launch_bounds__(NT) void spde_path(const PathArgs A) {
+  using Lat = Lattice<R, NX, NY, NT, TPB>;
+  using C = cpx<R>;
+  constexpr int CPT = Lat::CPT, PPT = Lat::PPT, NS = Lat::NS;
+  __shared__ C L[Lat::LDS];
+  __shared__ double red[2 * (NT / 64)];
+
+  int pidx, nci, j, group;
+  path_identity<R>(A, blockIdx.x, pidx, nci, j, group);
+  const uint32_t nsp = (uint32_t)(A.seed + A.npar + (j - 1) * A.e3);
+  const R hdt = (R)A.hdt[nci];
+  Drift<R> D;
+  D.init(A);
+  const PropOp<R> prop{static_cast<const C*>(A.px[nci]), NY > 1 ? static_cast<const C*>(A.py[nci]) : nullptr};
+  const C* twx = static_cast<const C*>(A.twx);
+  const C* twy = static_cast<const C*>(A.twy);
+  double* av = A.av + 2 * (long long)pidx * A.avlen;
+
+  C f0[PPT], f1[PPT], xi[PPT];
+  // ---- initial fields (xensemble.m:53-65)
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int pt = Lat::point(i, 0);
+    const int p = pt / NS, s = pt % NS;
+    const int traj = group * TPB + p;
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int site = s + v, iy = site / NX, ix = site % NX;
+      if (A.a0) {
+        const long long g = (long long)(traj < A.E ? traj : 0) * NS + site;
+        f0[2 * i + v] = C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]};
+      } else {
+        f0[2 * i + v] = initial_value<R>(A, nsp, (uint32_t)traj, site, ix, iy);
+      }
+    }
+  }
+
+  auto gen_noise = [&](uint32_t q, bool coarse) {
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const int pt = Lat::point(i, 0);
+      const int p = pt / NS, s = pt % NS;
+      const uint32_t traj = (uint32_t)(group * TPB + p);
+      C w0, w1;
+      draw_pair<R>(A, nsp, q, traj, s, w0, w1);
+      if (coarse) {
+        C u0, u1;
+        draw_pair<R>(A, nsp, q + 1, traj, s, u0, u1);
+        w0 = R(0.5) * (w0 + u0);
+        w1 = R(0.5) * (w1 + u1);
+      }
+      xi[2 * i] = w0;
+      xi[2 * i + 1] = w1;
+    }
+  };
+
+  auto propagate = [&](C (&f)[PPT]) {
+    if (A.has_linear) Lat::spectral(L, f, twx, twy, prop);
+  };
+
+  auto method_step = [&]() {
+    if (A.method == XSH_RK4) {
+      // RK4.m:12-38 with the two propagations of each pair batched
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) f1[k] = f0[k] + hdt * D(f0[k], xi[k]);
+      propagate(f0);  // a  = P a
+      propagate(f1);  // a1 = P a1
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        const C A0 = f0[k], A1 = f1[k];
+        const C a2 = A0 + hdt * D(A1, xi[k]);
+        const C a3 = A0 + (R(2) * hdt) * D(a2, xi[k]);
+        f0[k] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
+        f1[k] = a3;
+      }
+      propagate(f0);
+      propagate(f1);
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) f0[k] = f0[k] + (hdt / R(3)) * D(f1[k], xi[k]);
+    } else {
+      // MP.m:53-69
+      propagate(f0);
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) f1[k] = f0[k];
+      for (int it = 0; it < A.iterations; ++it) {
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) f0[k] = f1[k] + hdt * D(f0[k], xi[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) f0[k] = R(2) * f0[k] - f1[k];
+      propagate(f0);
+    }
+  };
+
+  auto observe = [&](int n) {
+    for (int o = 0; o < A.nobs; ++o) {
+      const int kind = A.obs[o];
+      double* dst = av + 2 * A.obs_off[o];
+      if (kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR) {
+        if (kind == XSH_OBS_CORR) {
+#pragma unroll
+          for (int k = 0; k < PPT; ++k) f1[k] = f0[k];
+          Lat::spectral(L, f1, twx, twy, CorrOp<R>{R(1.0 / ((double)NS * (double)NS))});
+        }
+#pragma unroll
+        for (int i = 0; i < CPT; ++i) {
+          const int pt = Lat::point(i, 0);
+          const int p = pt / NS, s = pt % NS;
+          if (group * TPB + p >= A.E) continue;
+#pragma unroll
+          for (int v = 0; v < 2; ++v) {
+            const C a = kind == XSH_OBS_CORR ? f1[2 * i + v] : f0[2 * i + v];
+            double re = (double)a.x, im = (double)a.y;
+            if (kind == XSH_OBS_ABS2) {
+              re = (double)a.x * a.x + (double)a.y * a.y;
+              im = 0.0;
+            }
+            atomic_add_c(dst + 2 * ((long long)n * NS + s + v), re * A.inv_E, im * A.inv_E);
+          }
+        }
+      } else {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int i = 0; i < CPT; ++i) {
+          const int pt = Lat::point(i, 0);
+          const int p = pt / NS;
+          if (group * TPB + p >= A.E) continue;
+#pragma unroll
+          for (int v = 0; v < 2; ++v) {
+            const C a = f0[2 * i + v];
+            const double m = (double)a.x * a.x + (double)a.y * a.y;
+            if (kind == XSH_OBS_A_AVE) {
+              re += (double)a.x;
+              im += (double)a.y;
+            } else if (kind == XSH_OBS_ABS4_AVE) {
+              re += m * m;
+            } else {
+              re += m;
+            }
+          }
+        }
+        block_sum2<NT>(re, im, red);
+        const double sc = (kind == XSH_OBS_ABS2_INT ? A.dv : 1.0 / NS) * A.inv_E;
+        if (threadIdx.x == 0) atomic_add_c(dst + 2 * (long long)n, re * sc, im * sc);
+      }
+    }
+  };
+
+  // ---- the time loop of xpath.m:29-86
+  uint32_t q = 0;
+  for (int n = 0; n < A.npts; ++n) {
+    for (int st = 0; st < A.steps; ++st) {
+      if (n > 0 || st > 0 || A.npts == 1) {
+        if (A.checks && nci == 1) {
+          gen_noise(q, true);
+          q += 2;
+          method_step();
+        } else {
+          for (int c = 0; c <= A.checks; ++c) {
+            gen_noise(q++, false);
+            method_step();
+          }
+        }
+      }
+      if (st == 0) observe(n);
+    }
+  }
+  if (A.a_out) {
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const int pt = Lat::point(i, 0);
+      const int p = pt / NS, s = pt % NS;
+      const int traj = group * TPB + p;
+      if (traj >= A.E) continue;
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        const long long g = (long long)traj * NS + s + v;
+        A.a_out[2 * g] = (double)f0[2 * i + v].x;
+        A.a_out[2 * g + 1] = (double)f0[2 * i + v].y;
+      }
+    }
+  }
+}
+
+// propagation probe: planes [nplanes][NS] complex R in place
+template <class R, int NX, int NY, int NT, int TPB>
+__global__ __launch_bounds__(NT) void probe_propagate(const PathArgs A, int nci, int nplanes, cpx<R>* planes) {
+  using Lat = Lattice<R, NX, NY, NT, TPB>;
+  using C = cpx<R>;
+  __shared__ C L[Lat::LDS];
+  C f[Lat::PPT];
+  const long long base = (long long)blockIdx.x * Lat::NPTS;
+#pragma unroll
+  for (int i = 0; i < Lat::CPT; ++i)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const long long g = base + Lat::point(i, v);
+      f[2 * i + v] = g < (long long)nplanes * Lat::NS ? planes[g] : C{R(0), R(0)};
+    }
+  const PropOp<R> prop{static_cast<const C*>(A.px[nci]), NY > 1 ? static_cast<const C*>(A.py[nci]) : nullptr};
+  Lat::spectral(L, f, static_cast<const C*>(A.twx), static_cast<const C*>(A.twy), prop);
+#pragma unroll
+  for (int i = 0; i < Lat::CPT; ++i)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const long long g = base + Lat::point(i, v);
+      if (g < (long long)nplanes * Lat::NS) planes[g] = f[2 * i + v];
+    }
+}
+
+template <class R>
+__global__ void probe_normals(uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t inst, uint32_t draw,
+                              int traj0, int ntraj, int count, double* out) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int groups = (count + 3) / 4;
+  if (id >= ntraj * groups) return;
+  const int t = id / groups, g = id % groups;
+  R n[4];
+  normals4<R>(u4{(uint32_t)g, draw, (uint32_t)(traj0 + t), inst}, nsp, key1(tag, seq), n);
+  for (int l = 0; l < 4; ++l)
+    if (4 * g + l < count) out[(long long)t * count + 4 * g + l] = (double)n[l];
+}
+
+// ================================================================ launchers
+template <class R>
+static hipError_t launch_sde(const PathArgs& A, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(sde_path<R>, dim3(nblocks), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+
+template <class R, int NX, int NY, int NT, int TPB>
+static hipError_t launch_spde(const PathArgs& A, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((spde_path<R, NX, NY, NT, TPB>), dim3(nblocks), dim3(NT), 0, s, A);
+  return hipGetLastError();
+}
+
+template <class R, int NX, int NY, int NT, int TPB>
+static hipError_t launch_probe(const PathArgs& A, int nci, int nplanes, void* planes, hipStream_t s) {
+  const int nb = (nplanes + TPB - 1) / TPB;
+  hipLaunchKernelGGL((probe_propagate<R, NX, NY, NT, TPB>), dim3(nb), dim3(NT), 0, s, A, nci, nplanes,
+                     static_cast<cpx<R>*>(planes));
+  return hipGetLastError();
+}
+
+#define XSH_ENTRY(R, NX, NY, NT, TPB)                                                           \
+  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, &launch_spde<R, NX, NY, NT, TPB>, \
+   &launch_probe<R, NX, NY, NT, TPB>}
+
+static const KernelEntry kTable[] = {
+    // fp32, 1D lattices
+    XSH_ENTRY(float, 32, 1, 256, 32), XSH_ENTRY(float, 64, 1, 256, 32),
+    XSH_ENTRY(float, 128, 1, 256, 32), XSH_ENTRY(float, 256, 1, 256, 16),
+    XSH_ENTRY(float, 512, 1, 256, 8), XSH_ENTRY(float, 1024, 1, 256, 4),
+    // fp32, 2D lattices
+    XSH_ENTRY(float, 32, 32, 256, 1), XSH_ENTRY(float, 64, 64, 256, 1),
+    XSH_ENTRY(float, 128, 128, 512, 1),
+    // fp64, 1D lattices
+    XSH_ENTRY(double, 32, 1, 256, 16), XSH_ENTRY(double, 64, 1, 256, 16),
+    XSH_ENTRY(double, 128, 1, 256, 16), XSH_ENTRY(double, 256, 1, 256, 8),
+    XSH_ENTRY(double, 512, 1, 256, 4), XSH_ENTRY(double, 1024, 1, 256, 2),
+    // fp64, 2D lattices
+    XSH_ENTRY(double, 32, 32, 256, 1), XSH_ENTRY(double, 64, 64, 512, 1),
+};
+
+const KernelEntry* find_kernel(int precision, int nx, int ny) {
+  for (const auto& e : kTable)
+    if (e.precision == precision && e.nx == nx && e.ny == ny) return &e;
+  return nullptr;
+}
+
+hipError_t launch_sde_path(int precision, const PathArgs& A, int nblocks, hipStream_t s) {
+  return precision == XSH_FP64 ? launch_sde<double>(A, nblocks, s) : launch_sde<float>(A, nblocks, s);
+}
+
+hipError_t launch_probe_normals(int precision, uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t inst,
+                                uint32_t draw, int traj0, int ntraj, int count, double* out, hipStream_t s) {
+  const int groups = (count + 3) / 4;
+  const int total = ntraj * groups;
+  const int nb = (total + 255) / 256;
+  if (precision == XSH_FP64)
+    hipLaunchKernelGGL(probe_normals<double>, dim3(nb), dim3(256), 0, s, nsp, tag, seq, inst, draw, traj0,
+                       ntraj, count, out);
+  else
+    hipLaunchKernelGGL(probe_normals<float>, dim3(nb), dim3(256), 0, s, nsp, tag, seq, inst, draw, traj0,
+                       ntraj, count, out);
+  return hipGetLastError();
+}
+
+}  // namespace xsh

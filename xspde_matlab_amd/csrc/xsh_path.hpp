@@ -1,0 +1,331 @@
+// Path kernels of the xSPDE ensemble integration path on gfx950.
+//
+// One launch runs whole paths (xpath.m:29-86): every workgroup owns a group
+// of trajectories of one (sub-ensemble j, check nc) path and keeps their
+// fields resident on chip - in VGPRs - from the initial state to the last
+// surplus step.  LDS is the workspace of the spectral transforms only:
+//
+//   forward  : in-place decimation-in-frequency stages (natural -> digit
+//              reversed order), one LDS read+write per stage
+//   propagate: the last forward stage, the multiply by the propagator
+//              P(kx,ky) = exp(L dtr/2) and the first inverse stage are one
+//              register-resident step (xprop.m:104-106)
+//   inverse  : in-place decimation-in-time stages (digit reversed ->
+//              natural order), so no permutation pass is ever needed.
+//
+// Noise is generated in registers per step from a Philox4x32-10 counter
+// (xnoise.m:138-141 semantics), held constant over the method's stages
+// (RK4.m:13-37).  Observables are reduced in FP64 and accumulated with
+// FP64 atomics into the per-path ensemble means (xdata.m:192).
+#pragma once
+#include "xsh_common.hpp"
+#include "../../include/xspde_hip.h"
+
+namespace xsh {
+
+struct PathArgs {
+  int nx, ny, ns;
+  double x0, dx, y0, dy, dv;
+  int npts, steps, checks, method, iterations;
+  double hdt[2];           // dtr/2 of the fine (0) and coarse (1) runs
+  int E, e2, e3, npar, seed, seq, inst;
+  int j_only, nc_only;     // > 0: one path (j, nc)
+  int groups;              // trajectory groups per path
+  int noises, inrandoms, urandoms;
+  double noise_scale;      // sqrt(nscale * noisefactor), xnoise.m:139
+  double init_scale;       // sqrt(noisefactor), xrandom.m:189
+  int deriv;
+  double dp[6];
+  int initial;
+  double ip[6];
+  int nobs;
+  int obs[XSH_MAX_OBS];
+  long long obs_off[XSH_MAX_OBS];
+  long long avlen;
+  double inv_E;
+  int has_linear;
+  const void* px[2];       // [nc] propagator factor along x (incl. exp(l0 dt), 1/nx)
+  const void* py[2];       // [nc] propagator factor along y (incl. 1/ny)
+  const void* twx;         // W_nx^m, m < nx
+  const void* twy;
+  double* av;              // [paths][avlen] complex double
+  const double* a0;        // optional [E][ns] complex double initial fields
+  double* a_out;           // optional [E][ns] complex double final fields
+};
+
+// ------------------------------------------------------------ drift forms
+template <class R>
+struct Drift {
+  int kind;
+  cpx<R> g, u;
+  R s, c;
+  __device__ __forceinline__ void init(const PathArgs& A) {
+    kind = A.deriv;
+    g = cpx<R>{(R)A.dp[0], (R)A.dp[1]};
+    u = cpx<R>{(R)A.dp[2], (R)A.dp[3]};
+    s = (R)A.dp[4];
+    c = (R)A.dp[0];
+  }
+  // D(a, xi): Kubo.m:17 (i c w a) or the GP/NLS/CGL family
+  __device__ __forceinline__ cpx<R> operator()(cpx<R> a, cpx<R> xi) const {
+    if (kind == XSH_DERIV_KUBO) {
+      const R k = c * xi.x;
+      return cpx<R>{-k * a.y, k * a.x};
+    }
+    if (kind == XSH_DERIV_GP) {
+      const R n = norm2(a);
+      cpx<R> ga = g * a;
+      cpx<R> r = cpx<R>{n * ga.x, n * ga.y} + u * a;
+      return cpx<R>{r.x + s * xi.x, r.y + s * xi.y};
+    }
+    return cpx<R>{R(0), R(0)};
+  }
+};
+
+// block-wide sum of two doubles; every thread gets the result
+template <int NT>
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o, 64);
+    b += __shfl_down(b, o, 64);
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) {
+    red[2 * w] = a;
+    red[2 * w + 1] = b;
+  }
+  __syncthreads();
+  a = 0.0;
+  b = 0.0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) {
+    a += red[2 * i];
+    b += red[2 * i + 1];
+  }
+}
+
+__device__ __forceinline__ void atomic_add_c(double* p, double re, double im) {
+  unsafeAtomicAdd(p, re);
+  unsafeAtomicAdd(p + 1, im);
+}
+
+// ------------------------------------------------------- lattice engine
+// Trajectory planes of NX x NY sites, TPB planes per workgroup of NT
+// threads.  Field registers: chunk c = tid + NT*i (i < CPT) holds VEC=2
+// consecutive sites of one row.  LDS: plane p, row y, column x at
+// p*PS + y*RS + x with one pad element per row.
+template <int N>
+struct Split {  // N = N1 * N2, both <= 32, DIF radix order (N1 first)
+  static constexpr int N1 = N <= 4 ? (N >= 2 ? 2 : 1)
+                          : N == 8 ? 4 : N == 16 ? 4 : N == 32 ? 4 : N == 64 ? 8
+                          : N == 128 ? 16 : N == 256 ? 16 : N == 512 ? 16 : 32;
+  static constexpr int N2 = N / N1;
+};
+
+template <class R, int NX, int NY, int NT, int TPB>
+struct Lattice {
+  static constexpr int NS = NX * NY;
+  static constexpr int VEC = 2;
+  static constexpr int NPTS = TPB * NS;
+  static constexpr int CPT = NPTS / (VEC * NT);
+  static constexpr int PPT = CPT * VEC;
+  static constexpr int RS = NX + 1;
+  static constexpr int PS = NY * RS;
+  static constexpr int LDS = TPB * PS;
+  static_assert(NPTS % (VEC * NT) == 0, "points per workgroup must tile the threads");
+  static_assert(NX % 2 == 0, "rows hold an even number of sites");
+  using C = cpx<R>;
+
+  // point index of register slot (i, v)
+  __device__ static __forceinline__ int point(int i, int v) { return (threadIdx.x + NT * i) * VEC + v; }
+  __device__ static __forceinline__ int lds_of(int pt) {
+    const int p = pt / NS, s = pt - p * NS;
+    const int y = s / NX, x = s - y * NX;
+    return p * PS + y * RS + x;
+  }
+
+  __device__ static __forceinline__ void store(C* L, const C (&f)[PPT]) {
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const int a = lds_of(point(i, 0));
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) L[a + v] = f[i * VEC + v];
+    }
+  }
+  __device__ static __forceinline__ void load(const C* L, C (&f)[PPT]) {
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const int a = lds_of(point(i, 0));
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) f[i * VEC + v] = L[a + v];
+    }
+  }
+
+  // line geometry along dim 0 (x, rows) or 1 (y, columns)
+  template <int DIM>
+  struct Line {
+    static constexpr int N = DIM == 0 ? NX : NY;
+    static constexpr int NL = DIM == 0 ? TPB * NY : TPB * NX;
+    static constexpr int ES = DIM == 0 ? 1 : RS;
+    __device__ static __forceinline__ int base(int l) {
+      if (DIM == 0) {
+        const int p = l / NY, y = l - p * NY;
+        return p * PS + y * RS;
+      } else {
+        const int p = l / NX, x = l - p * NX;
+        return p * PS + x;
+      }
+    }
+  };
+
+  // decimation-in-frequency stage 1: natural -> (k1, n2) with twiddles
+  template <int DIM>
+  __device__ static __forceinline__ void dif1(C* L, const C* tw) {
+    using Ln = Line<DIM>;
+    constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+    for (int it = threadIdx.x; it < Ln::NL * N2; it += NT) {
+      const int l = it % Ln::NL, n2 = it / Ln::NL;
+      const int b = Ln::base(l);
+      C v[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) v[n1] = L[b + (n1 * N2 + n2) * Ln::ES];
+      dft<R, N1, -1>(v);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; ++k1) v[k1] = v[k1] * tw[n2 * k1];
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) L[b + (k1 * N2 + n2) * Ln::ES] = v[k1];
+    }
+  }
+  // forward stage 2 (no middle op): block k1 -> X[k1 + N1 k2] at k1*N2 + k2
+  template <int DIM>
+  __device__ static __forceinline__ void dif2(C* L) {
+    using Ln = Line<DIM>;
+    constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+    for (int it = threadIdx.x; it < Ln::NL * N1; it += NT) {
+      const int l = it % Ln::NL, k1 = it / Ln::NL;
+      const int b = Ln::base(l) + k1 * N2 * Ln::ES;
+      C v[N2];
+#pragma unroll
+      for (int n = 0; n < N2; ++n) v[n] = L[b + n * Ln::ES];
+      dft<R, N2, -1>(v);
+#pragma unroll
+      for (int n = 0; n < N2; ++n) L[b + n * Ln::ES] = v[n];
+    }
+  }
+  // inverse stage A (no middle op): inverse DFT-N2 of block k1, twiddle conj
+  template <int DIM>
+  __device__ static __forceinline__ void dita(C* L, const C* tw) {
+    using Ln = Line<DIM>;
+    constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+    for (int it = threadIdx.x; it < Ln::NL * N1; it += NT) {
+      const int l = it % Ln::NL, k1 = it / Ln::NL;
+      const int b = Ln::base(l) + k1 * N2 * Ln::ES;
+      C v[N2];
+#pragma unroll
+      for (int n = 0; n < N2; ++n) v[n] = L[b + n * Ln::ES];
+      dft<R, N2, 1>(v);
+#pragma unroll
+      for (int n2 = 1; n2 < N2; ++n2) v[n2] = v[n2] * conjc(tw[n2 * k1]);
+#pragma unroll
+      for (int n = 0; n < N2; ++n) L[b + n * Ln::ES] = v[n];
+    }
+  }
+  // forward stage 2 + op(X, l, k) + inverse stage A in registers
+  template <int DIM, class Op>
+  __device__ static __forceinline__ void mid(C* L, const C* tw, const Op& op) {
+    using Ln = Line<DIM>;
+    constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+    for (int it = threadIdx.x; it < Ln::NL * N1; it += NT) {
+      const int l = it % Ln::NL, k1 = it / Ln::NL;
+      const int b = Ln::base(l) + k1 * N2 * Ln::ES;
+      C v[N2];
+#pragma unroll
+      for (int n = 0; n < N2; ++n) v[n] = L[b + n * Ln::ES];
+      dft<R, N2, -1>(v);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) v[k2] = op(v[k2], l, k1 + N1 * k2);
+      dft<R, N2, 1>(v);
+#pragma unroll
+      for (int n2 = 1; n2 < N2; ++n2) v[n2] = v[n2] * conjc(tw[n2 * k1]);
+#pragma unroll
+      for (int n = 0; n < N2; ++n) L[b + n * Ln::ES] = v[n];
+    }
+  }
+  // decimation-in-time stage B: (k1, n2) -> natural order
+  template <int DIM>
+  __device__ static __forceinline__ void ditb(C* L) {
+    using Ln = Line<DIM>;
+    constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+    for (int it = threadIdx.x; it < Ln::NL * N2; it += NT) {
+      const int l = it % Ln::NL, n2 = it / Ln::NL;
+      const int b = Ln::base(l);
+      C v[N1];
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) v[k1] = L[b + (k1 * N2 + n2) * Ln::ES];
+      dft<R, N1, 1>(v);
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) L[b + (n1 * N2 + n2) * Ln::ES] = v[n1];
+    }
+  }
+
+  // frequency index held at position pos along x after the forward stages
+  __device__ static __forceinline__ int kx_of_pos(int pos) {
+    constexpr int N1 = Split<NX>::N1, N2 = Split<NX>::N2;
+    return pos / N2 + N1 * (pos % N2);
+  }
+
+  // a -> T^-1 op(T a): periodic transform of every plane held in f
+  template <class Op>
+  __device__ static __forceinline__ void spectral(C* L, C (&f)[PPT], const C* twx, const C* twy,
+                                                  const Op& op) {
+    __syncthreads();
+    store(L, f);
+    __syncthreads();
+    if constexpr (NY == 1) {
+      dif1<0>(L, twx);
+      __syncthreads();
+      mid<0>(L, twx, [&](C X, int l, int k) { return op(X, k, 0); });
+      __syncthreads();
+      ditb<0>(L);
+    } else {
+      dif1<0>(L, twx);
+      __syncthreads();
+      dif2<0>(L);
+      __syncthreads();
+      dif1<1>(L, twy);
+      __syncthreads();
+      mid<1>(L, twy, [&](C X, int l, int k) { return op(X, kx_of_pos(l % NX), k); });
+      __syncthreads();
+      ditb<1>(L);
+      __syncthreads();
+      dita<0>(L, twx);
+      __syncthreads();
+      ditb<0>(L);
+    }
+    __syncthreads();
+    load(L, f);
+  }
+};
+
+// propagator op: X * Px[kx] * Py[ky]
+template <class R>
+struct PropOp {
+  const cpx<R>* px;
+  const cpx<R>* py;
+  __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int kx, int ky) const {
+    cpx<R> r = X * px[kx];
+    return py ? r * py[ky] : r;
+  }
+};
+// correlation op: |X|^2 * scale
+template <class R>
+struct CorrOp {
+  R scale;
+  __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int, int) const {
+    return cpx<R>{norm2(X) * scale, R(0)};
+  }
+};
+
+}  // namespace xsh

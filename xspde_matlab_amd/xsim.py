@@ -1,0 +1,283 @@
+"""Host mirror of xSPDE's ``xsim``/``xensemble``/``xpath`` for the GPU path.
+
+A problem is written like an xSPDE project (a ``p`` dictionary with the
+reference's field names, ``doc/source/api.rst``) plus a ``gpu`` entry that
+names the built-in forms the GPU integrates::
+
+    p = dict(dimensions=3, points=[101, 128, 128], ranges=[4, 20, 20],
+             steps=[20, 1, 1], method='RK4', noises=2, checks=1,
+             ensembles=[64, 16, 1],
+             gpu=dict(deriv='gp', g=-(1 + 0.5j), u=0, s=0.05 / 2 ** 0.5,
+                      linear=dict(l0=1, l2=1 + 1j), initial='phase', amp=0.1,
+                      observe=['abs2_ave', 'corr'], precision='fp32'))
+    error, data, p = xsim(p)
+
+Names follow the reference: ``xpath`` is the p.path contract
+(Simulator/xpath.m:1), ``xensemble`` the sub-ensemble/check loops with
+D, D^2 accumulation (Simulator/xensemble.m:12-108), ``xsim`` the parallel
+ensemble loop and the final error statistics (Simulator/xsim.m:28-188).
+A drift given as a Python callable instead of a ``gpu`` form is rejected:
+in the MATLAB host such projects stay on the reference's CPU ``xpath``.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+
+
+class GpuFormError(ValueError):
+    """The project cannot run on the GPU path (not a built-in form)."""
+
+
+def _vec(x, n, default):
+    if x is None:
+        x = default
+    if np.isscalar(x):
+        x = [x]
+    x = list(x)
+    while len(x) < n:
+        x.append(default if np.isscalar(default) else x[-1])
+    return x[:n]
+
+
+def preferences(p):
+    """Defaults and derived sizes of the fields the path reads (xpreferences.m:42-175)."""
+    q = dict(p)
+    if "gpu" not in q:
+        raise GpuFormError("no 'gpu' forms: the project stays on the CPU xpath")
+    for k in ("deriv", "linear", "initial", "observe"):
+        if callable(q.get(k)) or (k == "observe" and q.get(k) is not None):
+            raise GpuFormError("user function handle '%s': not a built-in GPU form" % k)
+    nd = max(1, int(q.get("dimensions", 1)))
+    q["dimensions"] = nd
+    fields = int(q.get("fields", 1))
+    if fields != 1:
+        raise GpuFormError("the built-in forms integrate one field component")
+    q["noises"] = int(q["noises"]) if q.get("noises") is not None else fields
+    if q.get("inrandoms") is None and q.get("urandoms") is None:
+        q["inrandoms"] = q["noises"]                                 # xpreferences.m:74-76
+    q["inrandoms"] = int(q.get("inrandoms") or 0)
+    q["urandoms"] = int(q.get("urandoms") or 0)
+    q["ensembles"] = [int(e) for e in _vec(q.get("ensembles"), 3, 1)]
+    q["ranges"] = [float(r) for r in _vec(q.get("ranges"), nd, 10.0)]
+    steps = [int(s) for s in _vec(q.get("steps"), nd, 1)]
+    q["steps"] = steps
+    pts = q.get("points")
+    pts = [51] + [35] * (nd - 1) if pts is None else _vec(pts, nd, 1)
+    pts = [int(pts[0])] + [1 + (int(pts[d]) - 1) * steps[d] for d in range(1, nd)]   # :137
+    q["points"] = pts
+    og = q.get("origins")
+    q["origins"] = [0.0] + [-r / 2 for r in q["ranges"][1:]] if og is None else [float(o) for o in og]
+    ch = q.get("checks")
+    ch = [1] + [0] * (nd - 1) if ch is None else _vec(ch, nd, 0)
+    if any(ch[1:]):
+        raise GpuFormError("spatial checks are not on the GPU path")
+    q["checks"] = [int(c) for c in ch]
+    meth = q.get("method")
+    if meth is None:
+        meth = "RK4" if np.prod(q["ensembles"]) == 1 else "MP"       # xpreferences.m:101-105
+    if not isinstance(meth, str):
+        meth = getattr(meth, "__name__", str(meth))
+    if meth not in ("RK4", "MP"):
+        raise GpuFormError("method %s is not on the GPU path" % meth)
+    q["method"] = meth
+    q["iterations"] = int(q.get("iterations", 4))
+    q["order"] = int(q.get("order", 0))
+    q["seed"] = int(q.get("seed", 0))
+    q["dt"] = q["ranges"][0] / max(1, pts[0] - 1)                   # :174
+    return q
+
+
+def make_desc(q):
+    g = q["gpu"]
+    d = _lib.Desc()
+    d.precision = _lib.FP64 if str(g.get("precision", "fp32")).lower() in ("fp64", "double") else _lib.FP32
+    nd = q["dimensions"]
+    d.dimensions = nd
+    d.points = q["points"][0]
+    d.nx = q["points"][1] if nd > 1 else 1
+    d.ny = q["points"][2] if nd > 2 else 1
+    d.steps = q["steps"][0]
+    d.checks = q["checks"][0]
+    d.method = _lib.RK4 if q["method"] == "RK4" else _lib.MP
+    d.iterations = q["iterations"]
+    for i in range(3):
+        d.ensembles[i] = q["ensembles"][i]
+    d.seed = q["seed"]
+    d.noises, d.inrandoms, d.urandoms = q["noises"], q["inrandoms"], q["urandoms"]
+    bnd = q.get("boundaries") or []
+    for k in range(2):
+        if k < len(bnd) and bnd[k] is not None:
+            d.boundaries[k][0], d.boundaries[k][1] = int(bnd[k][0]), int(bnd[k][1])
+    for i in range(min(3, nd)):
+        d.ranges[i] = q["ranges"][i]
+        d.origins[i] = q["origins"][i]
+    kind = g.get("deriv", "zero")
+    if kind not in _lib.DERIV:
+        raise GpuFormError("unknown drift form %r" % (kind,))
+    d.deriv = _lib.DERIV[kind]
+    if kind == "kubo":
+        d.deriv_par[0] = float(g.get("c", 1.0))
+    elif kind == "gp":
+        gg, u = complex(g.get("g", 0)), complex(g.get("u", 0))
+        d.deriv_par[0], d.deriv_par[1], d.deriv_par[2], d.deriv_par[3] = gg.real, gg.imag, u.real, u.imag
+        d.deriv_par[4] = float(g.get("s", 0.0))
+    lin = g.get("linear")
+    if lin is not None:
+        d.has_linear = 1
+        l0 = complex(lin.get("l0", 0))
+        d.lin_l0[0], d.lin_l0[1] = l0.real, l0.imag
+        c1 = np.broadcast_to(np.asarray(lin.get("l1", 0), complex), (2,))
+        c2 = np.broadcast_to(np.asarray(lin.get("l2", 0), complex), (2,))
+        for k in range(2):
+            d.lin_l1[k][0], d.lin_l1[k][1] = c1[k].real, c1[k].imag
+            d.lin_l2[k][0], d.lin_l2[k][1] = c2[k].real, c2[k].imag
+    ini = g.get("initial", "const")
+    if ini not in _lib.INITIAL:
+        raise GpuFormError("unknown initial form %r" % (ini,))
+    d.initial = _lib.INITIAL[ini]
+    if ini == "const":
+        c = complex(g.get("c0", 1.0))
+        d.init_par[0], d.init_par[1] = c.real, c.imag
+    elif ini == "sech":
+        d.init_par[0], d.init_par[1], d.init_par[2] = g.get("amp", 1.0), g.get("width", 1.0), g.get("eps", 0.0)
+    elif ini == "gauss":
+        d.init_par[0], d.init_par[1] = g.get("amp", 1.0), g.get("width", 1.0)
+    elif ini == "phase":
+        d.init_par[0] = g.get("amp", 1.0)
+    elif ini == "coherent":
+        a = complex(g.get("alpha", 1.0))
+        d.init_par[0], d.init_par[1] = a.real, a.imag
+    obs = list(g.get("observe", ["a"]))
+    if not 1 <= len(obs) <= _lib.XSH_MAX_OBS:
+        raise GpuFormError("1..%d observables" % _lib.XSH_MAX_OBS)
+    d.nobserve = len(obs)
+    for o, name in enumerate(obs):
+        if name not in _lib.OBSERVE:
+            raise GpuFormError("unknown observe form %r" % (name,))
+        d.observe[o] = _lib.OBSERVE[name]
+    d.seq = int(g.get("seq", 0))
+    d.inst = int(g.get("inst", 0))
+    return d
+
+
+def split_averages(q, info, av):
+    """Flat per-path averages -> av[o] of shape [1, points, sites_o] (xpath.m:16,81)."""
+    out = []
+    npts = q["points"][0]
+    for o in range(len(q["gpu"].get("observe", ["a"]))):
+        off, ns = info.obs_offset[o], info.obs_sites[o]
+        out.append(av[off:off + npts * ns].reshape(1, npts, ns))
+    return out
+
+
+class Problem:
+    """A GPU handle plus the host-side bookkeeping of one xSPDE project."""
+
+    def __init__(self, p):
+        self.q = preferences(p)
+        self.handle = _lib.Handle(make_desc(self.q))
+        self.info = self.handle.info
+        outs = self.q.get("output")
+        nobs = len(self.q["gpu"].get("observe", ["a"]))
+        if outs is None:
+            outs = [None] * nobs
+        self.outputs = [o if o is not None else (lambda av, p, _n=n: av[_n]) for n, o in enumerate(outs)]
+
+    def xpath(self, nc, npar=1, j=1, a0=None, want_final=False):
+        """p.path for one (j, nc): returns av (and the final fields)."""
+        r = self.handle.run_path(npar, j, nc, a0=a0, want_final=want_final)
+        if want_final:
+            return split_averages(self.q, self.info, r[0]), r[1]
+        return split_averages(self.q, self.info, r)
+
+    def xensemble(self, npar):
+        """All sub-ensembles and checks of one parallel ensemble (xensemble.m:25-105)."""
+        q = self.q
+        self.handle.run_ensemble(npar)
+        self.handle.synchronize()
+        av_all = self.handle.read_averages()
+        e2, e3 = q["ensembles"][1], q["ensembles"][2]
+        nchk = 1 + q["checks"][0]
+        ne = 1.0 / (e2 * e3) - 1
+        m = nchk
+        m1 = max(3, m + 1)
+        data = [None] * len(self.outputs)
+        for j in range(e2):
+            D = {}
+            for nc in range(nchk):
+                av = split_averages(q, self.info, av_all[j, nc])
+                for o, out in enumerate(self.outputs):
+                    D[(nc + 1, o)] = np.real(np.asarray(out(av, q))).reshape(-1)
+            for o in range(len(self.outputs)):
+                nes = 1 + ne
+                if q["checks"][0] and q["order"] > 0:                    # xensemble.m:93-97
+                    dl = (D[(1, o)] - D[(2, o)]) / (2.0 ** q["order"] - 1.0)
+                    D[(2, o)] = D[(1, o)]
+                    D[(1, o)] = D[(1, o)] + dl
+                if data[o] is None:
+                    data[o] = np.zeros((D[(1, o)].size, m1))
+                for n1 in range(1, m + 1):
+                    data[o][:, n1 - 1] += D[(n1, o)] * nes
+                data[o][:, m1 - 1] += D[(1, o)] ** 2 * nes
+        return data
+
+    def close(self):
+        self.handle.close()
+
+
+def statistics(q, data):
+    """Step and sampling errors of xsim.m:94-108 and the RMS summary of :144-188
+    (no comparisons): returns (error vector, per-output [ndata, 3] arrays of
+    mean, step error, sampling error)."""
+    ne = q["ensembles"][1] * q["ensembles"][2]
+    nchk = 1 + q["checks"][0]
+    sx = max(3, 1 + nchk)
+    tol = float(q.get("tol", 1e-20))
+    er = np.zeros((len(data), max(6, nchk + 4)))
+    outd = []
+    for o, d in enumerate(data):
+        d = np.array(d, dtype=float)
+        mx = np.max(np.abs(d[:, 0])) ** 2
+        if not (q.get("relerr", 1) and mx > tol):
+            mx = 1.0
+        if ne > 1:
+            d[:, sx - 1] = np.abs(np.sqrt((d[:, sx - 1] - d[:, 0] ** 2).astype(complex) / (ne - 1)))
+        else:
+            d[:, sx - 1] = 0
+        for nc in range(2, nchk + 1):
+            d[:, nc - 1] = np.abs(d[:, nc - 1] - d[:, 0])
+            er[o, nc - 1] = np.mean(d[:, nc - 1] ** 2) / mx
+        er[o, sx - 1] = np.mean(d[:, sx - 1] ** 2) / mx
+        seg = er[o, 1:sx + 1]
+        er[o, 0] = np.sum(seg) / (1e-99 + np.sum(seg > 0))
+        er[o, :sx + 1] = np.sqrt(er[o, :sx + 1])
+        res = np.zeros((d.shape[0], 3))
+        res[:, 0] = d[:, 0]
+        res[:, 1] = np.sqrt(np.sum(d[:, 1:nchk] ** 2, axis=1)) if nchk > 1 else 0.0
+        res[:, 2] = d[:, sx - 1]
+        outd.append(res)
+    error = np.zeros(max(6, sx + 3))
+    for c in range(2, sx + 2):
+        pts = 1e-99 + np.sum(er[:, c - 1] > tol)
+        error[c - 1] = math.sqrt(np.sum(er[:, c - 1] ** 2) / pts)
+    error[0] = math.sqrt(np.sum(error[1:sx + 1] ** 2) / (1e-99 + np.sum(error[1:sx + 1] > tol)))
+    return error, outd
+
+
+def xsim(p):
+    """[error, data, p] = xsim(p) on the GPU path (xsim.m:28-188, no comparisons)."""
+    t0 = time.time()
+    prob = Problem(p)
+    try:
+        data = None
+        for npar in range(1, prob.q["ensembles"][2] + 1):          # parfor == serial (xsim.rst:275)
+            dp = prob.xensemble(npar)
+            data = dp if data is None else [a + b for a, b in zip(data, dp)]
+    finally:
+        prob.close()
+    error, outd = statistics(prob.q, data)
+    error[-1] = time.time() - t0
+    return error, outd, prob.q
