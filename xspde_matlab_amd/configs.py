@@ -43,8 +43,11 @@ def shrink(p, E=None, e2=None, points=None, steps=None):
     if e2 is not None:
         ens[1] = e2
     p["ensembles"] = ens
+    # keep the coarse time step ranges(1)/((points(1)-1)*steps(1)) unchanged
+    dtc = p["ranges"][0] / ((p["points"][0] - 1) * p["steps"][0])
     if points is not None:
         p["points"] = [points] + list(p["points"][1:])
     if steps is not None:
         p["steps"] = [steps] + list(p["steps"][1:])
+    p["ranges"] = [dtc * p["steps"][0] * (p["points"][0] - 1)] + list(p["ranges"][1:])
     return p

@@ -60,6 +60,10 @@ struct xsh_handle {
   double* d_a;       // [E][ns] complex (single path in/out)
   hipStream_t stream;
   hipEvent_t ev0, ev1;
+  xsh::PathArgs* d_args;   // kernel arguments in device memory
+  xsh::PathArgs* h_args;   // pinned staging copy
+  hipEvent_t ev_args;      // recorded after the last argument upload
+  bool args_pending;
   bool ran;
 };
 
@@ -238,8 +242,12 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   if (hipMalloc(&h->d_path_av, (size_t)h->avlen * 16) != hipSuccess) return cleanup(fail(XSH_ENOMEM, "hipMalloc path averages"));
   if (hipMalloc(&h->d_a, (size_t)h->E * h->ns * 16) != hipSuccess) return cleanup(fail(XSH_ENOMEM, "hipMalloc fields"));
   if (hipStreamCreate(&h->stream) != hipSuccess) return cleanup(fail(XSH_EDEVICE, "hipStreamCreate"));
-  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_args, hipEventDisableTiming) != hipSuccess)
     return cleanup(fail(XSH_EDEVICE, "hipEventCreate"));
+  if (hipMalloc(&h->d_args, sizeof(xsh::PathArgs)) != hipSuccess) return cleanup(fail(XSH_ENOMEM, "hipMalloc arguments"));
+  if (hipHostMalloc(&h->h_args, sizeof(xsh::PathArgs), hipHostMallocDefault) != hipSuccess)
+    return cleanup(fail(XSH_ENOMEM, "hipHostMalloc arguments"));
   *out = h;
   return XSH_OK;
 }
@@ -253,6 +261,9 @@ extern "C" void xsh_destroy(xsh_handle* h) {
   if (h->d_a) hipFree(h->d_a);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->ev_args) hipEventDestroy(h->ev_args);
+  if (h->d_args) hipFree(h->d_args);
+  if (h->h_args) hipHostFree(h->h_args);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
@@ -329,12 +340,25 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
   return A;
 }
 
+// upload the arguments on the handle's stream (the staging buffer is reused
+// only after the previous upload has completed)
+static int upload_args(xsh_handle* h, const xsh::PathArgs& A) {
+  if (h->args_pending) HIPCHK(hipEventSynchronize(h->ev_args));
+  *h->h_args = A;
+  HIPCHK(hipMemcpyAsync(h->d_args, h->h_args, sizeof(xsh::PathArgs), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipEventRecord(h->ev_args, h->stream));
+  h->args_pending = true;
+  return XSH_OK;
+}
+
 static int launch(xsh_handle* h, const xsh::PathArgs& A, int nblocks) {
+  int rc = upload_args(h, A);
+  if (rc) return rc;
   hipError_t e;
   if (h->nd == 1)
-    e = xsh::launch_sde_path(h->d.precision, A, nblocks, h->stream);
+    e = xsh::launch_sde_path(h->d.precision, h->d_args, nblocks, h->stream);
   else
-    e = h->kern->launch(A, nblocks, h->stream);
+    e = h->kern->launch(h->d_args, nblocks, h->stream);
   if (e != hipSuccess) return fail(XSH_EDEVICE, "kernel launch: %s", hipGetErrorString(e));
   return XSH_OK;
 }
@@ -428,8 +452,13 @@ extern "C" int xsh_probe_propagate(xsh_handle* h, int nc, int nplanes, double* p
     for (size_t i = 0; i < 2 * n; ++i) reinterpret_cast<float*>(buf.data())[i] = (float)planes[i];
   void* d = nullptr;
   HIPCHK(hipMalloc(&d, buf.size()));
+  int rc = upload_args(h, A);
+  if (rc) {
+    hipFree(d);
+    return rc;
+  }
   hipError_t e = hipMemcpyAsync(d, buf.data(), buf.size(), hipMemcpyHostToDevice, h->stream);
-  if (e == hipSuccess) e = h->kern->probe(A, nc - 1, nplanes, d, h->stream);
+  if (e == hipSuccess) e = h->kern->probe(h->d_args, nc - 1, nplanes, d, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(buf.data(), d, buf.size(), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   hipFree(d);

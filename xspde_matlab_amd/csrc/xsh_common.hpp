@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace xsh {
 
 template <class R>
@@ -38,74 +40,77 @@ __device__ __forceinline__ cpx<R> mul_i(cpx<R> a) {
 #include "xsh_twiddle64.inc"
 
 // ------------------------------------------------------------------ DFT
+// Compile-time loop: f(integral_constant<int, I>) for I in [B, E).  Every
+// register-array index below is a constant expression, so the arrays stay in
+// VGPRs whatever the unroller decides.
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+__host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+__host__ __device__ constexpr int bit_reverse(int i, int bits) {
+  int r = 0;
+  for (int b = 0; b < bits; ++b)
+    if (i & (1 << b)) r |= 1 << (bits - 1 - b);
+  return r;
+}
+
 // In-register DFT of length N (power of two, N <= 64), natural order in and
 // out.  SIGN = -1 forward (exp(-2 pi i nk/N)), +1 inverse (no 1/N).
-template <int N>
-struct bitrev {
-  static constexpr int bits = N <= 1 ? 0 : 1 + bitrev<N / 2>::bits;
-  __device__ static constexpr int of(int i) {
-    int r = 0;
-    for (int b = 0; b < bits; ++b)
-      if (i & (1 << b)) r |= 1 << (bits - 1 - b);
-    return r;
-  }
-};
-template <>
-struct bitrev<1> {
-  static constexpr int bits = 0;
-  __device__ static constexpr int of(int) { return 0; }
-};
-
+// Radix-2 decimation in time over the bit-reversed order; the twiddles are
+// compile-time constants (k = 0 and k = len/4 need no multiply).
 template <class R, int N, int SIGN>
 __device__ __forceinline__ void dft(cpx<R> (&v)[N]) {
   if constexpr (N == 1) {
     return;
   } else if constexpr (N == 2) {
-    cpx<R> a = v[0], b = v[1];
+    const cpx<R> a = v[0], b = v[1];
     v[0] = a + b;
     v[1] = a - b;
   } else if constexpr (N == 4) {
-    cpx<R> a0 = v[0] + v[2], a1 = v[0] - v[2];
-    cpx<R> b0 = v[1] + v[3], b1 = mul_i<SIGN>(v[1] - v[3]);
+    const cpx<R> a0 = v[0] + v[2], a1 = v[0] - v[2];
+    const cpx<R> b0 = v[1] + v[3], b1 = mul_i<SIGN>(v[1] - v[3]);
     v[0] = a0 + b0;
     v[2] = a0 - b0;
     v[1] = a1 + b1;
     v[3] = a1 - b1;
   } else {
-    // radix-2 decimation in time over the bit-reversed order, radix-4 leaves
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      constexpr int dummy = 0;
-      (void)dummy;
-      const int j = bitrev<N>::of(i);
-      if (j > i) {
-        cpx<R> t = v[i];
+    constexpr int BITS = ilog2(N);
+    static_for<0, N>([&](auto I) __attribute__((always_inline)) {
+      constexpr int i = decltype(I)::value;
+      constexpr int j = bit_reverse(i, BITS);
+      if constexpr (j > i) {
+        const cpx<R> t = v[i];
         v[i] = v[j];
         v[j] = t;
       }
-    }
-#pragma unroll
-    for (int len = 2; len <= N; len <<= 1) {
-      const int half = len >> 1;
-#pragma unroll
-      for (int i = 0; i < N; i += len) {
-#pragma unroll
-        for (int k = 0; k < half; ++k) {
-          cpx<R> a = v[i + k];
+    });
+    static_for<1, BITS + 1>([&](auto S) __attribute__((always_inline)) {
+      constexpr int len = 1 << decltype(S)::value;
+      constexpr int half = len / 2;
+      static_for<0, N / len>([&](auto G) __attribute__((always_inline)) {
+        static_for<0, half>([&](auto K) __attribute__((always_inline)) {
+          constexpr int i = decltype(G)::value * len;
+          constexpr int k = decltype(K)::value;
+          const cpx<R> a = v[i + k];
           cpx<R> b = v[i + k + half];
-          if (k == 0) {
-          } else if (4 * k == len) {
+          if constexpr (k == 0) {
+          } else if constexpr (4 * k == len) {
             b = mul_i<SIGN>(b);
           } else {
-            const int m = k * (64 / len);
+            constexpr int m = k * (64 / len);
             const cpx<R> w = {R(kCos64[m]), R(SIGN) * R(kSin64[m])};
             b = b * w;
           }
           v[i + k] = a + b;
           v[i + k + half] = a - b;
-        }
-      }
-    }
+        });
+      });
+    });
   }
 }
 

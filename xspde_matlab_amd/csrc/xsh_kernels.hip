@@ -11,18 +11,23 @@
 
 namespace xsh {
 
+// Block -> (path, trajectory group).  Blocks of the fine runs (nc = 1, twice
+// the steps of a coarse run) come first so the longest work starts first; the
+// averages of path (j, nc) go to slot (j-1)*nchk + nc-1, the layout of
+// xsh_read_averages.
 template <class R>
 __device__ __forceinline__ void path_identity(const PathArgs& A, int lb, int& pidx, int& nci, int& j,
                                               int& group) {
   group = lb % A.groups;
-  pidx = lb / A.groups;
+  const int b = lb / A.groups;
   if (A.j_only > 0) {
     j = A.j_only;
     nci = A.nc_only - 1;
     pidx = 0;
   } else {
-    nci = pidx / A.e2;
-    j = pidx % A.e2 + 1;
+    nci = b / A.e2;
+    j = b % A.e2 + 1;
+    pidx = (j - 1) * (1 + A.checks) + nci;
   }
 }
 
@@ -58,14 +63,15 @@ __device__ __forceinline__ R init_normal(const PathArgs& A, uint32_t nsp, uint32
   R n[4];
   normals4<R>(u4{(uint32_t)L >> 2, 0u, traj, (uint32_t)A.inst}, nsp,
               key1(TAG_GAUSS_INIT, (uint32_t)A.seq), n);
-  return n[L & 3];
+  const int l = L & 3;  // select, not a dynamically indexed private array
+  return l == 0 ? n[0] : l == 1 ? n[1] : l == 2 ? n[2] : n[3];
 }
 template <class R>
 __device__ __forceinline__ R init_uniform(const PathArgs& A, uint32_t nsp, uint32_t traj, int L) {
   const u4 x = philox10(u4{(uint32_t)L >> 2, 0u, traj, (uint32_t)A.inst}, nsp,
                         key1(TAG_UNIFORM_INIT, (uint32_t)A.seq));
-  const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
-  return uniform01<R>(xs[L & 3]);
+  const int l = L & 3;
+  return uniform01<R>(l == 0 ? x.x : l == 1 ? x.y : l == 2 ? x.z : x.w);
 }
 
 // built-in initial forms (user-level p.initial handles, xensemble.m:60)
@@ -111,7 +117,8 @@ __device__ __forceinline__ cpx<R> initial_value(const PathArgs& A, uint32_t nsp,
 
 // ===================================================================== SDE
 template <class R>
-__global__ __launch_bounds__(256) void sde_path(const PathArgs A) {
+__global__ __launch_bounds__(256) void sde_path(const PathArgs* __restrict__ Ap) {
+  const PathArgs& A = *Ap;
   using C = cpx<R>;
   __shared__ double red[8];
   int pidx, nci, j, group;
@@ -119,10 +126,10 @@ __global__ __launch_bounds__(256) void sde_path(const PathArgs A) {
   const int traj = group * 256 + threadIdx.x;
   const bool active = traj < A.E;
   const uint32_t nsp = (uint32_t)(A.seed + A.npar + (j - 1) * A.e3);
-  const R hdt = (R)A.hdt[nci];
+  const R hdt = (R)(nci ? A.hdt[1] : A.hdt[0]);
   Drift<R> D;
   D.init(A);
-  const C P = A.has_linear ? static_cast<const C*>(A.px[nci])[0] : C{R(1), R(0)};
+  const C P = A.has_linear ? static_cast<const C*>(nci ? A.px[1] : A.px[0])[0] : C{R(1), R(0)};
   double* av = A.av + 2 * (long long)pidx * A.avlen;
 
   C a;
@@ -132,14 +139,14 @@ __global__ __launch_bounds__(256) void sde_path(const PathArgs A) {
   } else
     a = initial_value<R>(A, nsp, (uint32_t)traj, 0, 0, 0);
 
-  auto draw = [&](uint32_t q) -> C {
+  auto draw = [&](uint32_t q) __attribute__((always_inline)) -> C {
     const R sc = (R)A.noise_scale;
     if (A.noises <= 0) return C{R(0), R(0)};
     R n[4];
     normals4<R>(u4{0u, q, (uint32_t)traj, (uint32_t)A.inst}, nsp, key1(TAG_NOISE, (uint32_t)A.seq), n);
     return C{sc * n[0], A.noises >= 2 ? sc * n[1] : R(0)};
   };
-  auto step = [&](C xi) {
+  auto step = [&](C xi) __attribute__((always_inline)) {
     if (A.method == XSH_RK4) {
       C a1 = a + hdt * D(a, xi);
       a1 = P * a1;
@@ -158,8 +165,10 @@ __global__ __launch_bounds__(256) void sde_path(const PathArgs A) {
       a = P * a;
     }
   };
-  auto observe = [&](int n) {
-    for (int o = 0; o < A.nobs; ++o) {
+  auto observe = [&](int n) __attribute__((always_inline)) {
+#pragma unroll
+    for (int o = 0; o < XSH_MAX_OBS; ++o) {
+      if (o >= A.nobs) break;
       double re = 0.0, im = 0.0;
       if (active) {
         switch (A.obs[o]) {
@@ -205,7 +214,8 @@ __global__ __launch_bounds__(256) void sde_path(const PathArgs A) {
 
 // ==================================================================== SPDE
 template <class R, int NX, int NY, int NT, int TPB>
-__global__ __launch_bounds__(NT) void spde_path(const PathArgs A) {
+__global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap) {
+  const PathArgs& A = *Ap;
   using Lat = Lattice<R, NX, NY, NT, TPB>;
   using C = cpx<R>;
   constexpr int CPT = Lat::CPT, PPT = Lat::PPT, NS = Lat::NS;
@@ -215,10 +225,11 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs A) {
   int pidx, nci, j, group;
   path_identity<R>(A, blockIdx.x, pidx, nci, j, group);
   const uint32_t nsp = (uint32_t)(A.seed + A.npar + (j - 1) * A.e3);
-  const R hdt = (R)A.hdt[nci];
+  const R hdt = (R)(nci ? A.hdt[1] : A.hdt[0]);
   Drift<R> D;
   D.init(A);
-  const PropOp<R> prop{static_cast<const C*>(A.px[nci]), NY > 1 ? static_cast<const C*>(A.py[nci]) : nullptr};
+  const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
+                       NY > 1 ? static_cast<const C*>(nci ? A.py[1] : A.py[0]) : nullptr};
   const C* twx = static_cast<const C*>(A.twx);
   const C* twy = static_cast<const C*>(A.twy);
   double* av = A.av + 2 * (long long)pidx * A.avlen;
@@ -242,7 +253,7 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs A) {
     }
   }
 
-  auto gen_noise = [&](uint32_t q, bool coarse) {
+  auto gen_noise = [&](uint32_t q, bool coarse) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < CPT; ++i) {
       const int pt = Lat::point(i, 0);
@@ -265,7 +276,7 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs A) {
     if (A.has_linear) Lat::spectral(L, f, twx, twy, prop);
   };
 
-  auto method_step = [&]() {
+  auto method_step = [&]() __attribute__((always_inline)) {
     if (A.method == XSH_RK4) {
       // RK4.m:12-38 with the two propagations of each pair batched
 #pragma unroll
@@ -299,8 +310,10 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs A) {
     }
   };
 
-  auto observe = [&](int n) {
-    for (int o = 0; o < A.nobs; ++o) {
+  auto observe = [&](int n) __attribute__((always_inline)) {
+#pragma unroll
+    for (int o = 0; o < XSH_MAX_OBS; ++o) {
+      if (o >= A.nobs) break;
       const int kind = A.obs[o];
       double* dst = av + 2 * A.obs_off[o];
       if (kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR) {
@@ -391,7 +404,8 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs A) {
 
 // propagation probe: planes [nplanes][NS] complex R in place
 template <class R, int NX, int NY, int NT, int TPB>
-__global__ __launch_bounds__(NT) void probe_propagate(const PathArgs A, int nci, int nplanes, cpx<R>* planes) {
+__global__ __launch_bounds__(NT) void probe_propagate(const PathArgs* __restrict__ Ap, int nci, int nplanes, cpx<R>* planes) {
+  const PathArgs& A = *Ap;
   using Lat = Lattice<R, NX, NY, NT, TPB>;
   using C = cpx<R>;
   __shared__ C L[Lat::LDS];
@@ -404,7 +418,8 @@ __global__ __launch_bounds__(NT) void probe_propagate(const PathArgs A, int nci,
       const long long g = base + Lat::point(i, v);
       f[2 * i + v] = g < (long long)nplanes * Lat::NS ? planes[g] : C{R(0), R(0)};
     }
-  const PropOp<R> prop{static_cast<const C*>(A.px[nci]), NY > 1 ? static_cast<const C*>(A.py[nci]) : nullptr};
+  const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
+                       NY > 1 ? static_cast<const C*>(nci ? A.py[1] : A.py[0]) : nullptr};
   Lat::spectral(L, f, static_cast<const C*>(A.twx), static_cast<const C*>(A.twy), prop);
 #pragma unroll
   for (int i = 0; i < Lat::CPT; ++i)
@@ -430,19 +445,19 @@ __global__ void probe_normals(uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t
 
 // ================================================================ launchers
 template <class R>
-static hipError_t launch_sde(const PathArgs& A, int nblocks, hipStream_t s) {
+static hipError_t launch_sde(const PathArgs* A, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(sde_path<R>, dim3(nblocks), dim3(256), 0, s, A);
   return hipGetLastError();
 }
 
 template <class R, int NX, int NY, int NT, int TPB>
-static hipError_t launch_spde(const PathArgs& A, int nblocks, hipStream_t s) {
+static hipError_t launch_spde(const PathArgs* A, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL((spde_path<R, NX, NY, NT, TPB>), dim3(nblocks), dim3(NT), 0, s, A);
   return hipGetLastError();
 }
 
 template <class R, int NX, int NY, int NT, int TPB>
-static hipError_t launch_probe(const PathArgs& A, int nci, int nplanes, void* planes, hipStream_t s) {
+static hipError_t launch_probe(const PathArgs* A, int nci, int nplanes, void* planes, hipStream_t s) {
   const int nb = (nplanes + TPB - 1) / TPB;
   hipLaunchKernelGGL((probe_propagate<R, NX, NY, NT, TPB>), dim3(nb), dim3(NT), 0, s, A, nci, nplanes,
                      static_cast<cpx<R>*>(planes));
@@ -475,7 +490,7 @@ const KernelEntry* find_kernel(int precision, int nx, int ny) {
   return nullptr;
 }
 
-hipError_t launch_sde_path(int precision, const PathArgs& A, int nblocks, hipStream_t s) {
+hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s) {
   return precision == XSH_FP64 ? launch_sde<double>(A, nblocks, s) : launch_sde<float>(A, nblocks, s);
 }
 

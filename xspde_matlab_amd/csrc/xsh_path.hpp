@@ -185,6 +185,7 @@ struct Lattice {
   __device__ static __forceinline__ void dif1(C* L, const C* tw) {
     using Ln = Line<DIM>;
     constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+#pragma unroll 1
     for (int it = threadIdx.x; it < Ln::NL * N2; it += NT) {
       const int l = it % Ln::NL, n2 = it / Ln::NL;
       const int b = Ln::base(l);
@@ -203,6 +204,7 @@ struct Lattice {
   __device__ static __forceinline__ void dif2(C* L) {
     using Ln = Line<DIM>;
     constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+#pragma unroll 1
     for (int it = threadIdx.x; it < Ln::NL * N1; it += NT) {
       const int l = it % Ln::NL, k1 = it / Ln::NL;
       const int b = Ln::base(l) + k1 * N2 * Ln::ES;
@@ -219,6 +221,7 @@ struct Lattice {
   __device__ static __forceinline__ void dita(C* L, const C* tw) {
     using Ln = Line<DIM>;
     constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+#pragma unroll 1
     for (int it = threadIdx.x; it < Ln::NL * N1; it += NT) {
       const int l = it % Ln::NL, k1 = it / Ln::NL;
       const int b = Ln::base(l) + k1 * N2 * Ln::ES;
@@ -237,6 +240,7 @@ struct Lattice {
   __device__ static __forceinline__ void mid(C* L, const C* tw, const Op& op) {
     using Ln = Line<DIM>;
     constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+#pragma unroll 1
     for (int it = threadIdx.x; it < Ln::NL * N1; it += NT) {
       const int l = it % Ln::NL, k1 = it / Ln::NL;
       const int b = Ln::base(l) + k1 * N2 * Ln::ES;
@@ -258,6 +262,7 @@ struct Lattice {
   __device__ static __forceinline__ void ditb(C* L) {
     using Ln = Line<DIM>;
     constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+#pragma unroll 1
     for (int it = threadIdx.x; it < Ln::NL * N2; it += NT) {
       const int l = it % Ln::NL, n2 = it / Ln::NL;
       const int b = Ln::base(l);
@@ -286,7 +291,7 @@ struct Lattice {
     if constexpr (NY == 1) {
       dif1<0>(L, twx);
       __syncthreads();
-      mid<0>(L, twx, [&](C X, int l, int k) { return op(X, k, 0); });
+      mid<0>(L, twx, [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, k, 0); });
       __syncthreads();
       ditb<0>(L);
     } else {
@@ -296,7 +301,7 @@ struct Lattice {
       __syncthreads();
       dif1<1>(L, twy);
       __syncthreads();
-      mid<1>(L, twy, [&](C X, int l, int k) { return op(X, kx_of_pos(l % NX), k); });
+      mid<1>(L, twy, [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, kx_of_pos(l % NX), k); });
       __syncthreads();
       ditb<1>(L);
       __syncthreads();
