@@ -234,15 +234,19 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
   const C* twy = static_cast<const C*>(A.twy);
   double* av = A.av + 2 * (long long)pidx * A.avlen;
 
-  C f0[PPT], f1[PPT], xi[PPT];
+  C f0[PPT], f1[PPT];
+  // Every loop that indexes f0/f1/xi is a compile-time static_for: a runtime
+  // index would demote the arrays from VGPRs to scratch memory.
+#define XSH_FOR(VAR, B, E) static_for<B, E>([&](auto VAR##_) __attribute__((always_inline)) { \
+    constexpr int VAR = decltype(VAR##_)::value;
+#define XSH_END });
+
   // ---- initial fields (xensemble.m:53-65)
-#pragma unroll
-  for (int i = 0; i < CPT; ++i) {
+  XSH_FOR(i, 0, CPT)
     const int pt = Lat::point(i, 0);
     const int p = pt / NS, s = pt % NS;
     const int traj = group * TPB + p;
-#pragma unroll
-    for (int v = 0; v < 2; ++v) {
+    XSH_FOR(v, 0, 2)
       const int site = s + v, iy = site / NX, ix = site % NX;
       if (A.a0) {
         const long long g = (long long)(traj < A.E ? traj : 0) * NS + site;
@@ -250,115 +254,136 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
       } else {
         f0[2 * i + v] = initial_value<R>(A, nsp, (uint32_t)traj, site, ix, iy);
       }
-    }
-  }
+      __builtin_amdgcn_sched_barrier(0);
+    XSH_END
+  XSH_END
 
-  auto gen_noise = [&](uint32_t q, bool coarse) __attribute__((always_inline)) {
-#pragma unroll
-    for (int i = 0; i < CPT; ++i) {
-      const int pt = Lat::point(i, 0);
-      const int p = pt / NS, s = pt % NS;
-      const uint32_t traj = (uint32_t)(group * TPB + p);
-      C w0, w1;
-      draw_pair<R>(A, nsp, q, traj, s, w0, w1);
-      if (coarse) {
-        C u0, u1;
-        draw_pair<R>(A, nsp, q + 1, traj, s, u0, u1);
-        w0 = R(0.5) * (w0 + u0);
-        w1 = R(0.5) * (w1 + u1);
-      }
-      xi[2 * i] = w0;
-      xi[2 * i + 1] = w1;
+  // Noise of draw q (coarse: mean of draws q, q+1, xnoise.m:140-141) for
+  // register chunk i.  It is regenerated from the counter at each of the
+  // method's drift stages instead of being held for the whole step: that keeps
+  // a single field resident during every transform, the peak of the register
+  // budget.
+  auto noise_chunk = [&](int i, uint32_t q, bool coarse, C& w0, C& w1) __attribute__((always_inline)) {
+    const int pt = Lat::point(i, 0);
+    const int p = pt / NS, s = pt % NS;
+    const uint32_t traj = (uint32_t)(group * TPB + p);
+    draw_pair<R>(A, nsp, q, traj, s, w0, w1);
+    if (coarse) {
+      C u0, u1;
+      draw_pair<R>(A, nsp, q + 1, traj, s, u0, u1);
+      w0 = R(0.5) * (w0 + u0);
+      w1 = R(0.5) * (w1 + u1);
     }
   };
 
-  auto propagate = [&](C (&f)[PPT]) {
+  auto propagate = [&](C (&f)[PPT]) __attribute__((always_inline)) {
     if (A.has_linear) Lat::spectral(L, f, twx, twy, prop);
   };
 
-  auto method_step = [&]() __attribute__((always_inline)) {
+  auto method_step = [&](uint32_t q, bool coarse) __attribute__((always_inline)) {
     if (A.method == XSH_RK4) {
       // RK4.m:12-38 with the two propagations of each pair batched
-#pragma unroll
-      for (int k = 0; k < PPT; ++k) f1[k] = f0[k] + hdt * D(f0[k], xi[k]);
+      XSH_FOR(i, 0, CPT)
+        C w[2];
+        noise_chunk(i, q, coarse, w[0], w[1]);
+        XSH_FOR(v, 0, 2)
+          constexpr int k = 2 * i + v;
+          f1[k] = f0[k] + hdt * D(f0[k], w[v]);
+        XSH_END
+        __builtin_amdgcn_sched_barrier(0);
+      XSH_END
       propagate(f0);  // a  = P a
       propagate(f1);  // a1 = P a1
-#pragma unroll
-      for (int k = 0; k < PPT; ++k) {
-        const C A0 = f0[k], A1 = f1[k];
-        const C a2 = A0 + hdt * D(A1, xi[k]);
-        const C a3 = A0 + (R(2) * hdt) * D(a2, xi[k]);
-        f0[k] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
-        f1[k] = a3;
-      }
+      XSH_FOR(i, 0, CPT)
+        C w[2];
+        noise_chunk(i, q, coarse, w[0], w[1]);
+        XSH_FOR(v, 0, 2)
+          constexpr int k = 2 * i + v;
+          const C A0 = f0[k], A1 = f1[k];
+          const C a2 = A0 + hdt * D(A1, w[v]);
+          const C a3 = A0 + (R(2) * hdt) * D(a2, w[v]);
+          f0[k] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
+          f1[k] = a3;
+        XSH_END
+        __builtin_amdgcn_sched_barrier(0);
+      XSH_END
       propagate(f0);
       propagate(f1);
-#pragma unroll
-      for (int k = 0; k < PPT; ++k) f0[k] = f0[k] + (hdt / R(3)) * D(f1[k], xi[k]);
+      XSH_FOR(i, 0, CPT)
+        C w[2];
+        noise_chunk(i, q, coarse, w[0], w[1]);
+        XSH_FOR(v, 0, 2)
+          constexpr int k = 2 * i + v;
+          f0[k] = f0[k] + (hdt / R(3)) * D(f1[k], w[v]);
+        XSH_END
+        __builtin_amdgcn_sched_barrier(0);
+      XSH_END
     } else {
-      // MP.m:53-69
+      // MP.m:53-69; the fixed-point passes run chunk by chunk
       propagate(f0);
-#pragma unroll
-      for (int k = 0; k < PPT; ++k) f1[k] = f0[k];
-      for (int it = 0; it < A.iterations; ++it) {
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) f0[k] = f1[k] + hdt * D(f0[k], xi[k]);
-      }
-#pragma unroll
-      for (int k = 0; k < PPT; ++k) f0[k] = R(2) * f0[k] - f1[k];
+      XSH_FOR(i, 0, CPT)
+        C w[2];
+        noise_chunk(i, q, coarse, w[0], w[1]);
+        XSH_FOR(v, 0, 2)
+          constexpr int k = 2 * i + v;
+          const C a0 = f0[k];
+          C a = a0;
+          for (int it = 0; it < A.iterations; ++it) a = a0 + hdt * D(a, w[v]);
+          f0[k] = R(2) * a - a0;
+        XSH_END
+        __builtin_amdgcn_sched_barrier(0);
+      XSH_END
       propagate(f0);
     }
   };
 
   auto observe = [&](int n) __attribute__((always_inline)) {
-#pragma unroll
-    for (int o = 0; o < XSH_MAX_OBS; ++o) {
-      if (o >= A.nobs) break;
+    for (int o = 0; o < A.nobs; ++o) {
       const int kind = A.obs[o];
       double* dst = av + 2 * A.obs_off[o];
       if (kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR) {
         if (kind == XSH_OBS_CORR) {
-#pragma unroll
-          for (int k = 0; k < PPT; ++k) f1[k] = f0[k];
+          XSH_FOR(k, 0, PPT) f1[k] = f0[k]; XSH_END
           Lat::spectral(L, f1, twx, twy, CorrOp<R>{R(1.0 / ((double)NS * (double)NS))});
         }
-#pragma unroll
-        for (int i = 0; i < CPT; ++i) {
+        XSH_FOR(i, 0, CPT)
           const int pt = Lat::point(i, 0);
           const int p = pt / NS, s = pt % NS;
-          if (group * TPB + p >= A.E) continue;
-#pragma unroll
-          for (int v = 0; v < 2; ++v) {
-            const C a = kind == XSH_OBS_CORR ? f1[2 * i + v] : f0[2 * i + v];
-            double re = (double)a.x, im = (double)a.y;
-            if (kind == XSH_OBS_ABS2) {
-              re = (double)a.x * a.x + (double)a.y * a.y;
-              im = 0.0;
-            }
-            atomic_add_c(dst + 2 * ((long long)n * NS + s + v), re * A.inv_E, im * A.inv_E);
+          if (group * TPB + p < A.E) {
+            XSH_FOR(v, 0, 2)
+              // select values, not array addresses: a pointer select would keep
+              // f0 and f1 in scratch memory instead of VGPRs
+              const C a0v = f0[2 * i + v], a1v = f1[2 * i + v];
+              const C a = kind == XSH_OBS_CORR ? a1v : a0v;
+              double re = (double)a.x, im = (double)a.y;
+              if (kind == XSH_OBS_ABS2) {
+                re = (double)a.x * a.x + (double)a.y * a.y;
+                im = 0.0;
+              }
+              atomic_add_c(dst + 2 * ((long long)n * NS + s + v), re * A.inv_E, im * A.inv_E);
+            XSH_END
           }
-        }
+        XSH_END
       } else {
         double re = 0.0, im = 0.0;
-#pragma unroll
-        for (int i = 0; i < CPT; ++i) {
+        XSH_FOR(i, 0, CPT)
           const int pt = Lat::point(i, 0);
           const int p = pt / NS;
-          if (group * TPB + p >= A.E) continue;
-#pragma unroll
-          for (int v = 0; v < 2; ++v) {
-            const C a = f0[2 * i + v];
-            const double m = (double)a.x * a.x + (double)a.y * a.y;
-            if (kind == XSH_OBS_A_AVE) {
-              re += (double)a.x;
-              im += (double)a.y;
-            } else if (kind == XSH_OBS_ABS4_AVE) {
-              re += m * m;
-            } else {
-              re += m;
-            }
+          if (group * TPB + p < A.E) {
+            XSH_FOR(v, 0, 2)
+              const C a = f0[2 * i + v];
+              const double m = (double)a.x * a.x + (double)a.y * a.y;
+              if (kind == XSH_OBS_A_AVE) {
+                re += (double)a.x;
+                im += (double)a.y;
+              } else if (kind == XSH_OBS_ABS4_AVE) {
+                re += m * m;
+              } else {
+                re += m;
+              }
+            XSH_END
           }
-        }
+        XSH_END
         block_sum2<NT>(re, im, red);
         const double sc = (kind == XSH_OBS_ABS2_INT ? A.dv : 1.0 / NS) * A.inv_E;
         if (threadIdx.x == 0) atomic_add_c(dst + 2 * (long long)n, re * sc, im * sc);
@@ -372,34 +397,31 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
     for (int st = 0; st < A.steps; ++st) {
       if (n > 0 || st > 0 || A.npts == 1) {
         if (A.checks && nci == 1) {
-          gen_noise(q, true);
+          method_step(q, true);
           q += 2;
-          method_step();
         } else {
-          for (int c = 0; c <= A.checks; ++c) {
-            gen_noise(q++, false);
-            method_step();
-          }
+          for (int c = 0; c <= A.checks; ++c) method_step(q++, false);
         }
       }
       if (st == 0) observe(n);
     }
   }
   if (A.a_out) {
-#pragma unroll
-    for (int i = 0; i < CPT; ++i) {
+    XSH_FOR(i, 0, CPT)
       const int pt = Lat::point(i, 0);
       const int p = pt / NS, s = pt % NS;
       const int traj = group * TPB + p;
-      if (traj >= A.E) continue;
-#pragma unroll
-      for (int v = 0; v < 2; ++v) {
-        const long long g = (long long)traj * NS + s + v;
-        A.a_out[2 * g] = (double)f0[2 * i + v].x;
-        A.a_out[2 * g + 1] = (double)f0[2 * i + v].y;
+      if (traj < A.E) {
+        XSH_FOR(v, 0, 2)
+          const long long g = (long long)traj * NS + s + v;
+          A.a_out[2 * g] = (double)f0[2 * i + v].x;
+          A.a_out[2 * g + 1] = (double)f0[2 * i + v].y;
+        XSH_END
       }
-    }
+    XSH_END
   }
+#undef XSH_FOR
+#undef XSH_END
 }
 
 // propagation probe: planes [nplanes][NS] complex R in place
@@ -474,7 +496,7 @@ static const KernelEntry kTable[] = {
     XSH_ENTRY(float, 128, 1, 256, 32), XSH_ENTRY(float, 256, 1, 256, 16),
     XSH_ENTRY(float, 512, 1, 256, 8), XSH_ENTRY(float, 1024, 1, 256, 4),
     // fp32, 2D lattices
-    XSH_ENTRY(float, 32, 32, 256, 1), XSH_ENTRY(float, 64, 64, 256, 1),
+    XSH_ENTRY(float, 32, 32, 256, 1), XSH_ENTRY(float, 64, 64, 512, 1),
     XSH_ENTRY(float, 128, 128, 512, 1),
     // fp64, 1D lattices
     XSH_ENTRY(double, 32, 1, 256, 16), XSH_ENTRY(double, 64, 1, 256, 16),

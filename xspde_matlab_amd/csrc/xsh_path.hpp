@@ -116,6 +116,15 @@ __device__ __forceinline__ void atomic_add_c(double* p, double re, double im) {
 // threads.  Field registers: chunk c = tid + NT*i (i < CPT) holds VEC=2
 // consecutive sites of one row.  LDS: plane p, row y, column x at
 // p*PS + y*RS + x with one pad element per row.
+// Thread index through an empty asm: address arithmetic derived from it is
+// recomputed where it is used instead of being hoisted out of the time loop
+// and held in VGPRs for the whole path (that hoisting caused register spills).
+__device__ __forceinline__ int opaque_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 template <int N>
 struct Split {  // N = N1 * N2, both <= 32, DIF radix order (N1 first)
   static constexpr int N1 = N <= 4 ? (N >= 2 ? 2 : 1)
@@ -139,7 +148,7 @@ struct Lattice {
   using C = cpx<R>;
 
   // point index of register slot (i, v)
-  __device__ static __forceinline__ int point(int i, int v) { return (threadIdx.x + NT * i) * VEC + v; }
+  __device__ static __forceinline__ int point(int i, int v) { return (opaque_tid() + NT * i) * VEC + v; }
   __device__ static __forceinline__ int lds_of(int pt) {
     const int p = pt / NS, s = pt - p * NS;
     const int y = s / NX, x = s - y * NX;
@@ -186,7 +195,7 @@ struct Lattice {
     using Ln = Line<DIM>;
     constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
 #pragma unroll 1
-    for (int it = threadIdx.x; it < Ln::NL * N2; it += NT) {
+    for (int it = opaque_tid(); it < Ln::NL * N2; it += NT) {
       const int l = it % Ln::NL, n2 = it / Ln::NL;
       const int b = Ln::base(l);
       C v[N1];
@@ -205,7 +214,7 @@ struct Lattice {
     using Ln = Line<DIM>;
     constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
 #pragma unroll 1
-    for (int it = threadIdx.x; it < Ln::NL * N1; it += NT) {
+    for (int it = opaque_tid(); it < Ln::NL * N1; it += NT) {
       const int l = it % Ln::NL, k1 = it / Ln::NL;
       const int b = Ln::base(l) + k1 * N2 * Ln::ES;
       C v[N2];
@@ -222,7 +231,7 @@ struct Lattice {
     using Ln = Line<DIM>;
     constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
 #pragma unroll 1
-    for (int it = threadIdx.x; it < Ln::NL * N1; it += NT) {
+    for (int it = opaque_tid(); it < Ln::NL * N1; it += NT) {
       const int l = it % Ln::NL, k1 = it / Ln::NL;
       const int b = Ln::base(l) + k1 * N2 * Ln::ES;
       C v[N2];
@@ -241,7 +250,7 @@ struct Lattice {
     using Ln = Line<DIM>;
     constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
 #pragma unroll 1
-    for (int it = threadIdx.x; it < Ln::NL * N1; it += NT) {
+    for (int it = opaque_tid(); it < Ln::NL * N1; it += NT) {
       const int l = it % Ln::NL, k1 = it / Ln::NL;
       const int b = Ln::base(l) + k1 * N2 * Ln::ES;
       C v[N2];
@@ -263,7 +272,7 @@ struct Lattice {
     using Ln = Line<DIM>;
     constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
 #pragma unroll 1
-    for (int it = threadIdx.x; it < Ln::NL * N2; it += NT) {
+    for (int it = opaque_tid(); it < Ln::NL * N2; it += NT) {
       const int l = it % Ln::NL, n2 = it / Ln::NL;
       const int b = Ln::base(l);
       C v[N1];
