@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--boundaries", default="periodic", choices=["periodic", "neumann"],
+                    help="periodic (FFT) or Neumann-Neumann (DCT-I) lattice")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs a single MI355X (--gpus %d not supported)"
@@ -69,6 +71,8 @@ def main():
     from xspde_matlab_amd.configs import cgl2d
 
     p = cgl2d(args.precision)
+    if args.boundaries == "neumann":
+        p["boundaries"] = [[-1, -1], [-1, -1]]
     log("bench: creating problem %s %s %s" % (p["name"], p["points"], p["gpu"]["precision"]))
     prob = Problem(p)
     info = prob.info
@@ -105,8 +109,9 @@ def main():
         "value": rate, "unit": "trajectory-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
-        "config": {"workload": "2D CGL SPDE 128x128 periodic, RK4IP+FFT, 1024 trajectories, "
-                               "2000 coarse steps with checks (6000 traj-steps/trajectory)",
+        "config": {"workload": "2D CGL SPDE 128x128 %s, RK4IP+%s, 1024 trajectories, "
+                               "2000 coarse steps with checks (6000 traj-steps/trajectory)"
+                               % (args.boundaries, "FFT" if args.boundaries == "periodic" else "DCT-I"),
                    "lattice": [128, 128], "trajectories": 1024, "ensembles": [64, 16, 1],
                    "points": 101, "steps": 20, "checks": 1, "traj_steps_per_step": traj_steps,
                    "precision": args.precision, "moments": "fp64", "finite": finite},

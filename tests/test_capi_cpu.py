@@ -52,7 +52,7 @@ def test_unsupported_forms_rejected():
     rc, msg = _create(d)
     assert rc == 2 and "drift" in msg
     d = make_desc(q)
-    d.nx = 200   # not a power of two
+    d.nx = 201   # odd lattice
     rc, msg = _create(d)
     assert rc == 2
     d = make_desc(q)

@@ -134,3 +134,56 @@ def test_ensemble_matches_oracle_fp64():
         ref = dat[o]
         # oracle per-site data are C-ordered [points, x]; identical for 1D
         _cmp(data_g[o][:, 0], ref[:, 0], 1e-10, "ensemble mean o=%d" % o)
+
+
+BOUNDS = {"NN": [-1, -1], "DD": [1, 1], "DN": [1, -1], "ND": [-1, 1], "PP": [0, 0]}
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 3e-6)])
+@pytest.mark.parametrize("shape,bnd", [((64,), ("NN",)), ((128,), ("DD",)), ((128,), ("DN",)), ((64,), ("ND",)),
+                                       ((256,), ("NN",)), ((64, 64), ("NN", "NN")), ((64, 64), ("PP", "NN")),
+                                       ((64, 64), ("DD", "ND")), ((128, 128), ("NN", "NN"))])
+def test_boundary_propagator_matches_oracle(prec, tol, shape, bnd):
+    """Dense DCT/DST (and DFT on mixed lattices) propagators against xprop/xtransform
+    (xtransform.m:208-245) for every boundary type of spdpreferences.m:56-67."""
+    if prec == "fp64" and shape in [(256,), (128, 128)]:
+        pytest.skip("no fp64 dense-transform kernel at this size")
+    nd = len(shape) + 1
+    b = [BOUNDS[x] for x in bnd]
+    p = dict(dimensions=nd, points=[3] + list(shape), ranges=[1.0] + [10.0] * len(shape), ensembles=[2, 1, 1],
+             noises=0, boundaries=b,
+             gpu=dict(deriv="zero", linear=dict(l0=0.3 - 0.2j, l2=0.4 + 0.9j), initial="const",
+                      observe=["a"], precision=prec))
+    prob = Problem(p)
+    rng = np.random.default_rng(11)
+    planes = rng.standard_normal((3,) + shape[::-1]) + 1j * rng.standard_normal((3,) + shape[::-1])
+    got = prob.handle.probe_propagate(1, planes.reshape(3, -1)).reshape(planes.shape)
+    q = X.xpreferences(oracle_problem(p))
+    q.dtr = q.dt / 2
+    prop = X.xpropfactor(q)
+    a = np.moveaxis(planes, 0, -1)
+    a = np.transpose(a, tuple(range(len(shape) - 1, -1, -1)) + (len(shape),))[None]
+    ref = X.xprop(a, q, prop)[0]
+    ref = np.moveaxis(np.transpose(ref, tuple(range(len(shape) - 1, -1, -1)) + (len(shape),)), -1, 0)
+    _cmp(got, ref, tol, "propagate %s %s %s" % (prec, shape, bnd))
+    prob.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 5e-5)])
+@pytest.mark.parametrize("method", ["RK4", "MP"])
+@pytest.mark.parametrize("bnd", ["NN", "DD", "DN"])
+def test_nls1d_boundary_path(prec, tol, method, bnd):
+    """Whole paths with non-periodic ends, including xsetbound's Dirichlet values."""
+    p = shrink(nls1d(prec, nx=128), E=64, e2=2, points=3, steps=2)
+    p["method"] = method
+    p["boundaries"] = [BOUNDS[bnd]]
+    _path_case(p, tol)
+
+
+@pytest.mark.parametrize("prec,tol,n", [("fp64", 1e-10, 64), ("fp32", 5e-5, 64), ("fp32", 5e-5, 128)])
+def test_cgl2d_neumann_path(prec, tol, n):
+    """Config 3-NN: 2D CGL with Neumann ends (DCT-I both dimensions)."""
+    p = shrink(cgl2d(prec, n=n), E=2, e2=2, points=3, steps=2)
+    p["boundaries"] = [[-1, -1], [-1, -1]]
+    p["gpu"]["observe"] = ["abs2_ave", "abs2"]
+    _path_case(p, tol)

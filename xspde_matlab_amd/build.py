@@ -15,7 +15,8 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libxspde_hip.so")
-SOURCES = ["xsh_kernels.hip", "xsh_capi.hip"]
+SOURCES = ["xsh_k_mat_f32.hip", "xsh_k_mat_f64.hip", "xsh_k_f32_2d.hip", "xsh_k_f64.hip",
+           "xsh_k_f32_1d.hip", "xsh_k_sde.hip", "xsh_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -37,7 +38,7 @@ def build(resource=False, verbose=False):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource else []
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
         results = list(ex.map(lambda s: _compile(s, objdir, extra), SOURCES))
     if resource or verbose:
         for _, out in results:

@@ -53,8 +53,11 @@ struct xsh_handle {
   long long obs_off[XSH_MAX_OBS];
   int obs_ns[XSH_MAX_OBS];
   size_t esize;
-  void* d_tables;   // px[2], py[2], twx, twy
-  size_t tab_off[6];
+  void* d_tables;   // px[2], py[2], twx, twy, transform matrices
+  size_t tab_off[10];
+  int btype[2];      // boundary type per space dimension (0 periodic)
+  int mat;           // dense per-dimension transforms in use
+  int cplx[2];       // per dimension: complex (DFT) matrix
   double* d_av;      // [e2*nchk][avlen] complex
   double* d_path_av; // [avlen] complex (single path)
   double* d_a;       // [E][ns] complex (single path in/out)
@@ -102,14 +105,12 @@ static int validate(const xsh_desc* d) {
   for (int k = 0; k < 2; ++k) {
     if (k < sdim) {
       if (n[k] < 2) return fail(XSH_EINVAL, "lattice size %d must be >= 2", n[k]);
-      for (int b = 0; b < 2; ++b) {
-        const int t = d->boundaries[k][b];
-        if (t != 0 && t != 1 && t != -1) return fail(XSH_EINVAL, "Invalid boundary: [%d,%d]", d->boundaries[k][0], d->boundaries[k][1]);
-      }
-      if (d->boundaries[k][0] != 0 || d->boundaries[k][1] != 0)
-        return fail(XSH_EUNSUPPORTED, "non-periodic boundaries [%d,%d] are not on the GPU path yet",
-                    d->boundaries[k][0], d->boundaries[k][1]);
-      if (!is_pow2(n[k])) return fail(XSH_EUNSUPPORTED, "lattice size %d is not a power of two", n[k]);
+      const int b0 = d->boundaries[k][0], b1 = d->boundaries[k][1];
+      if (b0 < -1 || b0 > 1 || b1 < -1 || b1 > 1) return fail(XSH_EINVAL, "Invalid boundary: [%d,%d]", b0, b1);
+      const int t = b0 + 2 * b1;  // type index of spdpreferences.m:53-54
+      if (t != -3 && t != -1 && t != 0 && t != 1 && t != 3)
+        return fail(XSH_EINVAL, "Invalid boundary type: [%d,%d]", b0, b1);
+      if (n[k] % 2) return fail(XSH_EUNSUPPORTED, "odd lattice size %d is not on the GPU path", n[k]);
       if (!(d->ranges[k + 1] > 0)) return fail(XSH_EINVAL, "ranges(%d) must be > 0", k + 2);
     } else if (n[k] != 1) {
       return fail(XSH_EINVAL, "lattice size beyond dimensions must be 1");
@@ -126,6 +127,67 @@ static void pack(std::vector<char>& buf, size_t off, const std::vector<cd>& v) {
     p[2 * i] = (R)v[i].real();
     p[2 * i + 1] = (R)v[i].imag();
   }
+}
+
+template <class R>
+static void pack_real(std::vector<char>& buf, size_t off, const std::vector<double>& v) {
+  R* p = reinterpret_cast<R*>(buf.data() + off);
+  for (size_t i = 0; i < v.size(); ++i) p[i] = (R)v[i];
+}
+
+// Matrices of the reference's spectral transforms on N points, M = N - 1
+// intervals, row-major T[k*N + n] so that out[k] = sum_n T[k][n] in[n]:
+//   DCT1 xdct1.m:14-20   (1/sqrt(2M)) c_n cos(pi n k / M), c_0 = c_M = 1, else 2
+//   DST1 xdst1.m:8-18    sqrt(2/M) sin(pi n k / M), interior n and k only
+//   DCT2 xdct2.m:15-29   sqrt(2/M) cos(pi k (n+1/2) / M), n, k < M
+//   DCT3 xdct3.m:17-33   sqrt(2/M) [n = 0: 1/2, else cos(pi n (k+1/2) / M)], n, k < M
+//   DST2 xdst2.m:17-33   sqrt(2/M) sin(pi (n-1/2) k / M), n, k >= 1
+//   DST3 xdst3.m:16-35   sqrt(2/M) [n = M: (-1)^(k-1)/2, else sin(pi n (k-1/2) / M)], n, k >= 1
+enum { XT_DCT1 = 1, XT_DCT2, XT_DCT3, XT_DST1, XT_DST2, XT_DST3 };
+static std::vector<double> trig_matrix(int kind, int N) {
+  const int M = N - 1;
+  std::vector<double> T((size_t)N * N, 0.0);
+  const double s = std::sqrt(2.0 / M);
+  for (int k = 0; k < N; ++k)
+    for (int n = 0; n < N; ++n) {
+      double v = 0.0;
+      switch (kind) {
+        case XT_DCT1:
+          v = (n == 0 || n == M ? 1.0 : 2.0) * std::cos(M_PI * n * k / M) / std::sqrt(2.0 * M);
+          break;
+        case XT_DST1:
+          if (n >= 1 && n <= M - 1 && k >= 1 && k <= M - 1) v = s * std::sin(M_PI * n * k / M);
+          break;
+        case XT_DCT2:
+          if (n < M && k < M) v = s * std::cos(M_PI * k * (n + 0.5) / M);
+          break;
+        case XT_DCT3:
+          if (n < M && k < M) v = s * (n == 0 ? 0.5 : std::cos(M_PI * n * (k + 0.5) / M));
+          break;
+        case XT_DST2:
+          if (n >= 1 && k >= 1) v = s * std::sin(M_PI * (n - 0.5) * k / M);
+          break;
+        case XT_DST3:
+          if (n >= 1 && k >= 1) v = s * (n == M ? 0.5 * ((k - 1) % 2 ? -1.0 : 1.0) : std::sin(M_PI * n * (k - 0.5) / M));
+          break;
+      }
+      T[(size_t)k * N + n] = v;
+    }
+  return T;
+}
+// DFT on N points as an interleaved complex matrix: forward exp(-2 pi i nk/N),
+// inverse exp(+2 pi i nk/N)/N (fft / ifft of xtransform.m:220,235)
+static std::vector<double> dft_matrix(int N, bool inverse) {
+  std::vector<double> T(2 * (size_t)N * N);
+  for (int k = 0; k < N; ++k)
+    for (int n = 0; n < N; ++n) {
+      const long long m = ((long long)n * k) % N;
+      cd w = std::polar(1.0, (inverse ? 2.0 : -2.0) * M_PI * m / N);
+      if (inverse) w /= (double)N;
+      T[2 * ((size_t)k * N + n)] = w.real();
+      T[2 * ((size_t)k * N + n) + 1] = w.imag();
+    }
+  return T;
 }
 
 extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
@@ -163,11 +225,19 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   h->esize = d.precision == XSH_FP64 ? 16 : 8;
 
   if (d.dimensions > 1) {
-    h->kern = xsh::find_kernel(d.precision, h->nx, h->ny);
+    for (int k = 0; k < d.dimensions - 1; ++k) {
+      h->btype[k] = d.boundaries[k][0] + 2 * d.boundaries[k][1];
+      if (h->btype[k] != 0) h->mat = 1;
+    }
+    h->kern = xsh::find_kernel(d.precision, h->nx, h->ny, h->mat);
+    if (!h->kern && !h->mat) {  // periodic lattice without an FFT kernel: dense DFT
+      h->mat = 1;
+      h->kern = xsh::find_kernel(d.precision, h->nx, h->ny, 1);
+    }
     if (!h->kern) {
       delete h;
-      return fail(XSH_EUNSUPPORTED, "no %s kernel for a %dx%d lattice", d.precision == XSH_FP64 ? "fp64" : "fp32",
-                  npt[0], npt[1]);
+      return fail(XSH_EUNSUPPORTED, "no %s %s kernel for a %dx%d lattice", d.precision == XSH_FP64 ? "fp64" : "fp32",
+                  h->mat ? "non-periodic" : "periodic", npt[0], npt[1]);
     }
     h->threads = h->kern->threads;
     h->tpb = h->kern->tpb;
@@ -190,23 +260,36 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
 
   // tables: per nc the separable propagator factors (xpropfactor.m:145-156):
   //   P(kx,ky) = exp(dt l0) exp(dt (l1x i kx + l2x (i kx)^2)) exp(dt (l1y i ky + l2y (i ky)^2))
-  // with dt = dtr/ipsteps, ipsteps = 2, k in FFT order (spdpreferences.m:61-62),
-  // and the inverse-FFT 1/N folded in; then the twiddles W_N^m.
+  // with dt = dtr/ipsteps, ipsteps = 2 and the k-grid of each boundary type
+  // (spdpreferences.m:56-67).  FFT lattices: k in FFT order, inverse-FFT 1/N
+  // folded in, then the twiddles W_N^m.  Dense-transform lattices: k in the
+  // natural order of the transform, normalisation inside the matrices.
   const cd l0(d.lin_l0[0], d.lin_l0[1]);
   std::vector<std::vector<cd>> tabs(6);
   for (int nci = 0; nci < 2; ++nci) {
     const double dtp = h->dtr[nci] / 2.0;
     for (int k = 0; k < 2; ++k) {
       const int n = npt[k];
+      const bool sp = k < d.dimensions - 1;
       std::vector<cd> t(n);
       const cd l1(d.lin_l1[k][0], d.lin_l1[k][1]), l2(d.lin_l2[k][0], d.lin_l2[k][1]);
       const double dk = 2.0 * M_PI / (n * h->dx[k]);
+      const double dkp = n > 1 ? M_PI / ((n - 1) * h->dx[k]) : 0.0;
       for (int m = 0; m < n; ++m) {
-        const int f = m < (n + 1) / 2 ? m : m - n;  // ifftshift((-N/2..N/2-1)) order
-        const double kk = f * dk;
+        double kk = 0.0;
+        if (sp) {
+          switch (h->btype[k]) {
+            case 0: kk = (m < (n + 1) / 2 ? m : m - n) * dk; break;  // ifftshift((-N/2..N/2-1))
+            case -3:
+            case 3: kk = m * dkp; break;
+            case -1: kk = m == 0 ? 0.0 : (m - 0.5) * dkp; break;
+            case 1: kk = m == n - 1 ? 0.0 : (m + 0.5) * dkp; break;
+          }
+        }
         cd L = l1 * cd(0, kk) + l2 * cd(-kk * kk, 0);
         if (k == 0) L += l0;
-        t[m] = (k < d.dimensions - 1) ? std::exp(L * dtp) / (double)n : std::exp(L * dtp);
+        t[m] = std::exp(L * dtp);
+        if (sp && !h->mat) t[m] /= (double)n;
       }
       tabs[2 * nci + k] = t;
     }
@@ -217,17 +300,49 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     for (int m = 0; m < n; ++m) t[m] = std::polar(1.0, -2.0 * M_PI * m / n);
     tabs[4 + k] = t;
   }
+  // dense transforms (xtransform.m:208-245): forward x, forward y, inverse x, inverse y;
+  // real matrices are stored with a zero imaginary part dropped (one value per entry)
+  std::vector<std::vector<double>> mats(4);
+  int cplx[2] = {0, 0};
+  if (h->mat) {
+    for (int k = 0; k < d.dimensions - 1; ++k) {
+      const int n = npt[k];
+      const int t = h->btype[k];
+      if (t == 0) {
+        cplx[k] = 1;
+        mats[k] = dft_matrix(n, false);
+        mats[2 + k] = dft_matrix(n, true);
+      } else {
+        const int fwd = t == -3 ? XT_DCT1 : t == -1 ? XT_DST3 : t == 1 ? XT_DCT3 : XT_DST1;
+        const int inv = t == -3 ? XT_DCT1 : t == -1 ? XT_DST2 : t == 1 ? XT_DCT2 : XT_DST1;
+        mats[k] = trig_matrix(fwd, n);
+        mats[2 + k] = trig_matrix(inv, n);
+      }
+    }
+  }
+  h->cplx[0] = cplx[0];
+  h->cplx[1] = cplx[1];
   size_t total = 0;
   for (int i = 0; i < 6; ++i) {
     h->tab_off[i] = total;
     total += ((tabs[i].size() * h->esize + 255) / 256) * 256;
   }
-  std::vector<char> host(total);
+  for (int i = 0; i < 4; ++i) {
+    h->tab_off[6 + i] = total;
+    total += ((mats[i].size() * (h->esize / 2) + 255) / 256) * 256;
+  }
+  std::vector<char> host(total + 256);
   for (int i = 0; i < 6; ++i) {
     if (d.precision == XSH_FP64)
       pack<double>(host, h->tab_off[i], tabs[i]);
     else
       pack<float>(host, h->tab_off[i], tabs[i]);
+  }
+  for (int i = 0; i < 4; ++i) {
+    if (d.precision == XSH_FP64)
+      pack_real<double>(host, h->tab_off[6 + i], mats[i]);
+    else
+      pack_real<float>(host, h->tab_off[6 + i], mats[i]);
   }
   auto cleanup = [&](int code) {
     xsh_destroy(h);
@@ -337,6 +452,14 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
   A.py[1] = t + h->tab_off[3];
   A.twx = t + h->tab_off[4];
   A.twy = t + h->tab_off[5];
+  A.mat = h->mat;
+  for (int k = 0; k < 2; ++k) {
+    A.cplx[k] = h->cplx[k];
+    A.dir_lo[k] = k < d.dimensions - 1 && d.boundaries[k][0] == 1;
+    A.dir_hi[k] = k < d.dimensions - 1 && d.boundaries[k][1] == 1;
+    if (A.dir_lo[k] || A.dir_hi[k]) A.dirichlet = 1;
+  }
+  for (int i = 0; i < 4; ++i) A.tm[i] = t + h->tab_off[6 + i];
   return A;
 }
 

@@ -1,0 +1,41 @@
+// SDE kernels, noise probe, and the lookup over every family's launch table.
+#include "xsh_kernels.hpp"
+
+namespace xsh {
+
+const KernelEntry* table_f32_1d(int* n);
+const KernelEntry* table_f32_2d(int* n);
+const KernelEntry* table_f64(int* n);
+const KernelEntry* table_mat_f32(int* n);
+const KernelEntry* table_mat_f64(int* n);
+
+const KernelEntry* find_kernel(int precision, int nx, int ny, int mat) {
+  const KernelEntry* (*tables[])(int*) = {table_f32_1d, table_f32_2d, table_f64, table_mat_f32, table_mat_f64};
+  for (auto tab : tables) {
+    int n = 0;
+    const KernelEntry* e = tab(&n);
+    for (int i = 0; i < n; ++i)
+      if (e[i].precision == precision && e[i].nx == nx && e[i].ny == ny && e[i].mat == mat) return &e[i];
+  }
+  return nullptr;
+}
+
+hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s) {
+  return precision == XSH_FP64 ? launch_sde<double>(A, nblocks, s) : launch_sde<float>(A, nblocks, s);
+}
+
+hipError_t launch_probe_normals(int precision, uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t inst,
+                                uint32_t draw, int traj0, int ntraj, int count, double* out, hipStream_t s) {
+  const int groups = (count + 3) / 4;
+  const int total = ntraj * groups;
+  const int nb = (total + 255) / 256;
+  if (precision == XSH_FP64)
+    hipLaunchKernelGGL(probe_normals<double>, dim3(nb), dim3(256), 0, s, nsp, tag, seq, inst, draw, traj0,
+                       ntraj, count, out);
+  else
+    hipLaunchKernelGGL(probe_normals<float>, dim3(nb), dim3(256), 0, s, nsp, tag, seq, inst, draw, traj0,
+                       ntraj, count, out);
+  return hipGetLastError();
+}
+
+}  // namespace xsh

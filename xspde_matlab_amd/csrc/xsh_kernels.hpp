@@ -1,4 +1,6 @@
-// Path kernels and their launch table.
+// Path kernel templates and their launch helpers.  Each xsh_k_*.hip
+// translation unit instantiates one family and exports its launch table, so
+// the families compile in parallel.
 //
 //  sde_path    dimensions = 1 (no space): one trajectory per lane, the whole
 //              path in registers (xpath.m:29-86 with RK4.m / MP.m on a scalar
@@ -6,6 +8,7 @@
 //  spde_path   dimensions = 2, 3 on periodic lattices: one workgroup per TPB
 //              trajectories, fields resident in VGPRs for the whole path.
 //  probes      normals and propagation of given planes (parity hooks).
+#pragma once
 #include "xsh_path.hpp"
 #include "xsh_launch.hpp"
 
@@ -213,7 +216,7 @@ __global__ __launch_bounds__(256) void sde_path(const PathArgs* __restrict__ Ap)
 }
 
 // ==================================================================== SPDE
-template <class R, int NX, int NY, int NT, int TPB>
+template <class R, int NX, int NY, int NT, int TPB, bool MAT>
 __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap) {
   const PathArgs& A = *Ap;
   using Lat = Lattice<R, NX, NY, NT, TPB>;
@@ -276,13 +279,48 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
     }
   };
 
+  // Dirichlet ends of the lattice (xsetbound.m:102-116, default boundary value 0)
+  auto dir_site = [&](int i, int v) __attribute__((always_inline)) -> bool {
+    const int site = Lat::point(i, v) % NS, iy = site / NX, ix = site % NX;
+    return (ix == 0 && A.dir_lo[0]) || (ix == NX - 1 && A.dir_hi[0]) ||
+           (NY > 1 && ((iy == 0 && A.dir_lo[1]) || (iy == NY - 1 && A.dir_hi[1])));
+  };
+  auto setbound = [&](C (&f)[PPT]) __attribute__((always_inline)) {
+    if (!A.dirichlet) return;
+    XSH_FOR(i, 0, CPT)
+      XSH_FOR(v, 0, 2)
+        if (dir_site(i, v)) f[2 * i + v] = C{R(0), R(0)};
+      XSH_END
+    XSH_END
+  };
+  // xprop.m:88-117: transform, propagator, inverse transform, then setbound
   auto propagate = [&](C (&f)[PPT]) __attribute__((always_inline)) {
-    if (A.has_linear) Lat::spectral(L, f, twx, twy, prop);
+    if (A.has_linear) {
+      if constexpr (MAT)
+        Lat::spectral_mat(L, f, A.tm, A.cplx[0], A.cplx[1], prop);
+      else
+        Lat::spectral(L, f, twx, twy, prop);
+    }
+    setbound(f);
+  };
+
+  // All propagations of a step go through one code site that transforms f1
+  // (the transform is by far the largest code; one site keeps the kernel
+  // small).  f0 holds the state between steps; where the reference propagates
+  // the other field, the two arrays swap roles (register moves).
+  auto swap01 = [&]() __attribute__((always_inline)) {
+    XSH_FOR(k, 0, PPT)
+      const C t = f0[k];
+      f0[k] = f1[k];
+      f1[k] = t;
+    XSH_END
   };
 
   auto method_step = [&](uint32_t q, bool coarse) __attribute__((always_inline)) {
-    if (A.method == XSH_RK4) {
-      // RK4.m:12-38 with the two propagations of each pair batched
+    const bool rk4 = A.method == XSH_RK4;
+    if (rk4) {
+      // RK4.m:9-14: a1 = a + D(a) dt, into f1 (a stays in f0)
+      setbound(f0);
       XSH_FOR(i, 0, CPT)
         C w[2];
         noise_chunk(i, q, coarse, w[0], w[1]);
@@ -292,48 +330,64 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
         XSH_END
         __builtin_amdgcn_sched_barrier(0);
       XSH_END
-      propagate(f0);  // a  = P a
-      propagate(f1);  // a1 = P a1
-      XSH_FOR(i, 0, CPT)
-        C w[2];
-        noise_chunk(i, q, coarse, w[0], w[1]);
-        XSH_FOR(v, 0, 2)
-          constexpr int k = 2 * i + v;
-          const C A0 = f0[k], A1 = f1[k];
-          const C a2 = A0 + hdt * D(A1, w[v]);
-          const C a3 = A0 + (R(2) * hdt) * D(a2, w[v]);
-          f0[k] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
-          f1[k] = a3;
-        XSH_END
-        __builtin_amdgcn_sched_barrier(0);
-      XSH_END
-      propagate(f0);
+    }
+    // RK4: P a1 | swap | P a, stage 2 | P a3 | swap | P R
+    // MP : swap | P a, midpoint passes | P a | swap
+    const int nprop = rk4 ? 4 : 2;
+    for (int sp = 0; sp < nprop; ++sp) {
+      if (!rk4 && sp == 0) swap01();
       propagate(f1);
+      if (rk4 ? (sp % 2 == 0) : (sp == 1)) swap01();
+      if (rk4 && sp == 1) {
+        // RK4.m:18-32 per site: A = f1, A1 = f0 -> a3 into f1, R into f0
+        XSH_FOR(i, 0, CPT)
+          C w[2];
+          noise_chunk(i, q, coarse, w[0], w[1]);
+          XSH_FOR(v, 0, 2)
+            constexpr int k = 2 * i + v;
+            const C A0 = f1[k], A1 = f0[k];
+            const bool dz = A.dirichlet && dir_site(i, v);
+            C a2 = A0 + hdt * D(A1, w[v]);
+            if (dz) a2 = C{R(0), R(0)};
+            C a3 = A0 + (R(2) * hdt) * D(a2, w[v]);
+            if (dz) a3 = C{R(0), R(0)};
+            f0[k] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
+            f1[k] = a3;
+          XSH_END
+          __builtin_amdgcn_sched_barrier(0);
+        XSH_END
+      } else if (!rk4 && sp == 0) {
+        // MP.m:56-67: fixed-point passes at the midpoint, per site, on f1
+        XSH_FOR(i, 0, CPT)
+          C w[2];
+          noise_chunk(i, q, coarse, w[0], w[1]);
+          XSH_FOR(v, 0, 2)
+            constexpr int k = 2 * i + v;
+            const C a0 = f1[k];
+            const bool dz = A.dirichlet && dir_site(i, v);
+            C a = a0;
+            for (int it = 0; it < A.iterations; ++it) {
+              a = a0 + hdt * D(a, w[v]);
+              if (dz) a = C{R(0), R(0)};
+            }
+            f1[k] = R(2) * a - a0;
+          XSH_END
+          __builtin_amdgcn_sched_barrier(0);
+        XSH_END
+      }
+    }
+    if (rk4) {
+      // RK4.m:33-41: a = P R + D(P a3) dt/3 with P a3 in f0, P R in f1
       XSH_FOR(i, 0, CPT)
         C w[2];
         noise_chunk(i, q, coarse, w[0], w[1]);
         XSH_FOR(v, 0, 2)
           constexpr int k = 2 * i + v;
-          f0[k] = f0[k] + (hdt / R(3)) * D(f1[k], w[v]);
+          f0[k] = f1[k] + (hdt / R(3)) * D(f0[k], w[v]);
         XSH_END
         __builtin_amdgcn_sched_barrier(0);
       XSH_END
-    } else {
-      // MP.m:53-69; the fixed-point passes run chunk by chunk
-      propagate(f0);
-      XSH_FOR(i, 0, CPT)
-        C w[2];
-        noise_chunk(i, q, coarse, w[0], w[1]);
-        XSH_FOR(v, 0, 2)
-          constexpr int k = 2 * i + v;
-          const C a0 = f0[k];
-          C a = a0;
-          for (int it = 0; it < A.iterations; ++it) a = a0 + hdt * D(a, w[v]);
-          f0[k] = R(2) * a - a0;
-        XSH_END
-        __builtin_amdgcn_sched_barrier(0);
-      XSH_END
-      propagate(f0);
+      setbound(f0);
     }
   };
 
@@ -425,7 +479,7 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
 }
 
 // propagation probe: planes [nplanes][NS] complex R in place
-template <class R, int NX, int NY, int NT, int TPB>
+template <class R, int NX, int NY, int NT, int TPB, bool MAT>
 __global__ __launch_bounds__(NT) void probe_propagate(const PathArgs* __restrict__ Ap, int nci, int nplanes, cpx<R>* planes) {
   const PathArgs& A = *Ap;
   using Lat = Lattice<R, NX, NY, NT, TPB>;
@@ -442,7 +496,10 @@ __global__ __launch_bounds__(NT) void probe_propagate(const PathArgs* __restrict
     }
   const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
                        NY > 1 ? static_cast<const C*>(nci ? A.py[1] : A.py[0]) : nullptr};
-  Lat::spectral(L, f, static_cast<const C*>(A.twx), static_cast<const C*>(A.twy), prop);
+  if constexpr (MAT)
+    Lat::spectral_mat(L, f, A.tm, A.cplx[0], A.cplx[1], prop);
+  else
+    Lat::spectral(L, f, static_cast<const C*>(A.twx), static_cast<const C*>(A.twy), prop);
 #pragma unroll
   for (int i = 0; i < Lat::CPT; ++i)
 #pragma unroll
@@ -472,62 +529,25 @@ static hipError_t launch_sde(const PathArgs* A, int nblocks, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <class R, int NX, int NY, int NT, int TPB>
+template <class R, int NX, int NY, int NT, int TPB, bool MAT>
 static hipError_t launch_spde(const PathArgs* A, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((spde_path<R, NX, NY, NT, TPB>), dim3(nblocks), dim3(NT), 0, s, A);
+  hipLaunchKernelGGL((spde_path<R, NX, NY, NT, TPB, MAT>), dim3(nblocks), dim3(NT), 0, s, A);
   return hipGetLastError();
 }
 
-template <class R, int NX, int NY, int NT, int TPB>
+template <class R, int NX, int NY, int NT, int TPB, bool MAT>
 static hipError_t launch_probe(const PathArgs* A, int nci, int nplanes, void* planes, hipStream_t s) {
   const int nb = (nplanes + TPB - 1) / TPB;
-  hipLaunchKernelGGL((probe_propagate<R, NX, NY, NT, TPB>), dim3(nb), dim3(NT), 0, s, A, nci, nplanes,
+  hipLaunchKernelGGL((probe_propagate<R, NX, NY, NT, TPB, MAT>), dim3(nb), dim3(NT), 0, s, A, nci, nplanes,
                      static_cast<cpx<R>*>(planes));
   return hipGetLastError();
 }
 
-#define XSH_ENTRY(R, NX, NY, NT, TPB)                                                           \
-  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, &launch_spde<R, NX, NY, NT, TPB>, \
-   &launch_probe<R, NX, NY, NT, TPB>}
-
-static const KernelEntry kTable[] = {
-    // fp32, 1D lattices
-    XSH_ENTRY(float, 32, 1, 256, 32), XSH_ENTRY(float, 64, 1, 256, 32),
-    XSH_ENTRY(float, 128, 1, 256, 32), XSH_ENTRY(float, 256, 1, 256, 16),
-    XSH_ENTRY(float, 512, 1, 256, 8), XSH_ENTRY(float, 1024, 1, 256, 4),
-    // fp32, 2D lattices
-    XSH_ENTRY(float, 32, 32, 256, 1), XSH_ENTRY(float, 64, 64, 512, 1),
-    XSH_ENTRY(float, 128, 128, 512, 1),
-    // fp64, 1D lattices
-    XSH_ENTRY(double, 32, 1, 256, 16), XSH_ENTRY(double, 64, 1, 256, 16),
-    XSH_ENTRY(double, 128, 1, 256, 16), XSH_ENTRY(double, 256, 1, 256, 8),
-    XSH_ENTRY(double, 512, 1, 256, 4), XSH_ENTRY(double, 1024, 1, 256, 2),
-    // fp64, 2D lattices
-    XSH_ENTRY(double, 32, 32, 256, 1), XSH_ENTRY(double, 64, 64, 512, 1),
-};
-
-const KernelEntry* find_kernel(int precision, int nx, int ny) {
-  for (const auto& e : kTable)
-    if (e.precision == precision && e.nx == nx && e.ny == ny) return &e;
-  return nullptr;
-}
-
-hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s) {
-  return precision == XSH_FP64 ? launch_sde<double>(A, nblocks, s) : launch_sde<float>(A, nblocks, s);
-}
-
-hipError_t launch_probe_normals(int precision, uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t inst,
-                                uint32_t draw, int traj0, int ntraj, int count, double* out, hipStream_t s) {
-  const int groups = (count + 3) / 4;
-  const int total = ntraj * groups;
-  const int nb = (total + 255) / 256;
-  if (precision == XSH_FP64)
-    hipLaunchKernelGGL(probe_normals<double>, dim3(nb), dim3(256), 0, s, nsp, tag, seq, inst, draw, traj0,
-                       ntraj, count, out);
-  else
-    hipLaunchKernelGGL(probe_normals<float>, dim3(nb), dim3(256), 0, s, nsp, tag, seq, inst, draw, traj0,
-                       ntraj, count, out);
-  return hipGetLastError();
-}
+#define XSH_ENTRY(R, NX, NY, NT, TPB)                                                                  \
+  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, 0, &launch_spde<R, NX, NY, NT, TPB, false>, \
+   &launch_probe<R, NX, NY, NT, TPB, false>}
+#define XSH_MAT_ENTRY(R, NX, NY, NT, TPB)                                                             \
+  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, 1, &launch_spde<R, NX, NY, NT, TPB, true>, \
+   &launch_probe<R, NX, NY, NT, TPB, true>}
 
 }  // namespace xsh

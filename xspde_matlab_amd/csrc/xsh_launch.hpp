@@ -12,11 +12,12 @@ typedef hipError_t (*ProbeLaunch)(const PathArgs*, int nci, int nplanes, void* p
 
 struct KernelEntry {
   int precision, nx, ny, threads, tpb;
+  int mat;  // dense per-dimension transforms (non-periodic lattices)
   SpdeLaunch launch;
   ProbeLaunch probe;
 };
 
-const KernelEntry* find_kernel(int precision, int nx, int ny);
+const KernelEntry* find_kernel(int precision, int nx, int ny, int mat);
 hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s);
 hipError_t launch_probe_normals(int precision, uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t inst,
                                 uint32_t draw, int traj0, int ntraj, int count, double* out, hipStream_t s);

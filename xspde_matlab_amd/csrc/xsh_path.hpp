@@ -48,6 +48,11 @@ struct PathArgs {
   const void* py[2];       // [nc] propagator factor along y (incl. 1/ny)
   const void* twx;         // W_nx^m, m < nx
   const void* twy;
+  int mat;                 // 1: dense per-dimension transforms (non-periodic lattice)
+  int cplx[2];             // per dimension: 1 if its matrix is complex (periodic DFT)
+  const void* tm[4];       // forward x, forward y, inverse x, inverse y (row-major N x N)
+  int dirichlet;           // any Dirichlet end (xsetbound.m:102-116)
+  int dir_lo[2], dir_hi[2];// per dimension: lower / upper end is Dirichlet
   double* av;              // [paths][avlen] complex double
   const double* a0;        // optional [E][ns] complex double initial fields
   double* a_out;           // optional [E][ns] complex double final fields
@@ -282,6 +287,107 @@ struct Lattice {
 #pragma unroll
       for (int n1 = 0; n1 < N1; ++n1) L[b + (n1 * N2 + n2) * Ln::ES] = v[n1];
     }
+  }
+
+  // Dense transform along DIM, in place in LDS: out[k] = sum_n T[k*N + n] in[n]
+  // with T real (the DCT/DST matrices of the non-periodic boundaries) or
+  // complex (the DFT of a periodic dimension on a mixed lattice).  Work items
+  // are (line, block of KB outputs); lanes run over lines, so the block - and
+  // the row of T - is wave-uniform.  Results are buffered in registers until
+  // every thread has read its inputs, then written back through op(out, line, k).
+  template <int DIM, bool CPLX, class Op>
+  __device__ static __forceinline__ void mat(C* L, const R* __restrict__ T, const Op& op) {
+    using Ln = Line<DIM>;
+    constexpr int N = Ln::N, NL = Ln::NL, KB = 8;
+    static_assert(NL % 64 == 0, "matrix transforms need whole waves of lines");
+    static_assert(N % KB == 0 && N % 8 == 0, "line length must be a multiple of 8");
+    constexpr int ITEMS = NL * (N / KB);
+    static_assert(ITEMS % NT == 0, "items must tile the threads");
+    constexpr int IT = ITEMS / NT;
+    constexpr int TW = CPLX ? 2 : 1;
+    C out[IT * KB];
+    static_for<0, IT>([&](auto I) __attribute__((always_inline)) {
+      constexpr int it = decltype(I)::value;
+      const int item = opaque_tid() + it * NT;
+      const int l = item % NL;
+      const int kb = __builtin_amdgcn_readfirstlane(item / NL);
+      const int b = Ln::base(l);
+      const R* __restrict__ Trow = T + (long long)kb * KB * N * TW;
+      C acc[KB];
+      static_for<0, KB>([&](auto J) __attribute__((always_inline)) {
+        acc[decltype(J)::value] = C{R(0), R(0)};
+      });
+#pragma unroll 1
+      for (int n0 = 0; n0 < N; n0 += 8) {
+        C x[8];
+        static_for<0, 8>([&](auto Q) __attribute__((always_inline)) {
+          constexpr int q = decltype(Q)::value;
+          x[q] = L[b + (n0 + q) * Ln::ES];
+        });
+        static_for<0, KB>([&](auto J) __attribute__((always_inline)) {
+          constexpr int j = decltype(J)::value;
+          static_for<0, 8>([&](auto Q) __attribute__((always_inline)) {
+            constexpr int q = decltype(Q)::value;
+            if constexpr (CPLX) {
+              const C t = {Trow[2 * (j * N + n0 + q)], Trow[2 * (j * N + n0 + q) + 1]};
+              acc[j] = acc[j] + t * x[q];
+            } else {
+              acc[j] = acc[j] + Trow[j * N + n0 + q] * x[q];
+            }
+          });
+        });
+      }
+      static_for<0, KB>([&](auto J) __attribute__((always_inline)) {
+        constexpr int j = decltype(J)::value;
+        out[it * KB + j] = op(acc[j], l, kb * KB + j);
+      });
+    });
+    __syncthreads();
+    static_for<0, IT>([&](auto I) __attribute__((always_inline)) {
+      constexpr int it = decltype(I)::value;
+      const int item = opaque_tid() + it * NT;
+      const int l = item % NL;
+      const int kb = item / NL;
+      const int b = Ln::base(l);
+      static_for<0, KB>([&](auto J) __attribute__((always_inline)) {
+        constexpr int j = decltype(J)::value;
+        L[b + (kb * KB + j) * Ln::ES] = out[it * KB + j];
+      });
+    });
+  }
+
+  template <int DIM, class Op>
+  __device__ static __forceinline__ void mat_any(C* L, const void* T, bool cplx, const Op& op) {
+    if (cplx)
+      mat<DIM, true>(L, static_cast<const R*>(T), op);
+    else
+      mat<DIM, false>(L, static_cast<const R*>(T), op);
+  }
+
+  // a -> T^-1 op(T a) with dense per-dimension transforms (boundary types of
+  // xtransform.m:208-245); op receives natural-order (kx, ky).
+  template <class Op>
+  __device__ static __forceinline__ void spectral_mat(C* L, C (&f)[PPT], const void* const* Tm, int cplx_x,
+                                                      int cplx_y, const Op& op) {
+    auto id = [](C X, int, int) __attribute__((always_inline)) { return X; };
+    __syncthreads();
+    store(L, f);
+    __syncthreads();
+    if constexpr (NY == 1) {
+      mat_any<0>(L, Tm[0], cplx_x, [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, k, 0); });
+      __syncthreads();
+      mat_any<0>(L, Tm[2], cplx_x, id);
+    } else {
+      mat_any<0>(L, Tm[0], cplx_x, id);
+      __syncthreads();
+      mat_any<1>(L, Tm[1], cplx_y, [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, l % NX, k); });
+      __syncthreads();
+      mat_any<1>(L, Tm[3], cplx_y, id);
+      __syncthreads();
+      mat_any<0>(L, Tm[2], cplx_x, id);
+    }
+    __syncthreads();
+    load(L, f);
   }
 
   // frequency index held at position pos along x after the forward stages
