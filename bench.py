@@ -10,7 +10,15 @@ Workload (one "step" = one full ensemble pass of the hot path):
   random-phase initial fields and counter-based noise, generated on device.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--precision fp32|fp64] [--boundaries periodic|neumann]
+       python bench.py --workload tw     (control variate vs plain ensemble)
 Prints one JSON line (rank 0, N=1).
+
+--workload tw measures the second half of the headline metric, the wall time
+to a fixed error bar: on-site g2 of a truncated-Wigner 2D Bose lattice
+(128x128 periodic, coherent n0 = 100 plus vacuum noise, RK4IP, 1000 steps of
+dt = 0.001), plain 65536-trajectory ensemble against the control-variate
+estimate (4096 training trajectories, GPU-trained network, 10^6 cheap samples).
 """
 import argparse
 import json
@@ -22,15 +30,39 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
+VALU_PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}   # FLOP/s, MI355X vector
+
+
+def step_flops(ns, transforms=8, drift_evals=4, c_drift=20):
+    """Algorithmic flops of one RK4IP trajectory-step on an ns-site periodic
+    lattice: 8 FFTs of 5 ns log2 ns, 4 propagator products (6 ns), 4 drift
+    evaluations (c_drift ns) and the RK combinations (10 ns)."""
+    import math
+    return transforms * 5 * ns * math.log2(ns) + 4 * 6 * ns + drift_evals * c_drift * ns + 10 * ns
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if it was
+    taken for this kernel and workload (else None)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("workload") != workload:
+        return None
+    return rec.get("bytes_per_launch")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seconds=12.0):
-    """FP64 oracle (numpy restatement of the reference path), 1 core, bounded sample."""
-    import numpy as np
+def _cpu_worker(args):
+    """One single-threaded worker of the CPU baseline (like one parfor worker)."""
+    seconds, seed = args
+    os.environ["OMP_NUM_THREADS"] = "1"
     from oracle import forms
     from oracle import xspde as X
     from xspde_matlab_amd.configs import cgl2d, shrink
@@ -39,17 +71,32 @@ def cpu_baseline(seconds=12.0):
     q = X.xpreferences(forms.apply_forms(p))
     q.dtr = q.dt / (2 * q.steps[0])
     prop = X.xpropfactor(q)
-    v = X.xrandom(q, 1)
+    v = X.xrandom(q, 1 + seed)
     a = X.initial_field(q, v)
     n = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < seconds:
-        w = X.noise_draw(q, 1, n)
+        w = X.noise_draw(q, 1 + seed, n)
         a = X.RK4(a, w, q, prop)
         n += 1
-    dt = time.perf_counter() - t0
-    return dict(value=n * E / dt, unit="trajectory-steps/s", cores=1, kind="port",
-                sample="oracle RK4IP fp64 numpy, 128x128 CGL, %d trajectories x %d steps (%.1f s)" % (E, n, dt))
+    return n * E, time.perf_counter() - t0
+
+
+def cpu_baseline(seconds=12.0, workers=None):
+    """FP64 oracle (numpy restatement of the reference path) on the host cores:
+    one single-threaded worker process per core, each stepping its own 8
+    trajectories for a bounded time, as the reference's parfor over
+    ensembles(3) would (xsim.m:51)."""
+    import multiprocessing as mp
+    workers = workers or min(16, os.cpu_count() or 1)
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(workers) as pool:
+        res = pool.map(_cpu_worker, [(seconds, w) for w in range(workers)])
+    rate = sum(n / dt for n, dt in res)
+    steps = sum(n for n, _ in res)
+    return dict(value=rate, unit="trajectory-steps/s", cores=workers, kind="port",
+                sample="oracle RK4IP fp64 numpy, 128x128 CGL periodic, %d single-threaded workers x 8 "
+                       "trajectories, %d trajectory-steps in %.0f s" % (workers, steps, seconds))
 
 
 def main():
@@ -61,11 +108,14 @@ def main():
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--boundaries", default="periodic", choices=["periodic", "neumann"],
                     help="periodic (FFT) or Neumann-Neumann (DCT-I) lattice")
+    ap.add_argument("--workload", default="cgl2d", choices=["cgl2d", "tw"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs a single MI355X (--gpus %d not supported)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "tw":
+        return bench_tw(args)
 
     from xspde_matlab_amd import Problem
     from xspde_matlab_amd.configs import cgl2d
@@ -95,6 +145,8 @@ def main():
     rate = traj_steps / (ms_step / 1e3)
     alg_bytes = traj_steps * 2 * ns * esize            # state read + write per trajectory-step
     achieved = alg_bytes / (kavg / 1e3)
+    flops = traj_steps * step_flops(ns) / (kavg / 1e3)
+    wl = "cgl2d128-%s-%s" % (args.precision, args.boundaries)
     # a sanity read of the moments so the run is checked, outside the timed region
     av = prob.handle.read_averages()
     import numpy as np
@@ -116,8 +168,90 @@ def main():
                    "points": 101, "steps": 20, "checks": 1, "traj_steps_per_step": traj_steps,
                    "precision": args.precision, "moments": "fp64", "finite": finite},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None},
+                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic("spde_path", wl),
+                     "kernel": "spde_path", "kernel_ms": kavg,
+                     "valu": {"achieved": flops / 1e12, "peak": VALU_PEAK[args.precision] / 1e12,
+                              "unit": "TFLOP/s", "frac": flops / VALU_PEAK[args.precision],
+                              "flops_per_traj_step": step_flops(ns)}},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+# attractive lattice whose modulational-instability band (hop k^2 < 2 |g| n0)
+# holds the 32 feature modes: vacuum noise in those modes seeds the growth
+TW_G, TW_HOP = -0.035, 270.0
+
+
+def bench_tw(args):
+    """Wall time to the plain ensemble's error bar: plain vs control variate.
+
+    Plain: 65536 trajectories.  Control variate: the network is trained on
+    4096 trajectories, its mean is taken over 10^6 cheap device-drawn noise
+    samples, and then only as many independent trajectories are run as the
+    residual variance needs to reach the plain ensemble's error bar.  Its wall
+    time counts the training trajectories, the training, the cheap samples and
+    the estimation trajectories."""
+    import math
+    import numpy as np
+    from xspde_matlab_amd import cv
+    from xspde_matlab_amd.configs import tw2d
+
+    E, M, NA, NP = 1024, 10 ** 6, 4096, 65536
+
+    def mk(e2, seed):
+        return tw2d("fp32", E=E, e2=e2, seed=seed, g=TW_G, hop=TW_HOP)
+
+    tr = cv.feature_translations(mk(1, 0))
+    log("bench tw: warmup")
+    cv.run_samples(mk(1, 77))
+    log("bench tw: %d training trajectories" % NA)
+    (oA, zA, kA), wA = cv.timed(cv.run_samples, mk(NA // E, 0))
+    log("bench tw: plain ensemble (%d trajectories)" % NP)
+    (oP, zP, kP), wP = cv.timed(cv.run_samples, mk(NP // E, 1000))
+    log("bench tw: plain %.2f s (kernel %.2f s), training set %.2f s" % (wP, kP, wA))
+    spt = wP / NP
+    out = {}
+    for name in ("g2", "lowk"):
+        if name == "g2":
+            g_plain, yP = cv.g2_pseudo(*cv.g2_parts(oP))
+            _, yA = cv.g2_pseudo(*cv.g2_parts(oA))
+        else:
+            yP, yA = oP[:, 2], oA[:, 2]
+            g_plain = float(np.mean(yP))
+        sig_target = float(np.std(yP, ddof=1) / math.sqrt(NP))
+        model = cv.CVModel(seed=0, batch_max=8192)
+        info, wfit = cv.timed(model.fit, zA, yA, translations=tr)
+        (mu_f, var_f), wcheap = cv.timed(model.cheap_moments, M, 7)
+        # residual variance measured over the whole plain ensemble (independent of training)
+        var_res = float(np.var(yP - model.predict(zP), ddof=1))
+        n_need = cv.trajectories_to_error_bar(var_res, var_f, M, sig_target)
+        nB = int(min(NP, max(E, math.ceil(n_need / E) * E)))
+        # the estimate itself, on the first nB plain trajectories
+        if name == "g2":
+            _, yB = cv.g2_pseudo(*cv.g2_parts(oP[:nB]))
+        else:
+            yB = yP[:nB]
+        r = cv.estimate(model, zP[:nB], yB, m_cheap=M)
+        model.close()
+        t_cv = wA + wfit + wcheap + nB * spt
+        out[name] = dict(plain_mean=g_plain, plain_sigma=sig_target, cv_mean=r["cv_mean"], cv_sigma=r["cv_sigma"],
+                         var_ratio=float(np.var(yP, ddof=1) / var_res), beta=info["beta"],
+                         estimate_trajectories=nB, cv_seconds=t_cv, plain_seconds=wP, speedup=wP / t_cv,
+                         fit_seconds=wfit, cheap_seconds=wcheap, training_seconds=wA)
+        log("bench tw: %s var ratio %.2f, %d estimate trajectories, cv %.2f s vs plain %.2f s (x%.2f)"
+            % (name, out[name]["var_ratio"], nB, t_cv, wP, wP / t_cv))
+    line = {
+        "metric": "wall time to fixed error bar (on-site g2, truncated-Wigner 2D 128^2)",
+        "value": out["g2"]["cv_seconds"], "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * out["g2"]["cv_seconds"], "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "truncated-Wigner GP 128x128 periodic, n0=100, g=%g, hop=%g, RK4IP 1000 x "
+                               "dt=0.001; plain 65536 trajectories vs control variate (network trained on "
+                               "4096 trajectories, 64-256-256-1 GELU on bf16 MFMA, 10^6 cheap samples)"
+                               % (TW_G, TW_HOP),
+                   "plain_seconds": wP, "plain_kernel_seconds": kP,
+                   "g2": out["g2"], "low_mode_occupation": out["lowk"]},
     }
     print(json.dumps(line), flush=True)
 

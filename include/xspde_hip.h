@@ -57,9 +57,13 @@ enum {
   XSH_OBS_ABS2_AVE = 3, /* lattice mean |a|^2          */
   XSH_OBS_ABS2_INT = 4, /* sum |a|^2 dV                */
   XSH_OBS_ABS4_AVE = 5, /* lattice mean |a|^4          */
-  XSH_OBS_CORR = 6      /* (1/Ns) sum_x a*(x) a(x+r), per r (periodic) */
+  XSH_OBS_CORR = 6,     /* (1/Ns) sum_x a*(x) a(x+r), per r (periodic) */
+  XSH_OBS_LOWK = 7      /* sum over the nonzero noise modes k of (|a_k|^2 - 1/2),
+                           a_k = sqrt(dv/Ns) sum_x a(x) e^{-ik.x}: occupation of the
+                           lowest lattice modes (periodic, nmodes > 0) */
 };
 #define XSH_MAX_OBS 8
+#define XSH_MAX_MODES 32
 
 /* Problem descriptor: the subset of the reference's struct p that the path
  * reads (xpreferences.m:42-219, spdpreferences.m:10-72).  Lattice sizes are
@@ -96,6 +100,9 @@ typedef struct {
   int observe[XSH_MAX_OBS];
   int seq;              /* sequence index (random stream key)                */
   int inst;             /* instance id (random stream counter)               */
+  int traj_out;         /* 1: keep lattice-mean observables per trajectory    */
+  int nmodes;           /* Fourier modes of the initial noise kept per trajectory */
+  int modes[XSH_MAX_MODES][2]; /* (kx, ky) FFT indices of those modes          */
 } xsh_desc;
 
 typedef struct xsh_handle xsh_handle;
@@ -149,6 +156,14 @@ int xsh_synchronize(xsh_handle* h, double* ms);
  * [ensembles(2)][nchk][avlen] complex (interleaved doubles). */
 int xsh_read_averages(xsh_handle* h, double* av_out);
 
+/* Per-trajectory data of the last xsh_run_ensemble (no reference counterpart;
+ * feeds the control-variate estimator):
+ *   obs  [ensembles(2)][nchk][ensembles(1)][points][nobserve] real parts of the
+ *        lattice-mean observables (traj_out = 1, NULL to skip)
+ *   feat [ensembles(2)][nchk][ensembles(1)][2*nmodes] Re, Im of the chosen
+ *        Fourier modes of the initial noise, scaled to unit variance (NULL to skip) */
+int xsh_read_trajectories(xsh_handle* h, double* obs, float* feat);
+
 /* Test hooks (no reference counterpart; used by the parity tests). */
 /* Standard normals of one draw: out[t][L], L < count, for trajectories
  * traj0 .. traj0+ntraj-1, in the working precision of the handle. */
@@ -158,6 +173,35 @@ int xsh_probe_normals(xsh_handle* h, int nsp, int tag, int draw, int traj0, int 
  * planes ([nplanes][nx*ny] complex, in/out on the host):
  * a -> T^-1 (P . T a) with the handle's boundary transforms. */
 int xsh_probe_propagate(xsh_handle* h, int nc, int nplanes, double* planes);
+
+/* ---- control-variate network (truncated-Wigner estimator) -------------
+ * No reference counterpart: the reference has no learned component.  A
+ * 64 -> 256 -> 256 -> 1 GELU network on MFMA (bf16 operands, FP32
+ * accumulation) predicts a trajectory's observable from the lowest Fourier
+ * modes of its initial noise; its mean over cheap Gaussian samples plus the
+ * ensemble mean of (observable - prediction) is an unbiased, lower-variance
+ * estimate of the observable's mean.  Parameters are a flat float array
+ * W1[nin][nh] b1[nh] W2[nh][nh] b2[nh] w3[nh] b3. */
+typedef struct xsh_mlp xsh_mlp;
+int xsh_mlp_create(int nin, int nh, int batch_max, const float* theta0, xsh_mlp** out);
+void xsh_mlp_destroy(xsh_mlp* h);
+const char* xsh_mlp_last_error(void);
+int xsh_mlp_get_params(xsh_mlp* h, float* theta);
+/* y[i] = MLP(x[i]) for n rows of nin floats (host arrays). */
+int xsh_mlp_forward(xsh_mlp* h, long long n, const float* x, float* y);
+/* Loss sum((y-t)^2)/(2n) and its gradient (host arrays, n <= batch_max). */
+int xsh_mlp_loss_grad(xsh_mlp* h, int n, const float* x, const float* t, double* loss, float* grad);
+/* Adam over `epochs` passes of minibatches; order[e*n + i] = sample at position i of epoch e. */
+int xsh_mlp_train(xsh_mlp* h, int n, const float* x, const float* t, int epochs, int batch, const int* order,
+                  float lr, double* epoch_loss);
+/* Translation augmentation for xsh_mlp_train: input pair (2m, 2m+1) holds Re, Im of
+ * the Fourier mode of wavenumber (kxy[2m], kxy[2m+1]) of an nx x ny periodic
+ * lattice; every epoch each training sample is moved by a random lattice
+ * translation (its modes multiplied by exp(-2 pi i (kx sx/nx + ky sy/ny))), an
+ * exact symmetry of translation-invariant observables.  nmodes = 0 turns it off. */
+int xsh_mlp_set_translations(xsh_mlp* h, int nmodes, const int* kxy, int nx, int ny, int seed);
+/* Sum and sum of squares of MLP(z) over n standard-normal inputs drawn on the device. */
+int xsh_mlp_gaussian_moments(xsh_mlp* h, long long n, int seed, double* sum, double* sumsq);
 
 #ifdef __cplusplus
 }

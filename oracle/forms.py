@@ -23,6 +23,8 @@ initial  'const'  c0                   a = c0
          'coherent' alpha              a = alpha + (v0 + i v1)/2  (Wigner vacuum)
 observe  'a', 'abs2', 'a_ave', 'abs2_ave', 'abs2_int', 'abs4_ave', 'corr'
 """
+import math
+
 import numpy as np
 
 DERIVS = ("zero", "kubo", "gp")
@@ -119,7 +121,27 @@ def make_initial(g):
     raise ValueError("unknown initial form %r" % (ini,))
 
 
-def make_observe(name):
+def lowest_modes(nx, ny, count):
+    """FFT indices of the `count` lattice modes of smallest |m|^2, ties broken by
+    the angle atan2(my, mx) (the feature modes of the control variate)."""
+    cand = sorted((a * a + b * b, math.atan2(b, a), a, b)
+                  for a in range(-(nx // 2), nx - nx // 2) for b in range(-(ny // 2), ny - ny // 2))
+    return [(a % nx, b % ny) for _, _, a, b in cand[:count]]
+
+
+def make_observe(name, g=None):
+    if name == "lowk":
+        # occupation of the feature modes other than k = 0:
+        # sum_k (|a_k|^2 - 1/2), a_k = sqrt(dv/Ns) sum_x a(x) exp(-i k.x)
+        def f(a, p):
+            nx, ny = p.space[0], (p.space[1] if len(p.space) > 1 else 1)
+            m = g["modes"]
+            modes = lowest_modes(nx, ny, m) if isinstance(m, int) else [tuple(t) for t in m]
+            A = np.fft.fftn(a[0:1], axes=_space_axes(p)) * math.sqrt(p.dv / p.nspace)
+            A = A.reshape((1, nx, ny) + A.shape[-1:]) if A.ndim == 3 else A
+            r = sum(np.abs(A[0, kx, ky]) ** 2 - 0.5 for kx, ky in modes if (kx, ky) != (0, 0))
+            return np.asarray(r)[None, :]
+        return f
     if name == "a":
         return lambda a, p: a[0:1]
     if name == "abs2":
@@ -155,5 +177,5 @@ def apply_forms(p_in):
     p["deriv"] = make_deriv(g)
     p["linear"] = make_linear(g)
     p["initial"] = make_initial(g)
-    p["observe"] = [make_observe(o) for o in g.get("observe", ["a"])]
+    p["observe"] = [make_observe(o, g) for o in g.get("observe", ["a"])]
     return p

@@ -13,17 +13,20 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libxspde_hip.so")
 
 XSH_MAX_OBS = 8
+XSH_MAX_MODES = 32
 FP32, FP64 = 0, 1
 RK4, MP = 0, 1
 DERIV = {"zero": 0, "kubo": 1, "gp": 2}
 INITIAL = {"const": 0, "sech": 1, "gauss": 2, "phase": 3, "coherent": 4}
-OBSERVE = {"a": 0, "abs2": 1, "a_ave": 2, "abs2_ave": 3, "abs2_int": 4, "abs4_ave": 5, "corr": 6}
+OBSERVE = {"a": 0, "abs2": 1, "a_ave": 2, "abs2_ave": 3, "abs2_int": 4, "abs4_ave": 5, "corr": 6, "lowk": 7}
 STATUS = {0: "XSH_OK", 1: "XSH_EINVAL", 2: "XSH_EUNSUPPORTED", 3: "XSH_ENOMEM", 4: "XSH_EDEVICE",
           5: "XSH_ESTATE"}
 
 EXPORTS = ["xsh_version", "xsh_last_error", "xsh_create", "xsh_destroy", "xsh_query", "xsh_run_path",
-           "xsh_run_ensemble", "xsh_synchronize", "xsh_read_averages", "xsh_probe_normals",
-           "xsh_probe_propagate"]
+           "xsh_run_ensemble", "xsh_synchronize", "xsh_read_averages", "xsh_read_trajectories",
+           "xsh_probe_normals", "xsh_probe_propagate", "xsh_mlp_create", "xsh_mlp_destroy",
+           "xsh_mlp_last_error", "xsh_mlp_get_params", "xsh_mlp_forward", "xsh_mlp_loss_grad",
+           "xsh_mlp_train", "xsh_mlp_set_translations", "xsh_mlp_gaussian_moments"]
 
 
 class Desc(C.Structure):
@@ -38,6 +41,7 @@ class Desc(C.Structure):
         ("lin_l1", (C.c_double * 2) * 2), ("lin_l2", (C.c_double * 2) * 2),
         ("has_linear", C.c_int), ("initial", C.c_int), ("init_par", C.c_double * 6),
         ("nobserve", C.c_int), ("observe", C.c_int * XSH_MAX_OBS), ("seq", C.c_int), ("inst", C.c_int),
+        ("traj_out", C.c_int), ("nmodes", C.c_int), ("modes", (C.c_int * 2) * XSH_MAX_MODES),
     ]
 
 
@@ -79,6 +83,19 @@ def load():
     lib.xsh_read_averages.argtypes = [C.c_void_p, dp]
     lib.xsh_probe_normals.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp]
     lib.xsh_probe_propagate.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+    fp = C.POINTER(C.c_float)
+    lib.xsh_read_trajectories.argtypes = [C.c_void_p, dp, fp]
+    lib.xsh_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, fp, C.POINTER(C.c_void_p)]
+    lib.xsh_mlp_destroy.argtypes = [C.c_void_p]
+    lib.xsh_mlp_destroy.restype = None
+    lib.xsh_mlp_last_error.restype = C.c_char_p
+    lib.xsh_mlp_get_params.argtypes = [C.c_void_p, fp]
+    lib.xsh_mlp_forward.argtypes = [C.c_void_p, C.c_longlong, fp, fp]
+    lib.xsh_mlp_loss_grad.argtypes = [C.c_void_p, C.c_int, fp, fp, dp, fp]
+    lib.xsh_mlp_train.argtypes = [C.c_void_p, C.c_int, fp, fp, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                  C.c_float, dp]
+    lib.xsh_mlp_gaussian_moments.argtypes = [C.c_void_p, C.c_longlong, C.c_int, dp, dp]
+    lib.xsh_mlp_set_translations.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int]
     _lib = lib
     return lib
 
@@ -147,6 +164,20 @@ class Handle:
         check(load().xsh_read_averages(self._h, _ptr(out)))
         return (out[0::2] + 1j * out[1::2]).reshape(self.desc.ensembles[1], self.info.nchk, self.info.avlen)
 
+    def read_trajectories(self):
+        """Per-trajectory lattice means [e2, nchk, E, points, nobs] and noise modes [e2, nchk, E, 2*nmodes]."""
+        d = self.desc
+        e2, nchk, E = d.ensembles[1], self.info.nchk, d.ensembles[0]
+        obs = np.zeros(e2 * nchk * E * d.points * d.nobserve) if d.traj_out else None
+        feat = np.zeros(e2 * nchk * E * 2 * d.nmodes, dtype=np.float32) if d.nmodes else None
+        fptr = feat.ctypes.data_as(C.POINTER(C.c_float)) if feat is not None else None
+        check(load().xsh_read_trajectories(self._h, _ptr(obs), fptr))
+        if obs is not None:
+            obs = obs.reshape(e2, nchk, E, d.points, d.nobserve)
+        if feat is not None:
+            feat = feat.reshape(e2, nchk, E, 2 * d.nmodes)
+        return obs, feat
+
     def probe_normals(self, nsp, tag, draw, traj0, ntraj, count):
         out = np.zeros(ntraj * count)
         check(load().xsh_probe_normals(self._h, nsp, tag, draw, traj0, ntraj, count, _ptr(out)))
@@ -157,3 +188,78 @@ class Handle:
         buf = np.ascontiguousarray(np.stack([planes.real, planes.imag], -1).ravel())
         check(load().xsh_probe_propagate(self._h, nc, planes.shape[0], _ptr(buf)))
         return (buf[0::2] + 1j * buf[1::2]).reshape(planes.shape)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def mcheck(rc):
+    if rc != 0:
+        raise XshError(rc, load().xsh_mlp_last_error().decode())
+
+
+class MLP:
+    """The control-variate network on the GPU (xsh_mlp_* entry points)."""
+
+    def __init__(self, theta0, nin=64, nh=256, batch_max=65536):
+        lib = load()
+        th = np.ascontiguousarray(theta0, dtype=np.float32)
+        h = C.c_void_p()
+        mcheck(lib.xsh_mlp_create(nin, nh, batch_max, _fptr(th), C.byref(h)))
+        self._h, self.nin, self.nh, self.P = h, nin, nh, th.size
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().xsh_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.P, dtype=np.float32)
+        mcheck(load().xsh_mlp_get_params(self._h, _fptr(out)))
+        return out
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(x.shape[0], dtype=np.float32)
+        mcheck(load().xsh_mlp_forward(self._h, x.shape[0], _fptr(x), _fptr(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.P, dtype=np.float32)
+        loss = C.c_double()
+        mcheck(load().xsh_mlp_loss_grad(self._h, x.shape[0], _fptr(x), _fptr(t), C.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def train(self, x, t, epochs, batch, lr, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        n = x.shape[0]
+        rng = np.random.default_rng(seed)
+        order = np.ascontiguousarray(np.concatenate([rng.permutation(n) for _ in range(epochs)]).astype(np.int32))
+        losses = np.zeros(epochs)
+        mcheck(load().xsh_mlp_train(self._h, n, _fptr(x), _fptr(t), epochs, batch,
+                                    order.ctypes.data_as(C.POINTER(C.c_int)), lr, _ptr(losses)))
+        return losses
+
+    def set_translations(self, modes, nx, ny, seed=0):
+        """Train with random lattice translations of the (kx, ky) feature modes (None: off)."""
+        if modes is None:
+            mcheck(load().xsh_mlp_set_translations(self._h, 0, None, 1, 1, 0))
+            return
+        k = np.ascontiguousarray(np.asarray(modes, dtype=np.int32).reshape(-1))
+        mcheck(load().xsh_mlp_set_translations(self._h, len(k) // 2, k.ctypes.data_as(C.POINTER(C.c_int)),
+                                               nx, ny, seed))
+
+    def gaussian_moments(self, n, seed):
+        s, s2 = C.c_double(), C.c_double()
+        mcheck(load().xsh_mlp_gaussian_moments(self._h, n, seed, C.byref(s), C.byref(s2)))
+        return s.value, s2.value

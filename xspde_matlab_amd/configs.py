@@ -51,3 +51,19 @@ def shrink(p, E=None, e2=None, points=None, steps=None):
         p["steps"] = [steps] + list(p["steps"][1:])
     p["ranges"] = [dtc * p["steps"][0] * (p["points"][0] - 1)] + list(p["ranges"][1:])
     return p
+
+
+def tw2d(precision="fp32", E=1024, e2=64, n=128, steps=100, points=11, g=0.01, n0=100.0, modes=32, seed=0,
+         hop=0.5):
+    """Truncated-Wigner 2D Bose lattice GP: a(0) = sqrt(n0) + Wigner vacuum noise
+    (variance 1/2 per mode on a unit-spacing lattice), da = [i hop lap a
+    - i g (|a|^2 - 1) a] dt, RK4 interaction picture, dt = 0.001 to t = 1.
+    Observables: lattice means of |a|^2 and |a|^4 (g2 numerator and
+    denominator) and the occupation of the nonzero feature modes, kept per
+    trajectory with the lowest Fourier modes of the initial noise (features of
+    the control variate)."""
+    return dict(name="TW2D", dimensions=3, points=[points, n, n], ranges=[(points - 1) * steps * 0.001, n - 1, n - 1],
+                steps=[steps, 1, 1], checks=0, noises=0, inrandoms=2, method="RK4", ensembles=[E, e2, 1], seed=seed,
+                gpu=dict(deriv="gp", g=-1j * g, u=1j * g, s=0.0, linear=dict(l2=1j * hop), initial="coherent",
+                         alpha=n0 ** 0.5, observe=["abs2_ave", "abs4_ave", "lowk"], precision=precision,
+                         traj_out=True, modes=modes))

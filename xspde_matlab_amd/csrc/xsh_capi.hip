@@ -61,6 +61,9 @@ struct xsh_handle {
   double* d_av;      // [e2*nchk][avlen] complex
   double* d_path_av; // [avlen] complex (single path)
   double* d_a;       // [E][ns] complex (single path in/out)
+  double* d_traj;    // [e2*nchk][E][points][nobs] per-trajectory lattice means
+  float* d_feat;     // [e2*nchk][E][2*nmodes] noise modes
+  int* d_fmap;       // [nx*ny] mode index per FFT position
   hipStream_t stream;
   hipEvent_t ev0, ev1;
   xsh::PathArgs* d_args;   // kernel arguments in device memory
@@ -98,8 +101,10 @@ static int validate(const xsh_desc* d) {
   if (d->initial == XSH_INIT_SECH && !(d->init_par[1] > 0)) return fail(XSH_EINVAL, "sech width must be > 0");
   if (d->initial == XSH_INIT_GAUSS && !(d->init_par[1] > 0)) return fail(XSH_EINVAL, "gauss width must be > 0");
   if (d->nobserve < 1 || d->nobserve > XSH_MAX_OBS) return fail(XSH_EINVAL, "nobserve must be 1..%d", XSH_MAX_OBS);
-  for (int o = 0; o < d->nobserve; ++o)
-    if (d->observe[o] < XSH_OBS_A || d->observe[o] > XSH_OBS_CORR) return fail(XSH_EUNSUPPORTED, "unknown observe form %d", d->observe[o]);
+  for (int o = 0; o < d->nobserve; ++o) {
+    if (d->observe[o] == XSH_OBS_LOWK && d->nmodes < 1) return fail(XSH_EINVAL, "observe form lowk needs noise modes");
+    if (d->observe[o] < XSH_OBS_A || d->observe[o] > XSH_OBS_LOWK) return fail(XSH_EUNSUPPORTED, "unknown observe form %d", d->observe[o]);
+  }
   const int sdim = d->dimensions - 1;
   const int n[2] = {d->nx, d->ny};
   for (int k = 0; k < 2; ++k) {
@@ -117,6 +122,19 @@ static int validate(const xsh_desc* d) {
     }
   }
   if (!(d->ranges[0] > 0) && d->points > 1) return fail(XSH_EINVAL, "ranges(1) must be > 0");
+  if (d->traj_out != 0 && d->traj_out != 1) return fail(XSH_EINVAL, "traj_out must be 0 or 1");
+  if (d->nmodes < 0 || d->nmodes > XSH_MAX_MODES) return fail(XSH_EINVAL, "nmodes must be 0..%d", XSH_MAX_MODES);
+  if (d->nmodes > 0) {
+    if (d->initial != XSH_INIT_COHERENT) return fail(XSH_EUNSUPPORTED, "noise modes need the coherent initial form");
+    if (d->dimensions < 2) return fail(XSH_EINVAL, "noise modes need a lattice");
+    for (int m = 0; m < d->nmodes; ++m) {
+      const int kx = d->modes[m][0], ky = d->modes[m][1];
+      if (kx < 0 || kx >= d->nx || ky < 0 || ky >= (d->dimensions > 2 ? d->ny : 1))
+        return fail(XSH_EINVAL, "mode %d (%d,%d) outside the lattice", m, kx, ky);
+      for (int m2 = 0; m2 < m; ++m2)
+        if (d->modes[m2][0] == kx && d->modes[m2][1] == ky) return fail(XSH_EINVAL, "mode %d repeated", m);
+    }
+  }
   return XSH_OK;
 }
 
@@ -241,6 +259,14 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     }
     h->threads = h->kern->threads;
     h->tpb = h->kern->tpb;
+    // spectral observables and noise modes use the lattice FFT of the kernel
+    bool spectral_obs = d.nmodes > 0;
+    for (int o = 0; o < d.nobserve; ++o)
+      spectral_obs = spectral_obs || d.observe[o] == XSH_OBS_CORR || d.observe[o] == XSH_OBS_LOWK;
+    if (spectral_obs && h->mat) {
+      delete h;
+      return fail(XSH_EUNSUPPORTED, "corr/lowk observables and noise modes need a periodic lattice with an FFT kernel");
+    }
   } else {
     h->threads = 256;
     h->tpb = 256;
@@ -356,6 +382,21 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     return cleanup(fail(XSH_ENOMEM, "hipMalloc averages (%zu bytes)", avbytes));
   if (hipMalloc(&h->d_path_av, (size_t)h->avlen * 16) != hipSuccess) return cleanup(fail(XSH_ENOMEM, "hipMalloc path averages"));
   if (hipMalloc(&h->d_a, (size_t)h->E * h->ns * 16) != hipSuccess) return cleanup(fail(XSH_ENOMEM, "hipMalloc fields"));
+  if ((d.traj_out || d.nmodes) && h->tpb != 1)
+    return cleanup(fail(XSH_EUNSUPPORTED, "per-trajectory outputs need one trajectory per workgroup (2D lattices)"));
+  const size_t paths = (size_t)h->e2 * h->nchk;
+  if (d.traj_out &&
+      hipMalloc(&h->d_traj, paths * h->E * d.points * d.nobserve * sizeof(double)) != hipSuccess)
+    return cleanup(fail(XSH_ENOMEM, "hipMalloc per-trajectory observables"));
+  if (d.nmodes) {
+    if (hipMalloc(&h->d_feat, paths * h->E * 2 * d.nmodes * sizeof(float)) != hipSuccess ||
+        hipMalloc(&h->d_fmap, (size_t)h->ns * sizeof(int)) != hipSuccess)
+      return cleanup(fail(XSH_ENOMEM, "hipMalloc noise modes"));
+    std::vector<int> fmap(h->ns, -1);
+    for (int m = 0; m < d.nmodes; ++m) fmap[d.modes[m][0] + h->nx * d.modes[m][1]] = m;
+    if (hipMemcpy(h->d_fmap, fmap.data(), fmap.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(XSH_EDEVICE, "copy mode map"));
+  }
   if (hipStreamCreate(&h->stream) != hipSuccess) return cleanup(fail(XSH_EDEVICE, "hipStreamCreate"));
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_args, hipEventDisableTiming) != hipSuccess)
@@ -374,6 +415,9 @@ extern "C" void xsh_destroy(xsh_handle* h) {
   if (h->d_av) hipFree(h->d_av);
   if (h->d_path_av) hipFree(h->d_path_av);
   if (h->d_a) hipFree(h->d_a);
+  if (h->d_traj) hipFree(h->d_traj);
+  if (h->d_feat) hipFree(h->d_feat);
+  if (h->d_fmap) hipFree(h->d_fmap);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
   if (h->ev_args) hipEventDestroy(h->ev_args);
@@ -460,6 +504,11 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
     if (A.dir_lo[k] || A.dir_hi[k]) A.dirichlet = 1;
   }
   for (int i = 0; i < 4; ++i) A.tm[i] = t + h->tab_off[6 + i];
+  A.traj = h->d_traj;
+  A.nmodes = d.nmodes;
+  A.fmap = h->d_fmap;
+  A.feat = h->d_feat;
+  A.feat_scale = std::sqrt(h->dv / h->ns);
   return A;
 }
 
@@ -497,6 +546,8 @@ extern "C" int xsh_run_path(xsh_handle* h, int npar, int j, int nc, const double
   A.j_only = j;
   A.nc_only = nc;
   A.av = h->d_path_av;
+  A.traj = nullptr;  // per-trajectory outputs belong to xsh_run_ensemble
+  A.nmodes = 0;
   const size_t fbytes = (size_t)h->E * h->ns * 16;
   HIPCHK(hipMemsetAsync(h->d_path_av, 0, (size_t)h->avlen * 16, h->stream));
   if (a0) {
@@ -542,6 +593,24 @@ extern "C" int xsh_read_averages(xsh_handle* h, double* av_out) {
   if (!h || !av_out) return fail(XSH_EINVAL, "NULL argument");
   if (!h->ran) return fail(XSH_ESTATE, "xsh_run_ensemble has not run");
   HIPCHK(hipMemcpyAsync(av_out, h->d_av, (size_t)h->e2 * h->nchk * h->avlen * 16, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return XSH_OK;
+}
+
+extern "C" int xsh_read_trajectories(xsh_handle* h, double* obs, float* feat) {
+  if (!h) return fail(XSH_EINVAL, "handle is NULL");
+  if (!h->ran) return fail(XSH_ESTATE, "xsh_run_ensemble has not run");
+  const size_t paths = (size_t)h->e2 * h->nchk;
+  if (obs) {
+    if (!h->d_traj) return fail(XSH_ESTATE, "descriptor did not ask for traj_out");
+    HIPCHK(hipMemcpyAsync(obs, h->d_traj, paths * h->E * h->d.points * h->d.nobserve * sizeof(double),
+                          hipMemcpyDeviceToHost, h->stream));
+  }
+  if (feat) {
+    if (!h->d_feat) return fail(XSH_ESTATE, "descriptor did not ask for noise modes");
+    HIPCHK(hipMemcpyAsync(feat, h->d_feat, paths * h->E * 2 * h->d.nmodes * sizeof(float), hipMemcpyDeviceToHost,
+                          h->stream));
+  }
   HIPCHK(hipStreamSynchronize(h->stream));
   return XSH_OK;
 }

@@ -285,7 +285,9 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
     return (ix == 0 && A.dir_lo[0]) || (ix == NX - 1 && A.dir_hi[0]) ||
            (NY > 1 && ((iy == 0 && A.dir_lo[1]) || (iy == NY - 1 && A.dir_hi[1])));
   };
+  // (only dense-transform lattices can have Dirichlet ends)
   auto setbound = [&](C (&f)[PPT]) __attribute__((always_inline)) {
+    if constexpr (!MAT) return;
     if (!A.dirichlet) return;
     XSH_FOR(i, 0, CPT)
       XSH_FOR(v, 0, 2)
@@ -333,48 +335,70 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
     }
     // RK4: P a1 | swap | P a, stage 2 | P a3 | swap | P R
     // MP : swap | P a, midpoint passes | P a | swap
+    // Dense-transform kernels run every propagation through one code site (a
+    // runtime loop, swapping roles); FFT kernels unroll the loop, trading code
+    // size for the swaps.
     const int nprop = rk4 ? 4 : 2;
-    for (int sp = 0; sp < nprop; ++sp) {
+    // RK4.m:18-32 per site with A in `a` and A1 in `b`: a3 -> a, R -> b
+    auto rk4_stage2 = [&](C (&a)[PPT], C (&b)[PPT]) __attribute__((always_inline)) {
+      XSH_FOR(i, 0, CPT)
+        C w[2];
+        noise_chunk(i, q, coarse, w[0], w[1]);
+        XSH_FOR(v, 0, 2)
+          constexpr int k = 2 * i + v;
+          const C A0 = a[k], A1 = b[k];
+          const bool dz = MAT && A.dirichlet && dir_site(i, v);
+          C a2 = A0 + hdt * D(A1, w[v]);
+          if (dz) a2 = C{R(0), R(0)};
+          C a3 = A0 + (R(2) * hdt) * D(a2, w[v]);
+          if (dz) a3 = C{R(0), R(0)};
+          b[k] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
+          a[k] = a3;
+        XSH_END
+        __builtin_amdgcn_sched_barrier(0);
+      XSH_END
+    };
+    // MP.m:56-67: fixed-point passes at the midpoint, per site, on f
+    auto mp_stage = [&](C (&f)[PPT]) __attribute__((always_inline)) {
+      XSH_FOR(i, 0, CPT)
+        C w[2];
+        noise_chunk(i, q, coarse, w[0], w[1]);
+        XSH_FOR(v, 0, 2)
+          constexpr int k = 2 * i + v;
+          const C a0 = f[k];
+          const bool dz = MAT && A.dirichlet && dir_site(i, v);
+          C a = a0;
+          for (int it = 0; it < A.iterations; ++it) {
+            a = a0 + hdt * D(a, w[v]);
+            if (dz) a = C{R(0), R(0)};
+          }
+          f[k] = R(2) * a - a0;
+        XSH_END
+        __builtin_amdgcn_sched_barrier(0);
+      XSH_END
+    };
+    // FFT kernels: A = P a in f0, A1 = P a1 in f1 -> a3 in f0, R in f1
+    auto swap_free_stage2 = [&]() __attribute__((always_inline)) { rk4_stage2(f0, f1); };
+    auto one = [&](int sp) __attribute__((always_inline)) {
       if (!rk4 && sp == 0) swap01();
       propagate(f1);
       if (rk4 ? (sp % 2 == 0) : (sp == 1)) swap01();
-      if (rk4 && sp == 1) {
-        // RK4.m:18-32 per site: A = f1, A1 = f0 -> a3 into f1, R into f0
-        XSH_FOR(i, 0, CPT)
-          C w[2];
-          noise_chunk(i, q, coarse, w[0], w[1]);
-          XSH_FOR(v, 0, 2)
-            constexpr int k = 2 * i + v;
-            const C A0 = f1[k], A1 = f0[k];
-            const bool dz = A.dirichlet && dir_site(i, v);
-            C a2 = A0 + hdt * D(A1, w[v]);
-            if (dz) a2 = C{R(0), R(0)};
-            C a3 = A0 + (R(2) * hdt) * D(a2, w[v]);
-            if (dz) a3 = C{R(0), R(0)};
-            f0[k] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
-            f1[k] = a3;
-          XSH_END
-          __builtin_amdgcn_sched_barrier(0);
-        XSH_END
-      } else if (!rk4 && sp == 0) {
-        // MP.m:56-67: fixed-point passes at the midpoint, per site, on f1
-        XSH_FOR(i, 0, CPT)
-          C w[2];
-          noise_chunk(i, q, coarse, w[0], w[1]);
-          XSH_FOR(v, 0, 2)
-            constexpr int k = 2 * i + v;
-            const C a0 = f1[k];
-            const bool dz = A.dirichlet && dir_site(i, v);
-            C a = a0;
-            for (int it = 0; it < A.iterations; ++it) {
-              a = a0 + hdt * D(a, w[v]);
-              if (dz) a = C{R(0), R(0)};
-            }
-            f1[k] = R(2) * a - a0;
-          XSH_END
-          __builtin_amdgcn_sched_barrier(0);
-        XSH_END
-      }
+      if (rk4 && sp == 1) rk4_stage2(f1, f0);
+      else if (!rk4 && sp == 0) mp_stage(f1);
+    };
+    if constexpr (MAT) {
+      for (int sp = 0; sp < nprop; ++sp) one(sp);
+    } else if (rk4) {
+      // the same sequence written out, with no role swaps
+      propagate(f0);
+      propagate(f1);
+      swap_free_stage2();
+      propagate(f0);  // P a3
+      propagate(f1);  // P R
+    } else {
+      propagate(f0);
+      mp_stage(f0);
+      propagate(f0);
     }
     if (rk4) {
       // RK4.m:33-41: a = P R + D(P a3) dt/3 with P a3 in f0, P R in f1
@@ -391,6 +415,21 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
     }
   };
 
+  // one transform instance for the observable-side ops (corr, noise modes) of f1
+  auto obs_transform = [&](int mode) __attribute__((always_inline)) {
+    const R fs = (R)A.feat_scale;
+    ObsOp<R> op{mode, mode == 0 ? R(1.0 / ((double)NS * (double)NS)) : (mode == 2 ? fs * fs : fs), NX, A.fmap,
+                A.feat + ((long long)pidx * A.E + group * TPB) * 2 * A.nmodes};
+    Lat::spectral(L, f1, twx, twy, op);
+  };
+
+  // Fourier modes of the initial noise (coherent initial form: a = alpha + n/2)
+  if (A.nmodes > 0) {
+    const C alpha = C{(R)A.ip[0], (R)A.ip[1]};
+    XSH_FOR(k, 0, PPT) f1[k] = R(2) * (f0[k] - alpha); XSH_END
+    obs_transform(1);
+  }
+
   auto observe = [&](int n) __attribute__((always_inline)) {
     for (int o = 0; o < A.nobs; ++o) {
       const int kind = A.obs[o];
@@ -398,7 +437,7 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
       if (kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR) {
         if (kind == XSH_OBS_CORR) {
           XSH_FOR(k, 0, PPT) f1[k] = f0[k]; XSH_END
-          Lat::spectral(L, f1, twx, twy, CorrOp<R>{R(1.0 / ((double)NS * (double)NS))});
+          obs_transform(0);
         }
         XSH_FOR(i, 0, CPT)
           const int pt = Lat::point(i, 0);
@@ -419,15 +458,21 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
           }
         XSH_END
       } else {
+        if (kind == XSH_OBS_LOWK) {
+          XSH_FOR(k, 0, PPT) f1[k] = f0[k]; XSH_END
+          obs_transform(2);
+        }
         double re = 0.0, im = 0.0;
         XSH_FOR(i, 0, CPT)
           const int pt = Lat::point(i, 0);
-          const int p = pt / NS;
+          const int p = pt / NS, s = pt % NS;
           if (group * TPB + p < A.E) {
             XSH_FOR(v, 0, 2)
               const C a = f0[2 * i + v];
               const double m = (double)a.x * a.x + (double)a.y * a.y;
-              if (kind == XSH_OBS_A_AVE) {
+              if (kind == XSH_OBS_LOWK) {
+                if (s + v == 0) re += (double)f1[2 * i + v].x;
+              } else if (kind == XSH_OBS_A_AVE) {
                 re += (double)a.x;
                 im += (double)a.y;
               } else if (kind == XSH_OBS_ABS4_AVE) {
@@ -439,8 +484,13 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
           }
         XSH_END
         block_sum2<NT>(re, im, red);
-        const double sc = (kind == XSH_OBS_ABS2_INT ? A.dv : 1.0 / NS) * A.inv_E;
-        if (threadIdx.x == 0) atomic_add_c(dst + 2 * (long long)n, re * sc, im * sc);
+        const double sc0 = kind == XSH_OBS_ABS2_INT ? A.dv : (kind == XSH_OBS_LOWK ? 1.0 : 1.0 / NS);
+        const double sc = sc0 * A.inv_E;
+        if (threadIdx.x == 0) {
+          atomic_add_c(dst + 2 * (long long)n, re * sc, im * sc);
+          if (A.traj && TPB == 1 && group < A.E)  // per-trajectory lattice mean
+            A.traj[(((long long)pidx * A.E + group) * A.npts + n) * A.nobs + o] = re * sc0;
+        }
       }
     }
   };

@@ -52,6 +52,11 @@ struct PathArgs {
   int cplx[2];             // per dimension: 1 if its matrix is complex (periodic DFT)
   const void* tm[4];       // forward x, forward y, inverse x, inverse y (row-major N x N)
   int dirichlet;           // any Dirichlet end (xsetbound.m:102-116)
+  double* traj;            // optional [paths][E][npts][nobs] per-trajectory lattice means
+  int nmodes;              // Fourier modes of the initial noise to keep
+  const int* fmap;         // [nx*ny] mode index of each FFT position, -1 if not kept
+  float* feat;             // [paths][E][2*nmodes]
+  double feat_scale;       // sqrt(dv / nsites): unit-variance Re/Im
   int dir_lo[2], dir_hi[2];// per dimension: lower / upper end is Dirichlet
   double* av;              // [paths][avlen] complex double
   const double* a0;        // optional [E][ns] complex double initial fields
@@ -439,12 +444,27 @@ struct PropOp {
     return py ? r * py[ky] : r;
   }
 };
-// correlation op: |X|^2 * scale
+// observable ops, sharing one transform instance:
+//   CORR : |X|^2 * scale (equal-time correlation, the observe form 'corr')
+//   MODES: record selected modes X(kx,ky) * scale as features, pass X through
+//   LOWK : |X|^2 * scale - 1/2 on the selected modes other than k = 0, else 0;
+//          the unnormalised inverse transform then holds their sum at x = 0
 template <class R>
-struct CorrOp {
+struct ObsOp {
+  int mode;  // 0 CORR, 1 MODES, 2 LOWK
   R scale;
-  __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int, int) const {
-    return cpx<R>{norm2(X) * scale, R(0)};
+  int nx;
+  const int* fmap;
+  float* feat;
+  __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int kx, int ky) const {
+    if (mode == 0) return cpx<R>{norm2(X) * scale, R(0)};
+    const int fi = fmap[kx + nx * ky];
+    if (mode == 2) return cpx<R>{(fi >= 0 && (kx | ky)) ? norm2(X) * scale - R(0.5) : R(0), R(0)};
+    if (fi >= 0) {
+      feat[2 * fi] = (float)(X.x * scale);
+      feat[2 * fi + 1] = (float)(X.y * scale);
+    }
+    return X;
   }
 };
 

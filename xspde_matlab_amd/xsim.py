@@ -160,7 +160,26 @@ def make_desc(q):
         d.observe[o] = _lib.OBSERVE[name]
     d.seq = int(g.get("seq", 0))
     d.inst = int(g.get("inst", 0))
+    d.traj_out = 1 if g.get("traj_out") else 0
+    modes = g.get("modes")
+    if modes:
+        if isinstance(modes, int):
+            modes = lowest_modes(d.nx, d.ny, modes)
+        if len(modes) > _lib.XSH_MAX_MODES:
+            raise GpuFormError("at most %d noise modes" % _lib.XSH_MAX_MODES)
+        d.nmodes = len(modes)
+        for m, (kx, ky) in enumerate(modes):
+            d.modes[m][0], d.modes[m][1] = int(kx), int(ky)
     return d
+
+
+def lowest_modes(nx, ny, count):
+    """The `count` lattice Fourier modes of smallest |k| (ties by angle), as FFT indices."""
+    fx = np.arange(nx) - nx // 2
+    fy = np.arange(ny) - ny // 2
+    cand = [(int(a * a + b * b), float(np.arctan2(b, a)), int(a), int(b)) for a in fx for b in fy]
+    cand.sort()
+    return [(a % nx, b % ny) for _, _, a, b in cand[:count]]
 
 
 def split_averages(q, info, av):
