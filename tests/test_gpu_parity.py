@@ -45,8 +45,6 @@ def test_normals_match_oracle(prec, tol):
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 3e-6)])
 @pytest.mark.parametrize("shape", [(32,), (64,), (128,), (256,), (512,), (1024,), (32, 32), (64, 64), (128, 128)])
 def test_propagator_matches_oracle(prec, tol, shape):
-    if prec == "fp64" and shape == (128, 128):
-        pytest.skip("fp64 128x128 planes exceed the LDS workspace of the fused kernel")
     nd = len(shape) + 1
     p = dict(dimensions=nd, points=[3] + list(shape), ranges=[1.0] + [10.0] * len(shape), ensembles=[2, 1, 1],
              noises=0, gpu=dict(deriv="zero", linear=dict(l0=0.3 - 0.2j, l1=0.1 + 0.05j, l2=0.4 + 0.9j),
@@ -100,8 +98,8 @@ def test_nls1d_path(prec, tol, nx, method):
     _path_case(p, tol)
 
 
-@pytest.mark.parametrize("prec,tol,n", [("fp64", 1e-10, 32), ("fp64", 1e-10, 64), ("fp32", 5e-5, 32),
-                                        ("fp32", 5e-5, 64), ("fp32", 5e-5, 128)])
+@pytest.mark.parametrize("prec,tol,n", [("fp64", 1e-10, 32), ("fp64", 1e-10, 64), ("fp64", 1e-10, 128),
+                                        ("fp32", 5e-5, 32), ("fp32", 5e-5, 64), ("fp32", 5e-5, 128)])
 @pytest.mark.parametrize("method", ["RK4", "MP"])
 def test_cgl2d_path(prec, tol, n, method):
     p = shrink(cgl2d(prec, n=n), E=3, e2=2, points=3, steps=2)
@@ -187,3 +185,29 @@ def test_cgl2d_neumann_path(prec, tol, n):
     p["boundaries"] = [[-1, -1], [-1, -1]]
     p["gpu"]["observe"] = ["abs2_ave", "abs2"]
     _path_case(p, tol)
+
+
+@pytest.mark.parametrize("method", ["RK4", "MP"])
+def test_stream_kernel_persistent_grid(method):
+    """FP64 128x128 runs on the streaming kernel, whose persistent grid holds
+    at most 1024 workgroups: with 1100 trajectories the last 76 are integrated
+    by workgroups on their second work item.  Trajectories depend only on their
+    own counter stream, so the first three must equal a 3-trajectory run
+    bit for bit, and the ensemble mean of |a|^2 at the last point must equal
+    the mean over the returned final fields of all 1100 trajectories."""
+    base = shrink(cgl2d("fp64", n=128), E=3, e2=1, points=2, steps=1)
+    base["method"] = method
+    base["gpu"] = dict(base["gpu"], observe=["abs2_ave"])
+    prob = Problem(base)
+    assert prob.info.threads == 512 and prob.info.traj_per_block == 1
+    av3, a3 = prob.xpath(1, npar=1, j=1, want_final=True)
+    prob.close()
+    big = dict(base, ensembles=[1100, 1, 1])
+    prob = Problem(big)
+    av, a = prob.xpath(1, npar=1, j=1, want_final=True)
+    prob.close()
+    a = np.asarray(a).reshape(1100, -1)
+    assert np.array_equal(a[:3], np.asarray(a3).reshape(3, -1))
+    m = np.mean(np.abs(a) ** 2)
+    print("ensemble |a|^2 %.15g, from final fields %.15g" % (av[0][0, -1, 0].real, m))
+    assert abs(av[0][0, -1, 0].real - m) < 1e-12 * m

@@ -53,8 +53,10 @@ struct xsh_handle {
   long long obs_off[XSH_MAX_OBS];
   int obs_ns[XSH_MAX_OBS];
   size_t esize;
-  void* d_tables;   // px[2], py[2], twx, twy, transform matrices
-  size_t tab_off[10];
+  void* d_tables;   // px[2], py[2], twx, twy, transform matrices, half-length twiddles
+  size_t tab_off[11];
+  void* d_work;      // streaming kernels: field planes per workgroup
+  int work_planes;
   int btype[2];      // boundary type per space dimension (0 periodic)
   int mat;           // dense per-dimension transforms in use
   int cplx[2];       // per dimension: complex (DFT) matrix
@@ -248,6 +250,7 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
       if (h->btype[k] != 0) h->mat = 1;
     }
     h->kern = xsh::find_kernel(d.precision, h->nx, h->ny, h->mat);
+    if (!h->kern && !h->mat) h->kern = xsh::find_stream_kernel(d.precision, h->nx, h->ny);
     if (!h->kern && !h->mat) {  // periodic lattice without an FFT kernel: dense DFT
       h->mat = 1;
       h->kern = xsh::find_kernel(d.precision, h->nx, h->ny, 1);
@@ -266,6 +269,15 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     if (spectral_obs && h->mat) {
       delete h;
       return fail(XSH_EUNSUPPORTED, "corr/lowk observables and noise modes need a periodic lattice with an FFT kernel");
+    }
+    if (h->kern->stream) {
+      bool lowk = false;
+      for (int o = 0; o < d.nobserve; ++o) lowk = lowk || d.observe[o] == XSH_OBS_LOWK;
+      if (d.nmodes > 0 || lowk) {
+        delete h;
+        return fail(XSH_EUNSUPPORTED, "noise modes and the lowk observable are not on the streaming %dx%d kernel",
+                    npt[0], npt[1]);
+      }
     }
   } else {
     h->threads = 256;
@@ -326,6 +338,9 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     for (int m = 0; m < n; ++m) t[m] = std::polar(1.0, -2.0 * M_PI * m / n);
     tabs[4 + k] = t;
   }
+  // streaming kernels: twiddles of the half-length column transforms
+  std::vector<cd> twh(std::max(1, npt[1] / 2));
+  for (int m = 0; m < (int)twh.size(); ++m) twh[m] = std::polar(1.0, -2.0 * M_PI * m / (npt[1] / 2 > 0 ? npt[1] / 2 : 1));
   // dense transforms (xtransform.m:208-245): forward x, forward y, inverse x, inverse y;
   // real matrices are stored with a zero imaginary part dropped (one value per entry)
   std::vector<std::vector<double>> mats(4);
@@ -357,6 +372,8 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     h->tab_off[6 + i] = total;
     total += ((mats[i].size() * (h->esize / 2) + 255) / 256) * 256;
   }
+  h->tab_off[10] = total;
+  total += ((twh.size() * h->esize + 255) / 256) * 256;
   std::vector<char> host(total + 256);
   for (int i = 0; i < 6; ++i) {
     if (d.precision == XSH_FP64)
@@ -370,6 +387,10 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     else
       pack_real<float>(host, h->tab_off[6 + i], mats[i]);
   }
+  if (d.precision == XSH_FP64)
+    pack<double>(host, h->tab_off[10], twh);
+  else
+    pack<float>(host, h->tab_off[10], twh);
   auto cleanup = [&](int code) {
     xsh_destroy(h);
     return code;
@@ -397,6 +418,16 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     if (hipMemcpy(h->d_fmap, fmap.data(), fmap.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(fail(XSH_EDEVICE, "copy mode map"));
   }
+  if (h->kern && h->kern->stream) {
+    // one workspace per resident workgroup of the persistent grid
+    bool corr = false;
+    for (int o = 0; o < d.nobserve; ++o) corr = corr || d.observe[o] == XSH_OBS_CORR;
+    h->work_planes = corr ? 3 : 2;
+    const long long blocks = std::min<long long>((long long)paths * h->groups, xsh::kStreamMaxBlocks);
+    const size_t wbytes = (size_t)blocks * h->work_planes * h->ns * h->esize;
+    if (hipMalloc(&h->d_work, wbytes) != hipSuccess)
+      return cleanup(fail(XSH_ENOMEM, "hipMalloc streaming workspace (%zu bytes)", wbytes));
+  }
   if (hipStreamCreate(&h->stream) != hipSuccess) return cleanup(fail(XSH_EDEVICE, "hipStreamCreate"));
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_args, hipEventDisableTiming) != hipSuccess)
@@ -418,6 +449,7 @@ extern "C" void xsh_destroy(xsh_handle* h) {
   if (h->d_traj) hipFree(h->d_traj);
   if (h->d_feat) hipFree(h->d_feat);
   if (h->d_fmap) hipFree(h->d_fmap);
+  if (h->d_work) hipFree(h->d_work);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
   if (h->ev_args) hipEventDestroy(h->ev_args);
@@ -504,6 +536,9 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
     if (A.dir_lo[k] || A.dir_hi[k]) A.dirichlet = 1;
   }
   for (int i = 0; i < 4; ++i) A.tm[i] = t + h->tab_off[6 + i];
+  A.twh = t + h->tab_off[10];
+  A.work = h->d_work;
+  A.work_planes = h->work_planes;
   A.traj = h->d_traj;
   A.nmodes = d.nmodes;
   A.fmap = h->d_fmap;

@@ -15,9 +15,15 @@ struct KernelEntry {
   int mat;  // dense per-dimension transforms (non-periodic lattices)
   SpdeLaunch launch;
   ProbeLaunch probe;
+  int stream;  // fields in a device-memory workspace (lattices too large for the chip)
 };
 
+// persistent grid of the streaming kernels: at most this many workgroups,
+// each with its own workspace planes
+constexpr int kStreamMaxBlocks = 1024;
+
 const KernelEntry* find_kernel(int precision, int nx, int ny, int mat);
+const KernelEntry* find_stream_kernel(int precision, int nx, int ny);
 hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s);
 hipError_t launch_probe_normals(int precision, uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t inst,
                                 uint32_t draw, int traj0, int ntraj, int count, double* out, hipStream_t s);

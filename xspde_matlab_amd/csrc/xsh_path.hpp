@@ -58,6 +58,9 @@ struct PathArgs {
   float* feat;             // [paths][E][2*nmodes]
   double feat_scale;       // sqrt(dv / nsites): unit-variance Re/Im
   int dir_lo[2], dir_hi[2];// per dimension: lower / upper end is Dirichlet
+  void* work;              // streaming kernels: per-workgroup field planes in device memory
+  int work_planes;         // planes per workgroup (2, or 3 with the corr observable)
+  const void* twh;         // streaming kernels: W_{ny/2}^m, m < ny/2
   double* av;              // [paths][avlen] complex double
   const double* a0;        // optional [E][ns] complex double initial fields
   double* a_out;           // optional [E][ns] complex double final fields

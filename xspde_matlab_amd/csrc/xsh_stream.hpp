@@ -1,0 +1,388 @@
+// Streaming whole-path kernel for lattices whose state does not fit on chip
+// (FP64 128x128: one field is 256 KB, the register file of a CU 512 KB and
+// its LDS 160 KB).  The fields live in a per-workgroup workspace in device
+// memory; the 2D FFT runs through LDS half a plane at a time:
+//
+//   x pass   rows of each half-plane (64 x 128), forward DIF in LDS
+//   y pass   radix-2 split of every column across the halves in registers
+//            (u = a[y] + a[y+H], w = (a[y] - a[y+H]) W^y), then the even and
+//            odd half-length column transforms in LDS, each fused with the
+//            propagator multiply and its inverse, then the radix-2 merge
+//   x pass   inverse DIT of each half-plane
+//
+// Every pointwise stage of the method (drift evaluations, RK combinations)
+// streams its sites between the workspace and registers with coalesced
+// 2-site accesses; noise is regenerated from the counter stream at each
+// stage, exactly as in the register-resident kernel, so both kernels follow
+// the same arithmetic (xpath.m:29-86, RK4.m:7-43, MP.m:10-28).
+#pragma once
+#include "xsh_kernels.hpp"
+
+namespace xsh {
+
+template <class R, int NX, int NY, int NT>
+struct Stream {
+  static constexpr int NS = NX * NY;
+  static constexpr int H = NY / 2;            // rows per half-plane
+  static constexpr int NSH = NX * H;          // sites per half-plane
+  static constexpr int PAIRS = NS / (2 * NT); // 2-site chunks per thread (plane)
+  static constexpr int HPAIRS = NSH / (2 * NT);
+  static_assert(NY % 2 == 0 && NSH % (2 * NT) == 0, "half-planes must tile the threads");
+  using Lh = Lattice<R, NX, H, NT, 1>;        // LDS geometry of a half-plane
+  using C = cpx<R>;
+  static constexpr int RS = Lh::RS;
+
+  // first site of 2-site chunk i of a region of the plane (consecutive threads,
+  // consecutive chunks: coalesced)
+  __device__ static __forceinline__ int site(int i) { return (opaque_tid() + NT * i) * 2; }
+
+  __device__ static __forceinline__ void to_lds(C* L, const C* F) {
+#pragma unroll 4
+    for (int i = 0; i < HPAIRS; ++i) {
+      const int s = site(i), y = s / NX, x = s - y * NX;
+      const C a = F[s], b = F[s + 1];
+      L[y * RS + x] = a;
+      L[y * RS + x + 1] = b;
+    }
+  }
+  __device__ static __forceinline__ void from_lds(const C* L, C* F) {
+#pragma unroll 4
+    for (int i = 0; i < HPAIRS; ++i) {
+      const int s = site(i), y = s / NX, x = s - y * NX;
+      F[s] = L[y * RS + x];
+      F[s + 1] = L[y * RS + x + 1];
+    }
+  }
+
+  // F -> T^-1 op(T F) in place (F one plane in device memory).  twx: W_NX^m,
+  // twy: W_NY^m (m < NY), twh: W_H^m (m < H).  op(X, kx, ky) as in spectral().
+  template <class Op>
+  __device__ static __forceinline__ void propagate(C* L, C* F, const C* twx, const C* twy, const C* twh,
+                                                    const Op& op) {
+    // forward x on each half-plane
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      __syncthreads();
+      to_lds(L, F + h * NSH);
+      __syncthreads();
+      Lh::template dif1<0>(L, twx);
+      __syncthreads();
+      Lh::template dif2<0>(L);
+      __syncthreads();
+      from_lds(L, F + h * NSH);
+    }
+    __syncthreads();
+    // y: radix-2 split across the halves; odd part parked in registers
+    C w[2 * HPAIRS];
+#pragma unroll
+    for (int i = 0; i < HPAIRS; ++i) {
+      const int s = site(i), y = s / NX, x = s - y * NX;
+      const C t = twy[y];
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        const C a0 = F[s + v], a1 = F[NSH + s + v];
+        L[y * RS + x + v] = a0 + a1;
+        w[2 * i + v] = (a0 - a1) * t;
+      }
+    }
+    half_y(L, twh, op, 0);
+    // even result out to the first half of F, odd input in
+#pragma unroll
+    for (int i = 0; i < HPAIRS; ++i) {
+      const int s = site(i), y = s / NX, x = s - y * NX;
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        F[s + v] = L[y * RS + x + v];
+        L[y * RS + x + v] = w[2 * i + v];
+      }
+    }
+    half_y(L, twh, op, 1);
+    // radix-2 merge: a[y] = e + W^-y o, a[y+H] = e - W^-y o
+#pragma unroll
+    for (int i = 0; i < HPAIRS; ++i) {
+      const int s = site(i), y = s / NX, x = s - y * NX;
+      const C t = conjc(twy[y]);
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        const C e = F[s + v], o = L[y * RS + x + v] * t;
+        F[s + v] = e + o;
+        F[NSH + s + v] = e - o;
+      }
+    }
+    // inverse x on each half-plane
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      __syncthreads();
+      to_lds(L, F + h * NSH);
+      __syncthreads();
+      Lh::template dita<0>(L, twx);
+      __syncthreads();
+      Lh::template ditb<0>(L);
+      __syncthreads();
+      from_lds(L, F + h * NSH);
+    }
+    __syncthreads();
+  }
+
+  // half-length column transforms of the half-plane in LDS (parity 0: ky = 2k,
+  // parity 1: ky = 2k + 1), fused with op and the inverse
+  template <class Op>
+  __device__ static __forceinline__ void half_y(C* L, const C* twh, const Op& op, int parity) {
+    __syncthreads();
+    Lh::template dif1<1>(L, twh);
+    __syncthreads();
+    Lh::template mid<1>(L, twh, [&](C X, int l, int k) __attribute__((always_inline)) {
+      return op(X, Lh::kx_of_pos(l % NX), 2 * k + parity);
+    });
+    __syncthreads();
+    Lh::template ditb<1>(L);
+    __syncthreads();
+  }
+};
+
+template <class R, int NX, int NY, int NT>
+__global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ Ap, int nwork) {
+  const PathArgs& A = *Ap;
+  using S = Stream<R, NX, NY, NT>;
+  using C = cpx<R>;
+  constexpr int NS = S::NS, PAIRS = S::PAIRS;
+  __shared__ C L[S::Lh::LDS];
+  __shared__ double red[2 * (NT / 64)];
+
+  C* const F0 = static_cast<C*>(A.work) + (long long)blockIdx.x * A.work_planes * NS;
+  C* const F1 = F0 + NS;
+  C* const F2 = F1 + NS;  // corr observable scratch (present when work_planes == 3)
+  const C* twx = static_cast<const C*>(A.twx);
+  const C* twy = static_cast<const C*>(A.twy);
+  const C* twh = static_cast<const C*>(A.twh);
+  Drift<R> D;
+  D.init(A);
+
+#pragma unroll 1
+  for (int wk = blockIdx.x; wk < nwork; wk += gridDim.x) {
+    int pidx, nci, j, traj;
+    path_identity<R>(A, wk, pidx, nci, j, traj);
+    const uint32_t nsp = (uint32_t)(A.seed + A.npar + (j - 1) * A.e3);
+    const R hdt = (R)(nci ? A.hdt[1] : A.hdt[0]);
+    const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
+                         static_cast<const C*>(nci ? A.py[1] : A.py[0])};
+    double* av = A.av + 2 * (long long)pidx * A.avlen;
+
+    // ---- initial field (xensemble.m:53-65)
+    __syncthreads();
+#pragma unroll 1
+    for (int i = 0; i < PAIRS; ++i) {
+      const int s = S::site(i);
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        const int st = s + v, iy = st / NX, ix = st - iy * NX;
+        if (A.a0) {
+          const long long g = (long long)traj * NS + st;
+          F0[st] = C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]};
+        } else {
+          F0[st] = initial_value<R>(A, nsp, (uint32_t)traj, st, ix, iy);
+        }
+      }
+    }
+
+    auto noise = [&](int s, uint32_t q, bool coarse, C& w0, C& w1) __attribute__((always_inline)) {
+      draw_pair<R>(A, nsp, q, (uint32_t)traj, s, w0, w1);
+      if (coarse) {
+        C u0, u1;
+        draw_pair<R>(A, nsp, q + 1, (uint32_t)traj, s, u0, u1);
+        w0 = R(0.5) * (w0 + u0);
+        w1 = R(0.5) * (w1 + u1);
+      }
+    };
+    auto propagate = [&](C* F) __attribute__((always_inline)) {
+      if (A.has_linear) S::propagate(L, F, twx, twy, twh, prop);
+    };
+
+    auto method_step = [&](uint32_t q, bool coarse) __attribute__((always_inline)) {
+      __syncthreads();
+      if (A.method == XSH_RK4) {
+        // RK4.m:9-14: a1 = a + D(a) dt into F1
+#pragma unroll 1
+        for (int i = 0; i < PAIRS; ++i) {
+          const int s = S::site(i);
+          C w[2];
+          noise(s, q, coarse, w[0], w[1]);
+#pragma unroll
+          for (int v = 0; v < 2; ++v) F1[s + v] = F0[s + v] + hdt * D(F0[s + v], w[v]);
+        }
+        propagate(F0);
+        propagate(F1);
+        // RK4.m:18-32: a3 -> F0, R -> F1
+#pragma unroll 1
+        for (int i = 0; i < PAIRS; ++i) {
+          const int s = S::site(i);
+          C w[2];
+          noise(s, q, coarse, w[0], w[1]);
+#pragma unroll
+          for (int v = 0; v < 2; ++v) {
+            const C A0 = F0[s + v], A1 = F1[s + v];
+            const C a2 = A0 + hdt * D(A1, w[v]);
+            const C a3 = A0 + (R(2) * hdt) * D(a2, w[v]);
+            F1[s + v] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
+            F0[s + v] = a3;
+          }
+        }
+        propagate(F0);
+        propagate(F1);
+        // RK4.m:33-41: a = P R + D(P a3) dt/3
+        __syncthreads();
+#pragma unroll 1
+        for (int i = 0; i < PAIRS; ++i) {
+          const int s = S::site(i);
+          C w[2];
+          noise(s, q, coarse, w[0], w[1]);
+#pragma unroll
+          for (int v = 0; v < 2; ++v) F0[s + v] = F1[s + v] + (hdt / R(3)) * D(F0[s + v], w[v]);
+        }
+      } else {
+        // MP.m:56-67
+        propagate(F0);
+#pragma unroll 1
+        for (int i = 0; i < PAIRS; ++i) {
+          const int s = S::site(i);
+          C w[2];
+          noise(s, q, coarse, w[0], w[1]);
+#pragma unroll
+          for (int v = 0; v < 2; ++v) {
+            const C a0 = F0[s + v];
+            C a = a0;
+            for (int it = 0; it < A.iterations; ++it) a = a0 + hdt * D(a, w[v]);
+            F0[s + v] = R(2) * a - a0;
+          }
+        }
+        propagate(F0);
+      }
+    };
+
+    auto observe = [&](int n) __attribute__((always_inline)) {
+      __syncthreads();
+      for (int o = 0; o < A.nobs; ++o) {
+        const int kind = A.obs[o];
+        double* dst = av + 2 * A.obs_off[o];
+        if (kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR) {
+          const C* src = F0;
+          if (kind == XSH_OBS_CORR) {
+#pragma unroll 1
+            for (int i = 0; i < PAIRS; ++i) {
+              const int s = S::site(i);
+              F2[s] = F0[s];
+              F2[s + 1] = F0[s + 1];
+            }
+            const ObsOp<R> op{0, R(1.0 / ((double)NS * (double)NS)), NX, nullptr, nullptr};
+            S::propagate(L, F2, twx, twy, twh, op);
+            src = F2;
+          }
+#pragma unroll 1
+          for (int i = 0; i < PAIRS; ++i) {
+            const int s = S::site(i);
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+              const C a = src[s + v];
+              double re = (double)a.x, im = (double)a.y;
+              if (kind == XSH_OBS_ABS2) {
+                re = (double)a.x * a.x + (double)a.y * a.y;
+                im = 0.0;
+              }
+              atomic_add_c(dst + 2 * ((long long)n * NS + s + v), re * A.inv_E, im * A.inv_E);
+            }
+          }
+        } else {
+          double re = 0.0, im = 0.0;
+#pragma unroll 1
+          for (int i = 0; i < PAIRS; ++i) {
+            const int s = S::site(i);
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+              const C a = F0[s + v];
+              const double m = (double)a.x * a.x + (double)a.y * a.y;
+              if (kind == XSH_OBS_A_AVE) {
+                re += (double)a.x;
+                im += (double)a.y;
+              } else if (kind == XSH_OBS_ABS4_AVE) {
+                re += m * m;
+              } else {
+                re += m;
+              }
+            }
+          }
+          block_sum2<NT>(re, im, red);
+          const double sc0 = kind == XSH_OBS_ABS2_INT ? A.dv : 1.0 / NS;
+          const double sc = sc0 * A.inv_E;
+          if (threadIdx.x == 0) {
+            atomic_add_c(dst + 2 * (long long)n, re * sc, im * sc);
+            if (A.traj) A.traj[(((long long)pidx * A.E + traj) * A.npts + n) * A.nobs + o] = re * sc0;
+          }
+        }
+      }
+    };
+
+    // ---- the time loop of xpath.m:29-86
+    uint32_t q = 0;
+    for (int n = 0; n < A.npts; ++n) {
+      for (int st = 0; st < A.steps; ++st) {
+        if (n > 0 || st > 0 || A.npts == 1) {
+          if (A.checks && nci == 1) {
+            method_step(q, true);
+            q += 2;
+          } else {
+            for (int c = 0; c <= A.checks; ++c) method_step(q++, false);
+          }
+        }
+        if (st == 0) observe(n);
+      }
+    }
+    __syncthreads();
+    if (A.a_out) {
+#pragma unroll 1
+      for (int i = 0; i < PAIRS; ++i) {
+        const int s = S::site(i);
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          const long long g = (long long)traj * NS + s + v;
+          A.a_out[2 * g] = (double)F0[s + v].x;
+          A.a_out[2 * g + 1] = (double)F0[s + v].y;
+        }
+      }
+    }
+  }
+}
+
+// propagation probe of the streaming transform: planes [nplanes][NS] in place
+template <class R, int NX, int NY, int NT>
+__global__ __launch_bounds__(NT) void probe_stream(const PathArgs* __restrict__ Ap, int nci, int nplanes,
+                                                   cpx<R>* planes) {
+  const PathArgs& A = *Ap;
+  using S = Stream<R, NX, NY, NT>;
+  using C = cpx<R>;
+  __shared__ C L[S::Lh::LDS];
+  if ((int)blockIdx.x >= nplanes) return;
+  const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
+                       static_cast<const C*>(nci ? A.py[1] : A.py[0])};
+  S::propagate(L, planes + (long long)blockIdx.x * S::NS, static_cast<const C*>(A.twx),
+               static_cast<const C*>(A.twy), static_cast<const C*>(A.twh), prop);
+}
+
+template <class R, int NX, int NY, int NT>
+static hipError_t launch_stream(const PathArgs* A, int nblocks, hipStream_t s) {
+  const int grid = nblocks < kStreamMaxBlocks ? nblocks : kStreamMaxBlocks;
+  hipLaunchKernelGGL((spde_stream<R, NX, NY, NT>), dim3(grid), dim3(NT), 0, s, A, nblocks);
+  return hipGetLastError();
+}
+
+template <class R, int NX, int NY, int NT>
+static hipError_t launch_probe_stream(const PathArgs* A, int nci, int nplanes, void* planes, hipStream_t s) {
+  hipLaunchKernelGGL((probe_stream<R, NX, NY, NT>), dim3(nplanes), dim3(NT), 0, s, A, nci, nplanes,
+                     static_cast<cpx<R>*>(planes));
+  return hipGetLastError();
+}
+
+#define XSH_STREAM_ENTRY(R, NX, NY, NT)                                                                 \
+  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, 1, 0, &launch_stream<R, NX, NY, NT>,              \
+   &launch_probe_stream<R, NX, NY, NT>, 1}
+
+}  // namespace xsh
