@@ -108,7 +108,7 @@ def main():
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--boundaries", default="periodic", choices=["periodic", "neumann"],
                     help="periodic (FFT) or Neumann-Neumann (DCT-I) lattice")
-    ap.add_argument("--workload", default="cgl2d", choices=["cgl2d", "tw"])
+    ap.add_argument("--workload", default="cgl2d", choices=["cgl2d", "tw", "batch"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs a single MI355X (--gpus %d not supported)"
@@ -116,6 +116,8 @@ def main():
         sys.exit(2)
     if args.workload == "tw":
         return bench_tw(args)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     from xspde_matlab_amd import Problem
     from xspde_matlab_amd.configs import cgl2d
@@ -252,6 +254,59 @@ def bench_tw(args):
                                % (TW_G, TW_HOP),
                    "plain_seconds": wP, "plain_kernel_seconds": kP,
                    "g2": out["g2"], "low_mode_occupation": out["lowk"]},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_batch(args):
+    """64 uneven GP instances (1D 32..512, 2D 32^2..128^2, E 256..4096, 500 RK4IP
+    steps, FP32) in one graph launch, autotuned, against the same instances
+    launched one after another on their own handles."""
+    import numpy as np
+    from xspde_matlab_amd import Problem
+    from xspde_matlab_amd.configs import batch_instances
+    from xspde_matlab_amd.xsim import BatchProblem
+
+    ps = batch_instances(count=64, seed=0, steps=500)
+    log("bench batch: creating + autotuning 64 instances")
+    (bp, t_tune) = (lambda t0: (BatchProblem(ps, autotune=True), time.perf_counter() - t0))(time.perf_counter())
+    total = sum(info.traj_steps for info in bp.infos)
+    log("bench batch: %d trajectory-steps per run, tuning %.1f s, plan %s" % (total, t_tune, bp.plan()))
+    for _ in range(args.warmup):
+        bp.run(1)
+        bp.synchronize()
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bp.run(1)
+        ms.append(bp.synchronize())
+    wall = (time.perf_counter() - t0) / args.steps
+    finite = all(bool(np.isfinite(bp.read_averages(i)).all()) for i in range(len(ps)))
+    plan = bp.plan()
+    bp.close()
+    log("bench batch: %.1f ms per batch run; now one launch per instance" % (1e3 * wall))
+    probs = [Problem(p) for p in ps]
+    for pr in probs:                       # warm every kernel once
+        pr.handle.run_ensemble(1)
+    for pr in probs:
+        pr.handle.synchronize()
+    t0 = time.perf_counter()
+    for pr in probs:
+        pr.handle.run_ensemble(1)
+        pr.handle.synchronize()
+    seq = time.perf_counter() - t0
+    for pr in probs:
+        pr.close()
+    line = {
+        "metric": "trajectory-steps/s (batched uneven ensembles, one graph launch)",
+        "value": total / wall, "unit": "trajectory-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "64 GP instances, 1D {32..512} / 2D {32^2..128^2} by seed, gamma U[0.01,0.5], "
+                               "g U[0.1,2], E {256..4096}, 500 RK4IP steps, FP32, autotuned",
+                   "traj_steps_per_run": total, "sequential_seconds": seq, "batch_seconds": wall,
+                   "speedup_vs_sequential": seq / wall, "tuning_seconds": t_tune, "plan": plan,
+                   "finite": finite},
     }
     print(json.dumps(line), flush=True)
 

@@ -174,6 +174,30 @@ int xsh_probe_normals(xsh_handle* h, int nsp, int tag, int draw, int traj0, int 
  * a -> T^-1 (P . T a) with the handle's boundary transforms. */
 int xsh_probe_propagate(xsh_handle* h, int nc, int nplanes, double* planes);
 
+/* ---- batches of independent instances -----------------------------------
+ * No single reference counterpart: the reference runs one project per xsim
+ * call (xsim.m:1) and each parallel ensemble as a parfor worker
+ * (xsim.m:51-56); a batch runs many projects - uneven lattices, ensemble sizes
+ * and drift parameters - in one captured HIP graph: one persistent kernel per
+ * lattice kernel family, the families concurrently.  Every instance keeps the
+ * semantics and the averages layout of xsh_run_ensemble. */
+typedef struct xsh_batch xsh_batch;
+#define XSH_BATCH_AUTOTUNE 1  /* time every kernel variant (threads, trajectories
+                                 per workgroup, first FFT radix) and persistent
+                                 grid per family on a shortened run; keep the fastest */
+int xsh_batch_create(int n, const xsh_desc* descs, int flags, xsh_batch** out);
+void xsh_batch_destroy(xsh_batch* b);
+/* All instances, parallel ensemble npar, asynchronously (one graph launch). */
+int xsh_batch_run(xsh_batch* b, int npar);
+/* Wait; *ms = time of the last xsh_batch_run if non-NULL. */
+int xsh_batch_synchronize(xsh_batch* b, double* ms);
+/* Averages of instance i, laid out as xsh_read_averages. */
+int xsh_batch_read_averages(xsh_batch* b, int i, double* av_out);
+int xsh_batch_query(const xsh_batch* b, int i, xsh_info* info);
+/* Kernel plan: *ngroups families; plan[8*g ..] = precision, nx, ny, instances,
+ * threads, trajectories per workgroup, first x radix, persistent grid. */
+int xsh_batch_plan(const xsh_batch* b, int* ngroups, int* plan);
+
 /* ---- control-variate network (truncated-Wigner estimator) -------------
  * No reference counterpart: the reference has no learned component.  A
  * 64 -> 256 -> 256 -> 1 GELU network on MFMA (bf16 operands, FP32

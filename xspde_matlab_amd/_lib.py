@@ -26,7 +26,10 @@ EXPORTS = ["xsh_version", "xsh_last_error", "xsh_create", "xsh_destroy", "xsh_qu
            "xsh_run_ensemble", "xsh_synchronize", "xsh_read_averages", "xsh_read_trajectories",
            "xsh_probe_normals", "xsh_probe_propagate", "xsh_mlp_create", "xsh_mlp_destroy",
            "xsh_mlp_last_error", "xsh_mlp_get_params", "xsh_mlp_forward", "xsh_mlp_loss_grad",
-           "xsh_mlp_train", "xsh_mlp_set_translations", "xsh_mlp_gaussian_moments"]
+           "xsh_mlp_train", "xsh_mlp_set_translations", "xsh_mlp_gaussian_moments",
+           "xsh_batch_create", "xsh_batch_destroy", "xsh_batch_run", "xsh_batch_synchronize",
+           "xsh_batch_read_averages", "xsh_batch_query", "xsh_batch_plan"]
+XSH_BATCH_AUTOTUNE = 1
 
 
 class Desc(C.Structure):
@@ -96,6 +99,14 @@ def load():
                                   C.c_float, dp]
     lib.xsh_mlp_gaussian_moments.argtypes = [C.c_void_p, C.c_longlong, C.c_int, dp, dp]
     lib.xsh_mlp_set_translations.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int]
+    lib.xsh_batch_create.argtypes = [C.c_int, C.POINTER(Desc), C.c_int, C.POINTER(C.c_void_p)]
+    lib.xsh_batch_destroy.argtypes = [C.c_void_p]
+    lib.xsh_batch_destroy.restype = None
+    lib.xsh_batch_run.argtypes = [C.c_void_p, C.c_int]
+    lib.xsh_batch_synchronize.argtypes = [C.c_void_p, dp]
+    lib.xsh_batch_read_averages.argtypes = [C.c_void_p, C.c_int, dp]
+    lib.xsh_batch_query.argtypes = [C.c_void_p, C.c_int, C.POINTER(Info)]
+    lib.xsh_batch_plan.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     _lib = lib
     return lib
 
@@ -263,3 +274,55 @@ class MLP:
         s, s2 = C.c_double(), C.c_double()
         mcheck(load().xsh_mlp_gaussian_moments(self._h, n, seed, C.byref(s), C.byref(s2)))
         return s.value, s2.value
+
+
+class Batch:
+    """Many instances in one graph launch (xsh_batch_*)."""
+
+    def __init__(self, descs, autotune=False):
+        lib = load()
+        for d in descs:
+            d.struct_size = C.sizeof(Desc)
+        arr = (Desc * len(descs))(*descs)
+        b = C.c_void_p()
+        check(lib.xsh_batch_create(len(descs), arr, XSH_BATCH_AUTOTUNE if autotune else 0, C.byref(b)))
+        self._b, self.descs = b, list(descs)
+        self.infos = []
+        for i in range(len(descs)):
+            info = Info()
+            check(lib.xsh_batch_query(self._b, i, C.byref(info)))
+            self.infos.append(info)
+
+    def close(self):
+        if getattr(self, "_b", None):
+            load().xsh_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, npar=1):
+        check(load().xsh_batch_run(self._b, npar))
+
+    def synchronize(self):
+        ms = C.c_double()
+        check(load().xsh_batch_synchronize(self._b, C.byref(ms)))
+        return ms.value
+
+    def read_averages(self, i):
+        d, info = self.descs[i], self.infos[i]
+        n = d.ensembles[1] * info.nchk * info.avlen
+        out = np.zeros(2 * n)
+        check(load().xsh_batch_read_averages(self._b, i, _ptr(out)))
+        return (out[0::2] + 1j * out[1::2]).reshape(d.ensembles[1], info.nchk, info.avlen)
+
+    def plan(self):
+        n = C.c_int()
+        check(load().xsh_batch_plan(self._b, C.byref(n), None))
+        buf = (C.c_int * (8 * n.value))()
+        check(load().xsh_batch_plan(self._b, C.byref(n), buf))
+        keys = ("precision", "nx", "ny", "instances", "threads", "traj_per_block", "radix", "grid")
+        return [dict(zip(keys, buf[8 * g:8 * g + 8])) for g in range(n.value)]

@@ -67,3 +67,37 @@ def tw2d(precision="fp32", E=1024, e2=64, n=128, steps=100, points=11, g=0.01, n
                 gpu=dict(deriv="gp", g=-1j * g, u=1j * g, s=0.0, linear=dict(l2=1j * hop), initial="coherent",
                          alpha=n0 ** 0.5, observe=["abs2_ave", "abs4_ave", "lowk"], precision=precision,
                          traj_out=True, modes=modes))
+
+
+BATCH_1D = (32, 64, 128, 256, 512)
+BATCH_2D = (32, 64, 128)
+BATCH_E = (256, 512, 1024, 2048, 4096)
+
+
+def batch_instances(count=64, seed=0, steps=500, precision="fp32", points=11):
+    """Uneven batch: `count` independent GP instances, lattice drawn by seed from
+    1D {32..512} and 2D {32^2, 64^2, 128^2}, damping gamma ~ U[0.01, 0.5],
+    nonlinearity g ~ U[0.1, 2], ensemble size from {256..4096}, `steps` RK4IP
+    steps (points-1 stored intervals), sech/plane-wave initial state plus noise:
+        da = [i/2 lap a - i g |a|^2 a - gamma a] dt + sqrt(gamma) dW."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = [(n,) for n in BATCH_1D] + [(n, n) for n in BATCH_2D]
+    per = steps // (points - 1)
+    out = []
+    for i in range(count):
+        shape = sizes[rng.integers(len(sizes))]
+        gamma = float(rng.uniform(0.01, 0.5))
+        g = float(rng.uniform(0.1, 2.0))
+        E = int(BATCH_E[rng.integers(len(BATCH_E))])
+        e2 = 4 if E >= 1024 else 1
+        nd = len(shape) + 1
+        length = 20.0
+        p = dict(name="B%02d" % i, dimensions=nd, points=[points] + list(shape),
+                 ranges=[0.005 * steps] + [length] * len(shape), steps=[per] + [1] * len(shape), checks=0,
+                 noises=2, inrandoms=2, method="RK4", ensembles=[E // e2, e2, 1], seed=i,
+                 gpu=dict(deriv="gp", g=-1j * g, u=-gamma, s=math.sqrt(gamma / 2.0),
+                          linear=dict(l2=0.5j), initial="sech", amp=1.0, width=1.0, eps=0.1,
+                          observe=["abs2_ave", "a_ave"], precision=precision))
+        out.append(p)
+    return out

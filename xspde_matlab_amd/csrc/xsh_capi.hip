@@ -562,10 +562,12 @@ static int launch(xsh_handle* h, const xsh::PathArgs& A, int nblocks) {
   int rc = upload_args(h, A);
   if (rc) return rc;
   hipError_t e;
-  if (h->nd == 1)
+  if (h->nd == 1) {
     e = xsh::launch_sde_path(h->d.precision, h->d_args, nblocks, h->stream);
-  else
-    e = h->kern->launch(h->d_args, nblocks, h->stream);
+  } else {
+    const int grid = h->kern->stream ? std::min(nblocks, xsh::kStreamMaxBlocks) : nblocks;
+    e = h->kern->launch(h->d_args, nullptr, 1, nblocks, grid, h->stream);
+  }
   if (e != hipSuccess) return fail(XSH_EDEVICE, "kernel launch: %s", hipGetErrorString(e));
   return XSH_OK;
 }
@@ -694,5 +696,346 @@ extern "C" int xsh_probe_propagate(xsh_handle* h, int nc, int nplanes, double* p
     std::memcpy(planes, buf.data(), n * 16);
   else
     for (size_t i = 0; i < 2 * n; ++i) planes[i] = reinterpret_cast<float*>(buf.data())[i];
+  return XSH_OK;
+}
+
+// ======================================================================
+// Batches: many independent instances (uneven lattices, ensembles and drift
+// parameters) in one graph launch.  Instances that share a lattice kernel
+// form a group; each group runs as one persistent kernel over the work items
+// of all its instances (spde_path / spde_stream with an instance table), and
+// the groups run concurrently on forked streams inside one captured HIP
+// graph, after fill kernels that clear the averages.
+namespace {
+
+__global__ void zero_doubles(double* p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+struct BatchGroup {
+  int precision, nx, ny, mat;
+  bool need_tpb1;
+  std::vector<int> inst;
+  std::vector<const xsh::KernelEntry*> cand;
+  const xsh::KernelEntry* kern;
+  int grid, nwork;
+  std::vector<int> wstart;
+  xsh::PathArgs* d_args;
+  int* d_wstart;
+  void* d_work;
+  size_t work_bytes;
+  hipStream_t s;
+  hipEvent_t join;
+};
+
+}  // namespace
+
+struct xsh_batch {
+  std::vector<xsh_handle*> h;
+  std::vector<BatchGroup> g;
+  hipStream_t s0;
+  hipEvent_t ev0, ev1, fork;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  xsh::PathArgs* h_args;  // pinned staging, one per instance
+  bool ran;
+};
+
+static void batch_free(xsh_batch* b) {
+  if (!b) return;
+  if (b->s0) hipStreamSynchronize(b->s0);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  for (auto& g : b->g) {
+    if (g.d_args) hipFree(g.d_args);
+    if (g.d_wstart) hipFree(g.d_wstart);
+    if (g.d_work) hipFree(g.d_work);
+    if (g.join) hipEventDestroy(g.join);
+    if (g.s) hipStreamDestroy(g.s);
+  }
+  for (auto* h : b->h) xsh_destroy(h);
+  if (b->h_args) hipHostFree(b->h_args);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->s0) hipStreamDestroy(b->s0);
+  delete b;
+}
+
+// work items of instance h under kernel k
+static long long batch_items(const xsh_handle* h, const xsh::KernelEntry* k) {
+  return (long long)((h->E + k->tpb - 1) / k->tpb) * h->e2 * h->nchk;
+}
+
+// (re)size a group for its chosen kernel: work table, grid, streaming workspace
+static int batch_layout(BatchGroup& g, const std::vector<xsh_handle*>& hs, int grid) {
+  g.wstart.assign(1, 0);
+  long long tot = 0;
+  for (int i : g.inst) {
+    tot += batch_items(hs[i], g.kern);
+    if (tot > (1LL << 30)) return fail(XSH_EINVAL, "batch group too large");
+    g.wstart.push_back((int)tot);
+  }
+  g.nwork = (int)tot;
+  g.grid = std::max(1, std::min(grid > 0 ? grid : g.nwork, g.nwork));
+  if (g.kern->stream) {
+    g.grid = std::min(g.grid, xsh::kStreamMaxBlocks);
+    const size_t need = (size_t)g.grid * 3 * g.nx * g.ny * (g.precision == XSH_FP64 ? 16 : 8);
+    if (need > g.work_bytes) {
+      if (g.d_work) hipFree(g.d_work);
+      g.d_work = nullptr;
+      g.work_bytes = 0;
+      if (hipMalloc(&g.d_work, need) != hipSuccess) return fail(XSH_ENOMEM, "batch streaming workspace");
+      g.work_bytes = need;
+    }
+  }
+  HIPCHK(hipMemcpy(g.d_wstart, g.wstart.data(), g.wstart.size() * sizeof(int), hipMemcpyHostToDevice));
+  return XSH_OK;
+}
+
+// kernel arguments of every instance of group g for parallel ensemble npar
+static void batch_args(xsh_batch* b, BatchGroup& g, int npar, xsh::PathArgs* out, int npts_cap, int steps_cap) {
+  for (size_t k = 0; k < g.inst.size(); ++k) {
+    const xsh_handle* h = b->h[g.inst[k]];
+    xsh::PathArgs A = make_args(h, npar);
+    A.av = h->d_av;
+    A.groups = (h->E + g.kern->tpb - 1) / g.kern->tpb;
+    if (g.kern->stream) {
+      A.work = g.d_work;
+      A.work_planes = 3;
+    }
+    if (npts_cap > 0) A.npts = std::min(A.npts, npts_cap);
+    if (steps_cap > 0) A.steps = std::min(A.steps, steps_cap);
+    out[k] = A;
+  }
+}
+
+static int batch_upload(xsh_batch* b, int npar) {
+  HIPCHK(hipStreamSynchronize(b->s0));  // the staging buffer is free again
+  int off = 0;
+  for (auto& g : b->g) {
+    batch_args(b, g, npar, b->h_args + off, 0, 0);
+    HIPCHK(hipMemcpyAsync(g.d_args, b->h_args + off, g.inst.size() * sizeof(xsh::PathArgs), hipMemcpyHostToDevice,
+                          b->s0));
+    off += (int)g.inst.size();
+  }
+  return XSH_OK;
+}
+
+// time one shortened launch of group g under its current kernel and grid (ms per trajectory-step)
+static int batch_time(xsh_batch* b, BatchGroup& g, double* ms_per_step) {
+  std::vector<xsh::PathArgs> A(g.inst.size());
+  batch_args(b, g, 1, A.data(), 2, 4);
+  long long steps = 0;
+  for (size_t k = 0; k < g.inst.size(); ++k) {
+    const xsh_handle* h = b->h[g.inst[k]];
+    steps += (long long)h->E * h->e2 * (2 * A[k].steps - 1) * (h->d.checks ? 3 : 1);
+  }
+  HIPCHK(hipMemcpy(g.d_args, A.data(), A.size() * sizeof(xsh::PathArgs), hipMemcpyHostToDevice));
+  float best = 1e30f;
+  for (int rep = 0; rep < 3; ++rep) {  // first launch loads the code object
+    HIPCHK(hipEventRecord(b->ev0, b->s0));
+    hipError_t e = g.kern->launch(g.d_args, g.d_wstart, (int)g.inst.size(), g.nwork, g.grid, b->s0);
+    if (e != hipSuccess) return fail(XSH_EDEVICE, "batch tuning launch: %s", hipGetErrorString(e));
+    HIPCHK(hipEventRecord(b->ev1, b->s0));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (rep > 0) best = std::min(best, ms);
+  }
+  *ms_per_step = best / (double)std::max(1LL, steps);
+  return XSH_OK;
+}
+
+static int batch_capture(xsh_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->s0, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (auto* h : b->h) {
+    const long long n = (long long)h->e2 * h->nchk * h->avlen * 2;
+    hipLaunchKernelGGL(zero_doubles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->s0, h->d_av, n);
+  }
+  e = hipEventRecord(b->fork, b->s0);
+  for (auto& g : b->g) {
+    if (e == hipSuccess) e = hipStreamWaitEvent(g.s, b->fork, 0);
+    if (e == hipSuccess) e = g.kern->launch(g.d_args, g.d_wstart, (int)g.inst.size(), g.nwork, g.grid, g.s);
+    if (e == hipSuccess) e = hipEventRecord(g.join, g.s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->s0, g.join, 0);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e2 = hipStreamEndCapture(b->s0, &graph);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    return fail(XSH_EDEVICE, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  b->graph = graph;
+  HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return XSH_OK;
+}
+
+extern "C" int xsh_batch_create(int n, const xsh_desc* descs, int flags, xsh_batch** out) {
+  if (!out || !descs) return fail(XSH_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (n < 1) return fail(XSH_EINVAL, "a batch needs at least one instance");
+  auto* b = new xsh_batch();
+  b->h.assign(n, nullptr);
+  for (int i = 0; i < n; ++i) {
+    if (descs[i].dimensions < 2) {
+      batch_free(b);
+      return fail(XSH_EUNSUPPORTED, "instance %d: batches hold lattice problems (dimensions >= 2)", i);
+    }
+    int rc = xsh_create(&descs[i], &b->h[i]);
+    if (rc) {
+      const std::string msg = g_err;
+      batch_free(b);
+      return fail(rc, "instance %d: %s", i, msg.c_str());
+    }
+  }
+  // group the instances by lattice kernel family
+  for (int i = 0; i < n; ++i) {
+    const xsh_handle* h = b->h[i];
+    const bool tpb1 = h->d.traj_out || h->d.nmodes;
+    BatchGroup* gp = nullptr;
+    for (auto& g : b->g)
+      if (g.precision == h->d.precision && g.nx == h->nx && g.ny == h->ny && g.mat == h->mat && g.kern == h->kern)
+        gp = &g;
+    if (!gp) {
+      BatchGroup g{};
+      g.precision = h->d.precision;
+      g.nx = h->nx;
+      g.ny = h->ny;
+      g.mat = h->mat;
+      g.kern = h->kern;
+      b->g.push_back(g);
+      gp = &b->g.back();
+    }
+    gp->need_tpb1 = gp->need_tpb1 || tpb1;
+    gp->inst.push_back(i);
+  }
+  auto bail = [&](int rc) {
+    batch_free(b);
+    return rc;
+  };
+  if (hipStreamCreate(&b->s0) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+      hipEventCreate(&b->ev1) != hipSuccess || hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(XSH_EDEVICE, "batch streams/events"));
+  if (hipHostMalloc(&b->h_args, n * sizeof(xsh::PathArgs), hipHostMallocDefault) != hipSuccess)
+    return bail(fail(XSH_ENOMEM, "batch argument staging"));
+  for (auto& g : b->g) {
+    if (hipStreamCreate(&g.s) != hipSuccess || hipEventCreateWithFlags(&g.join, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(XSH_EDEVICE, "batch group stream"));
+    if (hipMalloc(&g.d_args, g.inst.size() * sizeof(xsh::PathArgs)) != hipSuccess ||
+        hipMalloc(&g.d_wstart, (g.inst.size() + 1) * sizeof(int)) != hipSuccess)
+      return bail(fail(XSH_ENOMEM, "batch tables"));
+    // candidate kernels: every compiled variant of this lattice (streaming kernels have one)
+    const xsh::KernelEntry* v[16];
+    const int nv = g.kern->stream ? 0 : xsh::kernel_variants(g.precision, g.nx, g.ny, g.mat, v, 16);
+    g.cand.assign(1, g.kern);
+    for (int k = 0; k < nv; ++k)
+      if (v[k] != g.kern && (!g.need_tpb1 || v[k]->tpb == 1)) g.cand.push_back(v[k]);
+    int rc = batch_layout(g, b->h, 0);
+    if (rc) return bail(rc);
+  }
+  if (flags & XSH_BATCH_AUTOTUNE) {
+    // per group: every variant x persistent grid, timed on a shortened run
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
+    for (auto& g : b->g) {
+      const xsh::KernelEntry* best_k = g.kern;
+      int best_grid = 0;
+      double best = 1e300;
+      for (const auto* k : g.cand) {
+        g.kern = k;
+        const int grids[] = {0, cus, 2 * cus, 4 * cus, 8 * cus};
+        for (int gr : grids) {
+          int rc = batch_layout(g, b->h, gr);
+          if (rc) return bail(rc);
+          if (gr > 0 && g.grid >= g.nwork) continue;  // same as one item per workgroup
+          double t = 0.0;
+          rc = batch_time(b, g, &t);
+          if (rc) return bail(rc);
+          if (t < best) {
+            best = t;
+            best_k = k;
+            best_grid = gr;
+          }
+        }
+      }
+      g.kern = best_k;
+      int rc = batch_layout(g, b->h, best_grid);
+      if (rc) return bail(rc);
+    }
+  }
+  int rc = batch_capture(b);
+  if (rc) return bail(rc);
+  *out = b;
+  return XSH_OK;
+}
+
+extern "C" void xsh_batch_destroy(xsh_batch* b) { batch_free(b); }
+
+extern "C" int xsh_batch_run(xsh_batch* b, int npar) {
+  if (!b) return fail(XSH_EINVAL, "batch is NULL");
+  for (auto* h : b->h)
+    if (npar < 1 || npar > h->e3) return fail(XSH_EINVAL, "npar %d outside 1..ensembles(3)", npar);
+  int rc = batch_upload(b, npar);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(b->ev0, b->s0));
+  HIPCHK(hipGraphLaunch(b->exec, b->s0));
+  HIPCHK(hipEventRecord(b->ev1, b->s0));
+  for (auto* h : b->h) h->ran = false;  // the handles' own streams did not run
+  b->ran = true;
+  return XSH_OK;
+}
+
+extern "C" int xsh_batch_synchronize(xsh_batch* b, double* ms) {
+  if (!b) return fail(XSH_EINVAL, "batch is NULL");
+  HIPCHK(hipStreamSynchronize(b->s0));
+  if (ms) {
+    if (!b->ran) return fail(XSH_ESTATE, "nothing has run on this batch");
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, b->ev0, b->ev1));
+    *ms = f;
+  }
+  return XSH_OK;
+}
+
+extern "C" int xsh_batch_read_averages(xsh_batch* b, int i, double* av_out) {
+  if (!b || !av_out) return fail(XSH_EINVAL, "NULL argument");
+  if (i < 0 || i >= (int)b->h.size()) return fail(XSH_EINVAL, "instance %d outside 0..%d", i, (int)b->h.size() - 1);
+  if (!b->ran) return fail(XSH_ESTATE, "xsh_batch_run has not run");
+  const xsh_handle* h = b->h[i];
+  HIPCHK(hipStreamSynchronize(b->s0));
+  HIPCHK(hipMemcpy(av_out, h->d_av, (size_t)h->e2 * h->nchk * h->avlen * 16, hipMemcpyDeviceToHost));
+  return XSH_OK;
+}
+
+extern "C" int xsh_batch_query(const xsh_batch* b, int i, xsh_info* info) {
+  if (!b || !info) return fail(XSH_EINVAL, "NULL argument");
+  if (i < 0 || i >= (int)b->h.size()) return fail(XSH_EINVAL, "instance %d outside 0..%d", i, (int)b->h.size() - 1);
+  return xsh_query(b->h[i], info);
+}
+
+extern "C" int xsh_batch_plan(const xsh_batch* b, int* ngroups, int* plan) {
+  if (!b || !ngroups) return fail(XSH_EINVAL, "NULL argument");
+  *ngroups = (int)b->g.size();
+  if (plan)
+    for (size_t k = 0; k < b->g.size(); ++k) {
+      const BatchGroup& g = b->g[k];
+      int* p = plan + 8 * k;
+      p[0] = g.precision;
+      p[1] = g.nx;
+      p[2] = g.ny;
+      p[3] = (int)g.inst.size();
+      p[4] = g.kern->threads;
+      p[5] = g.kern->tpb;
+      p[6] = g.kern->radix;
+      p[7] = g.grid;
+    }
   return XSH_OK;
 }

@@ -8,6 +8,20 @@ const KernelEntry* table_f32_2d(int* n);
 const KernelEntry* table_f64(int* n);
 const KernelEntry* table_mat_f32(int* n);
 const KernelEntry* table_mat_f64(int* n);
+const KernelEntry* table_f32_var(int* n);
+
+int kernel_variants(int precision, int nx, int ny, int mat, const KernelEntry** out, int max) {
+  const KernelEntry* (*tables[])(int*) = {table_f32_1d, table_f32_2d, table_f64, table_mat_f32, table_mat_f64,
+                                          table_f32_var};
+  int k = 0;
+  for (auto tab : tables) {
+    int n = 0;
+    const KernelEntry* e = tab(&n);
+    for (int i = 0; i < n && k < max; ++i)
+      if (e[i].precision == precision && e[i].nx == nx && e[i].ny == ny && e[i].mat == mat) out[k++] = &e[i];
+  }
+  return k;
+}
 
 const KernelEntry* find_kernel(int precision, int nx, int ny, int mat) {
   const KernelEntry* (*tables[])(int*) = {table_f32_1d, table_f32_2d, table_f64, table_mat_f32, table_mat_f64};

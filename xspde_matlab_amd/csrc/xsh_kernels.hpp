@@ -216,17 +216,27 @@ __global__ __launch_bounds__(256) void sde_path(const PathArgs* __restrict__ Ap)
 }
 
 // ==================================================================== SPDE
-template <class R, int NX, int NY, int NT, int TPB, bool MAT>
-__global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap) {
-  const PathArgs& A = *Ap;
-  using Lat = Lattice<R, NX, NY, NT, TPB>;
+// Instance of work item wk in a batch: the last i with wstart[i] <= wk
+// (wstart[0] = 0; uniform over the workgroup).
+__device__ __forceinline__ int find_instance(const int* __restrict__ wstart, int ninst, int wk) {
+  int lo = 0, hi = ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (wstart[mid] <= wk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One work item (a group of TPB trajectories of one path) of one instance.
+template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX>
+__device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __restrict__ L, double* red) {
+  using Lat = Lattice<R, NX, NY, NT, TPB, RX>;
   using C = cpx<R>;
   constexpr int CPT = Lat::CPT, PPT = Lat::PPT, NS = Lat::NS;
-  __shared__ C L[Lat::LDS];
-  __shared__ double red[2 * (NT / 64)];
 
   int pidx, nci, j, group;
-  path_identity<R>(A, blockIdx.x, pidx, nci, j, group);
+  path_identity<R>(A, lb, pidx, nci, j, group);
   const uint32_t nsp = (uint32_t)(A.seed + A.npar + (j - 1) * A.e3);
   const R hdt = (R)(nci ? A.hdt[1] : A.hdt[0]);
   Drift<R> D;
@@ -528,11 +538,29 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ Ap)
 #undef XSH_END
 }
 
+// Whole-path kernel over a batch of instances sharing one lattice kernel:
+// work items [wstart[i], wstart[i+1]) belong to instance i (args[i]); a
+// persistent grid strides over all nwork items.  One instance: ninst = 1,
+// wstart unused.
+template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX>
+__global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ args, const int* __restrict__ wstart,
+                                                int ninst, int nwork) {
+  using Lat = Lattice<R, NX, NY, NT, TPB, RX>;
+  __shared__ cpx<R> L[Lat::LDS];
+  __shared__ double red[2 * (NT / 64)];
+#pragma unroll 1
+  for (int wk = blockIdx.x; wk < nwork; wk += gridDim.x) {
+    const int inst = ninst > 1 ? find_instance(wstart, ninst, wk) : 0;
+    spde_work<R, NX, NY, NT, TPB, MAT, RX>(args[inst], wk - (ninst > 1 ? wstart[inst] : 0), L, red);
+    __syncthreads();
+  }
+}
+
 // propagation probe: planes [nplanes][NS] complex R in place
-template <class R, int NX, int NY, int NT, int TPB, bool MAT>
+template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX>
 __global__ __launch_bounds__(NT) void probe_propagate(const PathArgs* __restrict__ Ap, int nci, int nplanes, cpx<R>* planes) {
   const PathArgs& A = *Ap;
-  using Lat = Lattice<R, NX, NY, NT, TPB>;
+  using Lat = Lattice<R, NX, NY, NT, TPB, RX>;
   using C = cpx<R>;
   __shared__ C L[Lat::LDS];
   C f[Lat::PPT];
@@ -579,25 +607,28 @@ static hipError_t launch_sde(const PathArgs* A, int nblocks, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <class R, int NX, int NY, int NT, int TPB, bool MAT>
-static hipError_t launch_spde(const PathArgs* A, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((spde_path<R, NX, NY, NT, TPB, MAT>), dim3(nblocks), dim3(NT), 0, s, A);
+template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX>
+static hipError_t launch_spde(const PathArgs* args, const int* wstart, int ninst, int nwork, int grid,
+                              hipStream_t s) {
+  hipLaunchKernelGGL((spde_path<R, NX, NY, NT, TPB, MAT, RX>), dim3(grid), dim3(NT), 0, s, args, wstart, ninst, nwork);
   return hipGetLastError();
 }
 
-template <class R, int NX, int NY, int NT, int TPB, bool MAT>
+template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX>
 static hipError_t launch_probe(const PathArgs* A, int nci, int nplanes, void* planes, hipStream_t s) {
   const int nb = (nplanes + TPB - 1) / TPB;
-  hipLaunchKernelGGL((probe_propagate<R, NX, NY, NT, TPB, MAT>), dim3(nb), dim3(NT), 0, s, A, nci, nplanes,
+  hipLaunchKernelGGL((probe_propagate<R, NX, NY, NT, TPB, MAT, RX>), dim3(nb), dim3(NT), 0, s, A, nci, nplanes,
                      static_cast<cpx<R>*>(planes));
   return hipGetLastError();
 }
 
-#define XSH_ENTRY(R, NX, NY, NT, TPB)                                                                  \
-  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, 0, &launch_spde<R, NX, NY, NT, TPB, false>, \
-   &launch_probe<R, NX, NY, NT, TPB, false>}
+#define XSH_ENTRY(R, NX, NY, NT, TPB) XSH_VAR_ENTRY(R, NX, NY, NT, TPB, 0)
+// a variant with first x radix RX (0: default split)
+#define XSH_VAR_ENTRY(R, NX, NY, NT, TPB, RX)                                                              \
+  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, 0, &launch_spde<R, NX, NY, NT, TPB, false, RX>, \
+   &launch_probe<R, NX, NY, NT, TPB, false, RX>, 0, Split<NX, RX>::N1}
 #define XSH_MAT_ENTRY(R, NX, NY, NT, TPB)                                                             \
-  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, 1, &launch_spde<R, NX, NY, NT, TPB, true>, \
-   &launch_probe<R, NX, NY, NT, TPB, true>}
+  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, 1, &launch_spde<R, NX, NY, NT, TPB, true, 0>, \
+   &launch_probe<R, NX, NY, NT, TPB, true, 0>, 0, 0}
 
 }  // namespace xsh

@@ -7,7 +7,10 @@ namespace xsh {
 struct PathArgs;
 
 // PathArgs pointers are device pointers (the arguments live in device memory)
-typedef hipError_t (*SpdeLaunch)(const PathArgs*, int nblocks, hipStream_t);
+// args[ninst] instances; work items [wstart[i], wstart[i+1]) belong to instance i
+// (wstart may be NULL for one instance); grid workgroups stride over nwork items
+typedef hipError_t (*SpdeLaunch)(const PathArgs* args, const int* wstart, int ninst, int nwork, int grid,
+                                 hipStream_t);
 typedef hipError_t (*ProbeLaunch)(const PathArgs*, int nci, int nplanes, void* planes, hipStream_t);
 
 struct KernelEntry {
@@ -16,6 +19,7 @@ struct KernelEntry {
   SpdeLaunch launch;
   ProbeLaunch probe;
   int stream;  // fields in a device-memory workspace (lattices too large for the chip)
+  int radix;   // first radix of the x transforms (FFT kernels)
 };
 
 // persistent grid of the streaming kernels: at most this many workgroups,
@@ -23,6 +27,8 @@ struct KernelEntry {
 constexpr int kStreamMaxBlocks = 1024;
 
 const KernelEntry* find_kernel(int precision, int nx, int ny, int mat);
+// all compiled variants (threads, trajectories per workgroup, radix) of a lattice kernel
+int kernel_variants(int precision, int nx, int ny, int mat, const KernelEntry** out, int max);
 const KernelEntry* find_stream_kernel(int precision, int nx, int ny);
 hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s);
 hipError_t launch_probe_normals(int precision, uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t inst,

@@ -138,15 +138,19 @@ __device__ __forceinline__ int opaque_tid() {
   return t;
 }
 
-template <int N>
-struct Split {  // N = N1 * N2, both <= 32, DIF radix order (N1 first)
-  static constexpr int N1 = N <= 4 ? (N >= 2 ? 2 : 1)
+// N = N1 * N2, both <= 32, DIF radix order (N1 first); R1 > 0 chooses N1
+template <int N, int R1 = 0>
+struct Split {
+  static constexpr int N1 = R1 > 0 ? R1
+                          : N <= 4 ? (N >= 2 ? 2 : 1)
                           : N == 8 ? 4 : N == 16 ? 4 : N == 32 ? 4 : N == 64 ? 8
                           : N == 128 ? 16 : N == 256 ? 16 : N == 512 ? 16 : 32;
   static constexpr int N2 = N / N1;
+  static_assert(N1 * N2 == N && N1 <= 32 && N2 <= 32, "radix split must factor N into two factors <= 32");
 };
 
-template <class R, int NX, int NY, int NT, int TPB>
+// RX > 0: first radix of the x transforms (an autotuning choice)
+template <class R, int NX, int NY, int NT, int TPB, int RX = 0>
 struct Lattice {
   static constexpr int NS = NX * NY;
   static constexpr int VEC = 2;
@@ -189,6 +193,7 @@ struct Lattice {
   template <int DIM>
   struct Line {
     static constexpr int N = DIM == 0 ? NX : NY;
+    static constexpr int N1 = Split<N, DIM == 0 ? RX : 0>::N1, N2 = Split<N, DIM == 0 ? RX : 0>::N2;
     static constexpr int NL = DIM == 0 ? TPB * NY : TPB * NX;
     static constexpr int ES = DIM == 0 ? 1 : RS;
     __device__ static __forceinline__ int base(int l) {
@@ -206,7 +211,7 @@ struct Lattice {
   template <int DIM>
   __device__ static __forceinline__ void dif1(C* L, const C* tw) {
     using Ln = Line<DIM>;
-    constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+    constexpr int N1 = Ln::N1, N2 = Ln::N2;
 #pragma unroll 1
     for (int it = opaque_tid(); it < Ln::NL * N2; it += NT) {
       const int l = it % Ln::NL, n2 = it / Ln::NL;
@@ -225,7 +230,7 @@ struct Lattice {
   template <int DIM>
   __device__ static __forceinline__ void dif2(C* L) {
     using Ln = Line<DIM>;
-    constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+    constexpr int N1 = Ln::N1, N2 = Ln::N2;
 #pragma unroll 1
     for (int it = opaque_tid(); it < Ln::NL * N1; it += NT) {
       const int l = it % Ln::NL, k1 = it / Ln::NL;
@@ -242,7 +247,7 @@ struct Lattice {
   template <int DIM>
   __device__ static __forceinline__ void dita(C* L, const C* tw) {
     using Ln = Line<DIM>;
-    constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+    constexpr int N1 = Ln::N1, N2 = Ln::N2;
 #pragma unroll 1
     for (int it = opaque_tid(); it < Ln::NL * N1; it += NT) {
       const int l = it % Ln::NL, k1 = it / Ln::NL;
@@ -261,7 +266,7 @@ struct Lattice {
   template <int DIM, class Op>
   __device__ static __forceinline__ void mid(C* L, const C* tw, const Op& op) {
     using Ln = Line<DIM>;
-    constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+    constexpr int N1 = Ln::N1, N2 = Ln::N2;
 #pragma unroll 1
     for (int it = opaque_tid(); it < Ln::NL * N1; it += NT) {
       const int l = it % Ln::NL, k1 = it / Ln::NL;
@@ -283,7 +288,7 @@ struct Lattice {
   template <int DIM>
   __device__ static __forceinline__ void ditb(C* L) {
     using Ln = Line<DIM>;
-    constexpr int N1 = Split<Ln::N>::N1, N2 = Split<Ln::N>::N2;
+    constexpr int N1 = Ln::N1, N2 = Ln::N2;
 #pragma unroll 1
     for (int it = opaque_tid(); it < Ln::NL * N2; it += NT) {
       const int l = it % Ln::NL, n2 = it / Ln::NL;
@@ -400,7 +405,7 @@ struct Lattice {
 
   // frequency index held at position pos along x after the forward stages
   __device__ static __forceinline__ int kx_of_pos(int pos) {
-    constexpr int N1 = Split<NX>::N1, N2 = Split<NX>::N2;
+    constexpr int N1 = Split<NX, RX>::N1, N2 = Split<NX, RX>::N2;
     return pos / N2 + N1 * (pos % N2);
   }
 

@@ -28,7 +28,7 @@ struct Stream {
   static constexpr int PAIRS = NS / (2 * NT); // 2-site chunks per thread (plane)
   static constexpr int HPAIRS = NSH / (2 * NT);
   static_assert(NY % 2 == 0 && NSH % (2 * NT) == 0, "half-planes must tile the threads");
-  using Lh = Lattice<R, NX, H, NT, 1>;        // LDS geometry of a half-plane
+  using Lh = Lattice<R, NX, H, NT, 1, 0>;     // LDS geometry of a half-plane
   using C = cpx<R>;
   static constexpr int RS = Lh::RS;
 
@@ -140,28 +140,31 @@ struct Stream {
   }
 };
 
+// Work items and instances as in spde_path; the workspace of instance i has
+// a region for every workgroup of the grid (A.work, A.work_planes planes each).
 template <class R, int NX, int NY, int NT>
-__global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ Ap, int nwork) {
-  const PathArgs& A = *Ap;
+__global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ args, const int* __restrict__ wstart,
+                                                  int ninst, int nwork) {
   using S = Stream<R, NX, NY, NT>;
   using C = cpx<R>;
   constexpr int NS = S::NS, PAIRS = S::PAIRS;
   __shared__ C L[S::Lh::LDS];
   __shared__ double red[2 * (NT / 64)];
 
-  C* const F0 = static_cast<C*>(A.work) + (long long)blockIdx.x * A.work_planes * NS;
-  C* const F1 = F0 + NS;
-  C* const F2 = F1 + NS;  // corr observable scratch (present when work_planes == 3)
-  const C* twx = static_cast<const C*>(A.twx);
-  const C* twy = static_cast<const C*>(A.twy);
-  const C* twh = static_cast<const C*>(A.twh);
-  Drift<R> D;
-  D.init(A);
-
 #pragma unroll 1
   for (int wk = blockIdx.x; wk < nwork; wk += gridDim.x) {
+    const int inst = ninst > 1 ? find_instance(wstart, ninst, wk) : 0;
+    const PathArgs& A = args[inst];
+    C* const F0 = static_cast<C*>(A.work) + (long long)blockIdx.x * A.work_planes * NS;
+    C* const F1 = F0 + NS;
+    C* const F2 = F1 + NS;  // corr observable scratch (present when work_planes == 3)
+    const C* twx = static_cast<const C*>(A.twx);
+    const C* twy = static_cast<const C*>(A.twy);
+    const C* twh = static_cast<const C*>(A.twh);
+    Drift<R> D;
+    D.init(A);
     int pidx, nci, j, traj;
-    path_identity<R>(A, wk, pidx, nci, j, traj);
+    path_identity<R>(A, wk - (ninst > 1 ? wstart[inst] : 0), pidx, nci, j, traj);
     const uint32_t nsp = (uint32_t)(A.seed + A.npar + (j - 1) * A.e3);
     const R hdt = (R)(nci ? A.hdt[1] : A.hdt[0]);
     const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
@@ -368,9 +371,9 @@ __global__ __launch_bounds__(NT) void probe_stream(const PathArgs* __restrict__ 
 }
 
 template <class R, int NX, int NY, int NT>
-static hipError_t launch_stream(const PathArgs* A, int nblocks, hipStream_t s) {
-  const int grid = nblocks < kStreamMaxBlocks ? nblocks : kStreamMaxBlocks;
-  hipLaunchKernelGGL((spde_stream<R, NX, NY, NT>), dim3(grid), dim3(NT), 0, s, A, nblocks);
+static hipError_t launch_stream(const PathArgs* args, const int* wstart, int ninst, int nwork, int grid,
+                                hipStream_t s) {
+  hipLaunchKernelGGL((spde_stream<R, NX, NY, NT>), dim3(grid), dim3(NT), 0, s, args, wstart, ninst, nwork);
   return hipGetLastError();
 }
 
@@ -383,6 +386,6 @@ static hipError_t launch_probe_stream(const PathArgs* A, int nci, int nplanes, v
 
 #define XSH_STREAM_ENTRY(R, NX, NY, NT)                                                                 \
   {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, 1, 0, &launch_stream<R, NX, NY, NT>,              \
-   &launch_probe_stream<R, NX, NY, NT>, 1}
+   &launch_probe_stream<R, NX, NY, NT>, 1, Split<NX>::N1}
 
 }  // namespace xsh

@@ -300,3 +300,28 @@ def xsim(p):
     error, outd = statistics(prob.q, data)
     error[-1] = time.time() - t0
     return error, outd, prob.q
+
+
+class BatchProblem:
+    """Many independent projects in one graph launch (uneven lattices, ensembles
+    and drift parameters); each keeps the averages of its own Problem."""
+
+    def __init__(self, problems, autotune=False):
+        self.qs = [preferences(p) for p in problems]
+        self.batch = _lib.Batch([make_desc(q) for q in self.qs], autotune=autotune)
+        self.infos = self.batch.infos
+
+    def run(self, npar=1):
+        self.batch.run(npar)
+
+    def synchronize(self):
+        return self.batch.synchronize()
+
+    def read_averages(self, i):
+        return self.batch.read_averages(i)
+
+    def plan(self):
+        return self.batch.plan()
+
+    def close(self):
+        self.batch.close()
