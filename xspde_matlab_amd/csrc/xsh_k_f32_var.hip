@@ -15,6 +15,7 @@ static const KernelEntry kEntries[] = {
     XSH_VAR_ENTRY(float, 512, 1, 256, 4, 0), XSH_VAR_ENTRY(float, 512, 1, 256, 8, 32),
     // 2D: threads per trajectory
     XSH_VAR_ENTRY(float, 32, 32, 256, 2, 0), XSH_VAR_ENTRY(float, 64, 64, 256, 1, 0),
+    XSH_VAR_ENTRY(float, 64, 64, 1024, 1, 0), XSH_VAR_ENTRY(float, 128, 128, 512, 1, 0),
 };
 
 const KernelEntry* table_f32_var(int* n) {
