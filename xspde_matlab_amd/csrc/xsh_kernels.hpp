@@ -328,20 +328,60 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     XSH_END
   };
 
+  // Pointwise stages visit the register chunks (2 sites each) of one or two
+  // fields.  2D lattices run a runtime loop that always works on slots 0, 1
+  // and then rotates the arrays by one chunk (register moves, static indices):
+  // one copy of the noise generator and drift per stage instead of CPT keeps
+  // the time step inside the instruction cache (128x128: 3.23 s -> 2.99 s per
+  // launch).  1D lattices, with few chunks, unroll.
+  auto rotate = [&](C (&f)[PPT]) __attribute__((always_inline)) {
+    const C t0 = f[0], t1 = f[1];
+    XSH_FOR(k, 0, PPT - 2) f[k] = f[k + 2]; XSH_END
+    f[PPT - 2] = t0;
+    f[PPT - 1] = t1;
+  };
+  // body(i, a0, a1, b0, b1): chunk i of fields a and b (distinct arrays), in place
+  auto chunks2 = [&](C (&a)[PPT], C (&b)[PPT], auto&& body) __attribute__((always_inline)) {
+    if constexpr (NY > 1) {
+#pragma unroll 1
+      for (int i = 0; i < CPT; ++i) {
+        body(i, a[0], a[1], b[0], b[1]);
+        rotate(a);
+        rotate(b);
+      }
+    } else {
+      XSH_FOR(i, 0, CPT)
+        body(i, a[2 * i], a[2 * i + 1], b[2 * i], b[2 * i + 1]);
+        __builtin_amdgcn_sched_barrier(0);
+      XSH_END
+    }
+  };
+  auto chunks1 = [&](C (&a)[PPT], auto&& body) __attribute__((always_inline)) {
+    if constexpr (NY > 1) {
+#pragma unroll 1
+      for (int i = 0; i < CPT; ++i) {
+        body(i, a[0], a[1]);
+        rotate(a);
+      }
+    } else {
+      XSH_FOR(i, 0, CPT)
+        body(i, a[2 * i], a[2 * i + 1]);
+        __builtin_amdgcn_sched_barrier(0);
+      XSH_END
+    }
+  };
+
   auto method_step = [&](uint32_t q, bool coarse) __attribute__((always_inline)) {
     const bool rk4 = A.method == XSH_RK4;
     if (rk4) {
       // RK4.m:9-14: a1 = a + D(a) dt, into f1 (a stays in f0)
       setbound(f0);
-      XSH_FOR(i, 0, CPT)
-        C w[2];
-        noise_chunk(i, q, coarse, w[0], w[1]);
-        XSH_FOR(v, 0, 2)
-          constexpr int k = 2 * i + v;
-          f1[k] = f0[k] + hdt * D(f0[k], w[v]);
-        XSH_END
-        __builtin_amdgcn_sched_barrier(0);
-      XSH_END
+      chunks2(f0, f1, [&](int i, C& a0, C& a1, C& b0, C& b1) __attribute__((always_inline)) {
+        C w0, w1;
+        noise_chunk(i, q, coarse, w0, w1);
+        b0 = a0 + hdt * D(a0, w0);
+        b1 = a1 + hdt * D(a1, w1);
+      });
     }
     // RK4: P a1 | swap | P a, stage 2 | P a3 | swap | P R
     // MP : swap | P a, midpoint passes | P a | swap
@@ -352,41 +392,41 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     const int nprop = rk4 ? 4 : 2;
     // RK4.m:18-32 per site with A in `a` and A1 in `b`: a3 -> a, R -> b
     auto rk4_stage2 = [&](C (&a)[PPT], C (&b)[PPT]) __attribute__((always_inline)) {
-      XSH_FOR(i, 0, CPT)
-        C w[2];
-        noise_chunk(i, q, coarse, w[0], w[1]);
-        XSH_FOR(v, 0, 2)
-          constexpr int k = 2 * i + v;
-          const C A0 = a[k], A1 = b[k];
+      chunks2(a, b, [&](int i, C& a0, C& a1, C& b0, C& b1) __attribute__((always_inline)) {
+        C w0, w1;
+        noise_chunk(i, q, coarse, w0, w1);
+        auto site = [&](int v, C& av, C& bv, C w) __attribute__((always_inline)) {
+          const C A0 = av, A1 = bv;
           const bool dz = MAT && A.dirichlet && dir_site(i, v);
-          C a2 = A0 + hdt * D(A1, w[v]);
+          C a2 = A0 + hdt * D(A1, w);
           if (dz) a2 = C{R(0), R(0)};
-          C a3 = A0 + (R(2) * hdt) * D(a2, w[v]);
+          C a3 = A0 + (R(2) * hdt) * D(a2, w);
           if (dz) a3 = C{R(0), R(0)};
-          b[k] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
-          a[k] = a3;
-        XSH_END
-        __builtin_amdgcn_sched_barrier(0);
-      XSH_END
+          bv = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
+          av = a3;
+        };
+        site(0, a0, b0, w0);
+        site(1, a1, b1, w1);
+      });
     };
     // MP.m:56-67: fixed-point passes at the midpoint, per site, on f
     auto mp_stage = [&](C (&f)[PPT]) __attribute__((always_inline)) {
-      XSH_FOR(i, 0, CPT)
-        C w[2];
-        noise_chunk(i, q, coarse, w[0], w[1]);
-        XSH_FOR(v, 0, 2)
-          constexpr int k = 2 * i + v;
-          const C a0 = f[k];
+      chunks1(f, [&](int i, C& x0, C& x1) __attribute__((always_inline)) {
+        C w0, w1;
+        noise_chunk(i, q, coarse, w0, w1);
+        auto site = [&](int v, C& xv, C w) __attribute__((always_inline)) {
+          const C a0 = xv;
           const bool dz = MAT && A.dirichlet && dir_site(i, v);
           C a = a0;
           for (int it = 0; it < A.iterations; ++it) {
-            a = a0 + hdt * D(a, w[v]);
+            a = a0 + hdt * D(a, w);
             if (dz) a = C{R(0), R(0)};
           }
-          f[k] = R(2) * a - a0;
-        XSH_END
-        __builtin_amdgcn_sched_barrier(0);
-      XSH_END
+          xv = R(2) * a - a0;
+        };
+        site(0, x0, w0);
+        site(1, x1, w1);
+      });
     };
     // FFT kernels: A = P a in f0, A1 = P a1 in f1 -> a3 in f0, R in f1
     auto swap_free_stage2 = [&]() __attribute__((always_inline)) { rk4_stage2(f0, f1); };
@@ -413,15 +453,12 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     }
     if (rk4) {
       // RK4.m:33-41: a = P R + D(P a3) dt/3 with P a3 in f0, P R in f1
-      XSH_FOR(i, 0, CPT)
-        C w[2];
-        noise_chunk(i, q, coarse, w[0], w[1]);
-        XSH_FOR(v, 0, 2)
-          constexpr int k = 2 * i + v;
-          f0[k] = f1[k] + (hdt / R(3)) * D(f0[k], w[v]);
-        XSH_END
-        __builtin_amdgcn_sched_barrier(0);
-      XSH_END
+      chunks2(f0, f1, [&](int i, C& a0, C& a1, C& b0, C& b1) __attribute__((always_inline)) {
+        C w0, w1;
+        noise_chunk(i, q, coarse, w0, w1);
+        a0 = b0 + (hdt / R(3)) * D(a0, w0);
+        a1 = b1 + (hdt / R(3)) * D(a1, w1);
+      });
       setbound(f0);
     }
   };
