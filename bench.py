@@ -108,7 +108,7 @@ def main():
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--boundaries", default="periodic", choices=["periodic", "neumann"],
                     help="periodic (FFT) or Neumann-Neumann (DCT-I) lattice")
-    ap.add_argument("--workload", default="cgl2d", choices=["cgl2d", "tw", "batch"])
+    ap.add_argument("--workload", default="cgl2d", choices=["cgl2d", "nls1d", "tw", "batch"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs a single MI355X (--gpus %d not supported)"
@@ -120,11 +120,16 @@ def main():
         return bench_batch(args)
 
     from xspde_matlab_amd import Problem
-    from xspde_matlab_amd.configs import cgl2d
+    from xspde_matlab_amd.configs import cgl2d, nls1d
 
-    p = cgl2d(args.precision)
-    if args.boundaries == "neumann":
-        p["boundaries"] = [[-1, -1], [-1, -1]]
+    if args.workload == "nls1d":
+        p = nls1d(args.precision)
+        if args.boundaries == "neumann":
+            p["boundaries"] = [[-1, -1]]
+    else:
+        p = cgl2d(args.precision)
+        if args.boundaries == "neumann":
+            p["boundaries"] = [[-1, -1], [-1, -1]]
     log("bench: creating problem %s %s %s" % (p["name"], p["points"], p["gpu"]["precision"]))
     prob = Problem(p)
     info = prob.info
@@ -148,7 +153,8 @@ def main():
     alg_bytes = traj_steps * 2 * ns * esize            # state read + write per trajectory-step
     achieved = alg_bytes / (kavg / 1e3)
     flops = traj_steps * step_flops(ns) / (kavg / 1e3)
-    wl = "cgl2d128-%s-%s" % (args.precision, args.boundaries)
+    kernel = ("spde_path", "spde_stream", "spde_path", "sde_path")[info.kernel_kind]
+    wl = "%s-%s-%s" % ("nls1d256" if args.workload == "nls1d" else "cgl2d128", args.precision, args.boundaries)
     # a sanity read of the moments so the run is checked, outside the timed region
     av = prob.handle.read_averages()
     import numpy as np
@@ -158,20 +164,30 @@ def main():
     if not args.no_cpu:
         log("bench: timing the CPU oracle sample")
         cpu = cpu_baseline()
+    tname = "FFT" if args.boundaries == "periodic" else "DCT-I"
+    if args.workload == "nls1d":
+        metric = "trajectory-steps/s (1D 256 RK4IP stochastic NLS)"
+        config = {"workload": "1D stochastic NLS 256 sites %s, RK4IP+%s, 4096 trajectories, 1000 coarse "
+                              "steps with checks (3000 traj-steps/trajectory)" % (args.boundaries, tname),
+                  "lattice": [256], "trajectories": 4096, "ensembles": p["ensembles"], "points": 101,
+                  "steps": 10, "checks": 1}
+    else:
+        metric = "trajectory-steps/s (2D 128^2 RK4IP, FP32 vs FP64) + wall time to fixed error bar"
+        config = {"workload": "2D CGL SPDE 128x128 %s, RK4IP+%s, 1024 trajectories, 2000 coarse steps "
+                              "with checks (6000 traj-steps/trajectory)" % (args.boundaries, tname),
+                  "lattice": [128, 128], "trajectories": 1024, "ensembles": [64, 16, 1],
+                  "points": 101, "steps": 20, "checks": 1}
+    config.update(traj_steps_per_step=traj_steps, precision=args.precision, moments="fp64", finite=finite,
+                  kernel=kernel, threads=info.threads, traj_per_block=info.traj_per_block)
     line = {
-        "metric": "trajectory-steps/s (2D 128^2 RK4IP, FP32 vs FP64) + wall time to fixed error bar",
+        "metric": metric,
         "value": rate, "unit": "trajectory-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
-        "config": {"workload": "2D CGL SPDE 128x128 %s, RK4IP+%s, 1024 trajectories, "
-                               "2000 coarse steps with checks (6000 traj-steps/trajectory)"
-                               % (args.boundaries, "FFT" if args.boundaries == "periodic" else "DCT-I"),
-                   "lattice": [128, 128], "trajectories": 1024, "ensembles": [64, 16, 1],
-                   "points": 101, "steps": 20, "checks": 1, "traj_steps_per_step": traj_steps,
-                   "precision": args.precision, "moments": "fp64", "finite": finite},
+        "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic("spde_path", wl),
-                     "kernel": "spde_path", "kernel_ms": kavg,
+                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(kernel, wl),
+                     "kernel": kernel, "kernel_ms": kavg,
                      "valu": {"achieved": flops / 1e12, "peak": VALU_PEAK[args.precision] / 1e12,
                               "unit": "TFLOP/s", "frac": flops / VALU_PEAK[args.precision],
                               "flops_per_traj_step": step_flops(ns)}},

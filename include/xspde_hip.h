@@ -115,6 +115,8 @@ typedef struct {
   int obs_sites[XSH_MAX_OBS];        /* sites per time point of observable o      */
   long long traj_steps;     /* trajectory-steps executed by xsh_run_ensemble    */
   int threads, traj_per_block; /* launch shape of the path kernel              */
+  int kernel_kind;          /* 0 register-resident FFT, 1 streaming (fields in
+                               device memory), 2 dense transforms, 3 SDE       */
 } xsh_info;
 
 /* Library version (XSH_ABI_VERSION). */

@@ -1,0 +1,56 @@
+"""FP32 against FP64 on the GPU, per observable, on the same seeded streams
+(the north-star's precision statement): 1D stochastic NLS at its full time
+span (1000 coarse RK4IP steps with checks) and 2D CGL at its full span
+(2000 coarse steps), fewer trajectories.  The FP32 error of each ensemble
+mean is stated relative to its FP64 value and to the sampling error bar."""
+import numpy as np
+import pytest
+
+from xspde_matlab_amd import Problem
+from xspde_matlab_amd.configs import cgl2d, nls1d, shrink
+
+pytestmark = pytest.mark.gpu
+
+
+def _means(p):
+    prob = Problem(p)
+    prob.handle.run_ensemble(1)
+    prob.handle.synchronize()
+    av = prob.handle.read_averages()          # [e2, nchk, avlen]
+    info = prob.info
+    prob.close()
+    return av, info
+
+
+def _compare(p32, p64, names, bound_rel, bound_sigma):
+    a32, info = _means(p32)
+    a64, _ = _means(p64)
+    e2 = a64.shape[0]
+    for o, name in enumerate(names):
+        off, ns = info.obs_offset[o], info.obs_sites[o]
+        npts = p64["points"][0]
+        m32 = a32[:, 0, off:off + npts * ns].real
+        m64 = a64[:, 0, off:off + npts * ns].real
+        diff = np.abs(m32.mean(0) - m64.mean(0))
+        scale = np.abs(m64.mean(0)).max()
+        sigma = m64.std(0, ddof=1) / np.sqrt(e2)
+        rel = diff.max() / scale
+        # points with no sampling spread (t = 0 of a deterministic initial state) are
+        # judged by the relative bound alone
+        spread = sigma > 1e-9 * scale
+        in_sigma = (diff[spread] / sigma[spread]).max() if spread.any() else 0.0
+        print("%s %s: max |fp32 - fp64| / max|fp64| = %.2e, max |fp32 - fp64| / sigma = %.2e"
+              % (p64["name"], name, rel, in_sigma), flush=True)
+        assert rel < bound_rel and in_sigma < bound_sigma, name
+
+
+def test_nls1d_fp32_vs_fp64_full_span():
+    p64 = shrink(nls1d("fp64"), E=512, e2=8)
+    p32 = shrink(nls1d("fp32"), E=512, e2=8)
+    _compare(p32, p64, ["abs2", "a"], 1e-3, 0.1)
+
+
+def test_cgl2d_fp32_vs_fp64_full_span():
+    p64 = shrink(cgl2d("fp64"), E=32, e2=4)
+    p32 = shrink(cgl2d("fp32"), E=32, e2=4)
+    _compare(p32, p64, ["abs2_ave", "corr"], 1e-3, 0.1)

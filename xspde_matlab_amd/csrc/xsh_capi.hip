@@ -474,6 +474,7 @@ extern "C" int xsh_query(const xsh_handle* h, xsh_info* info) {
   info->traj_steps = (long long)h->E * h->e2 * per * (h->d.checks ? 3 : 1);
   info->threads = h->threads;
   info->traj_per_block = h->tpb;
+  info->kernel_kind = h->nd == 1 ? 3 : h->mat ? 2 : (h->kern && h->kern->stream) ? 1 : 0;
   return XSH_OK;
 }
 

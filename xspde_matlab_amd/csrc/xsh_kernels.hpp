@@ -329,23 +329,27 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   };
 
   // Pointwise stages visit the register chunks (2 sites each) of one or two
-  // fields.  2D lattices run a runtime loop that always works on slots 0, 1
-  // and then rotates the arrays by one chunk (register moves, static indices):
-  // one copy of the noise generator and drift per stage instead of CPT keeps
-  // the time step inside the instruction cache (128x128: 3.23 s -> 2.99 s per
-  // launch).  1D lattices, with few chunks, unroll.
+  // fields.  2D lattices run a runtime loop that works on the first UR chunks
+  // and then rotates the arrays by UR chunks (register moves, static indices):
+  // UR copies of the noise generator and drift per stage instead of CPT keep
+  // the time step inside the instruction cache (128x128 per launch: unrolled
+  // 3.23 s, UR = 1 3.03 s, UR = 2 2.94 s, UR = 4 3.04 s).  1D lattices, with
+  // few chunks, unroll.
+  constexpr int UR = CPT % 2 == 0 ? 2 : 1;
   auto rotate = [&](C (&f)[PPT]) __attribute__((always_inline)) {
-    const C t0 = f[0], t1 = f[1];
-    XSH_FOR(k, 0, PPT - 2) f[k] = f[k + 2]; XSH_END
-    f[PPT - 2] = t0;
-    f[PPT - 1] = t1;
+    C t[2 * UR];
+    XSH_FOR(k, 0, 2 * UR) t[k] = f[k]; XSH_END
+    XSH_FOR(k, 0, PPT - 2 * UR) f[k] = f[k + 2 * UR]; XSH_END
+    XSH_FOR(k, 0, 2 * UR) f[PPT - 2 * UR + k] = t[k]; XSH_END
   };
   // body(i, a0, a1, b0, b1): chunk i of fields a and b (distinct arrays), in place
   auto chunks2 = [&](C (&a)[PPT], C (&b)[PPT], auto&& body) __attribute__((always_inline)) {
     if constexpr (NY > 1) {
 #pragma unroll 1
-      for (int i = 0; i < CPT; ++i) {
-        body(i, a[0], a[1], b[0], b[1]);
+      for (int i = 0; i < CPT; i += UR) {
+        XSH_FOR(u, 0, UR)
+          body(i + u, a[2 * u], a[2 * u + 1], b[2 * u], b[2 * u + 1]);
+        XSH_END
         rotate(a);
         rotate(b);
       }
@@ -359,8 +363,10 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   auto chunks1 = [&](C (&a)[PPT], auto&& body) __attribute__((always_inline)) {
     if constexpr (NY > 1) {
 #pragma unroll 1
-      for (int i = 0; i < CPT; ++i) {
-        body(i, a[0], a[1]);
+      for (int i = 0; i < CPT; i += UR) {
+        XSH_FOR(u, 0, UR)
+          body(i + u, a[2 * u], a[2 * u + 1]);
+        XSH_END
         rotate(a);
       }
     } else {
