@@ -21,7 +21,7 @@ SOURCES = ["xsh_k_f32_2d.hip", "xsh_k_mat_f32.hip", "xsh_k_mat_f64.hip", "xsh_k_
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-munsafe-fp-atomics", "-Wno-unused-result", "-Wno-unused-value"]
+         "-munsafe-fp-atomics", "-Wno-unused-result"]
 
 
 def _compile(src, objdir, extra):

@@ -441,21 +441,21 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
 
 extern "C" void xsh_destroy(xsh_handle* h) {
   if (!h) return;
-  if (h->stream) hipStreamSynchronize(h->stream);
-  if (h->d_tables) hipFree(h->d_tables);
-  if (h->d_av) hipFree(h->d_av);
-  if (h->d_path_av) hipFree(h->d_path_av);
-  if (h->d_a) hipFree(h->d_a);
-  if (h->d_traj) hipFree(h->d_traj);
-  if (h->d_feat) hipFree(h->d_feat);
-  if (h->d_fmap) hipFree(h->d_fmap);
-  if (h->d_work) hipFree(h->d_work);
-  if (h->ev0) hipEventDestroy(h->ev0);
-  if (h->ev1) hipEventDestroy(h->ev1);
-  if (h->ev_args) hipEventDestroy(h->ev_args);
-  if (h->d_args) hipFree(h->d_args);
-  if (h->h_args) hipHostFree(h->h_args);
-  if (h->stream) hipStreamDestroy(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->d_tables) (void)hipFree(h->d_tables);
+  if (h->d_av) (void)hipFree(h->d_av);
+  if (h->d_path_av) (void)hipFree(h->d_path_av);
+  if (h->d_a) (void)hipFree(h->d_a);
+  if (h->d_traj) (void)hipFree(h->d_traj);
+  if (h->d_feat) (void)hipFree(h->d_feat);
+  if (h->d_fmap) (void)hipFree(h->d_fmap);
+  if (h->d_work) (void)hipFree(h->d_work);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->ev_args) (void)hipEventDestroy(h->ev_args);
+  if (h->d_args) (void)hipFree(h->d_args);
+  if (h->h_args) (void)hipHostFree(h->h_args);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
@@ -664,7 +664,7 @@ extern "C" int xsh_probe_normals(xsh_handle* h, int nsp, int tag, int draw, int 
                                            (uint32_t)h->d.inst, (uint32_t)draw, traj0, ntraj, count, d, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return fail(XSH_EDEVICE, "probe normals: %s", hipGetErrorString(e));
   return XSH_OK;
 }
@@ -684,14 +684,14 @@ extern "C" int xsh_probe_propagate(xsh_handle* h, int nc, int nplanes, double* p
   HIPCHK(hipMalloc(&d, buf.size()));
   int rc = upload_args(h, A);
   if (rc) {
-    hipFree(d);
+    (void)hipFree(d);
     return rc;
   }
   hipError_t e = hipMemcpyAsync(d, buf.data(), buf.size(), hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess) e = h->kern->probe(h->d_args, nc - 1, nplanes, d, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(buf.data(), d, buf.size(), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return fail(XSH_EDEVICE, "probe propagate: %s", hipGetErrorString(e));
   if (h->d.precision == XSH_FP64)
     std::memcpy(planes, buf.data(), n * 16);
@@ -745,22 +745,22 @@ struct xsh_batch {
 
 static void batch_free(xsh_batch* b) {
   if (!b) return;
-  if (b->s0) hipStreamSynchronize(b->s0);
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->s0) (void)hipStreamSynchronize(b->s0);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
   for (auto& g : b->g) {
-    if (g.d_args) hipFree(g.d_args);
-    if (g.d_wstart) hipFree(g.d_wstart);
-    if (g.d_work) hipFree(g.d_work);
-    if (g.join) hipEventDestroy(g.join);
-    if (g.s) hipStreamDestroy(g.s);
+    if (g.d_args) (void)hipFree(g.d_args);
+    if (g.d_wstart) (void)hipFree(g.d_wstart);
+    if (g.d_work) (void)hipFree(g.d_work);
+    if (g.join) (void)hipEventDestroy(g.join);
+    if (g.s) (void)hipStreamDestroy(g.s);
   }
   for (auto* h : b->h) xsh_destroy(h);
-  if (b->h_args) hipHostFree(b->h_args);
-  if (b->ev0) hipEventDestroy(b->ev0);
-  if (b->ev1) hipEventDestroy(b->ev1);
-  if (b->fork) hipEventDestroy(b->fork);
-  if (b->s0) hipStreamDestroy(b->s0);
+  if (b->h_args) (void)hipHostFree(b->h_args);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->s0) (void)hipStreamDestroy(b->s0);
   delete b;
 }
 
@@ -784,7 +784,7 @@ static int batch_layout(BatchGroup& g, const std::vector<xsh_handle*>& hs, int g
     g.grid = std::min(g.grid, xsh::kStreamMaxBlocks);
     const size_t need = (size_t)g.grid * 3 * g.nx * g.ny * (g.precision == XSH_FP64 ? 16 : 8);
     if (need > g.work_bytes) {
-      if (g.d_work) hipFree(g.d_work);
+      if (g.d_work) (void)hipFree(g.d_work);
       g.d_work = nullptr;
       g.work_bytes = 0;
       if (hipMalloc(&g.d_work, need) != hipSuccess) return fail(XSH_ENOMEM, "batch streaming workspace");
@@ -850,8 +850,8 @@ static int batch_time(xsh_batch* b, BatchGroup& g, double* ms_per_step) {
 }
 
 static int batch_capture(xsh_batch* b) {
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
   HIPCHK(hipStreamBeginCapture(b->s0, hipStreamCaptureModeThreadLocal));
@@ -870,7 +870,7 @@ static int batch_capture(xsh_batch* b) {
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(b->s0, &graph);
   if (e != hipSuccess || e2 != hipSuccess) {
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
     return fail(XSH_EDEVICE, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
   }
   b->graph = graph;

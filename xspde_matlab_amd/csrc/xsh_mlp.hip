@@ -239,11 +239,11 @@ static void mlp_free(xsh_mlp* h) {
   float** bufs[] = {&h->th, &h->g, &h->m, &h->v, &h->X, &h->T, &h->Y, &h->Z1, &h->H1, &h->Z2,
                     &h->H2, &h->dY, &h->dZ2, &h->dZ1, &h->ones, &h->Xall, &h->Tall};
   for (auto b : bufs)
-    if (*b) hipFree(*b);
-  if (h->idx) hipFree(h->idx);
-  if (h->kxy) hipFree(h->kxy);
-  if (h->dacc) hipFree(h->dacc);
-  if (h->s) hipStreamDestroy(h->s);
+    if (*b) (void)hipFree(*b);
+  if (h->idx) (void)hipFree(h->idx);
+  if (h->kxy) (void)hipFree(h->kxy);
+  if (h->dacc) (void)hipFree(h->dacc);
+  if (h->s) (void)hipStreamDestroy(h->s);
   delete h;
 }
 
@@ -275,9 +275,11 @@ extern "C" int xsh_mlp_create(int nin, int nh, int batch_max, const float* theta
     mlp_free(h);
     return mfail(XSH_EDEVICE, "allocation");
   }
-  hipMemcpy(h->th, theta0, P * sizeof(float), hipMemcpyHostToDevice);
-  hipMemset(h->m, 0, P * sizeof(float));
-  hipMemset(h->v, 0, P * sizeof(float));
+  if (hipMemcpy(h->th, theta0, P * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(h->m, 0, P * sizeof(float)) != hipSuccess || hipMemset(h->v, 0, P * sizeof(float)) != hipSuccess) {
+    mlp_free(h);
+    return mfail(XSH_EDEVICE, "initial parameters");
+  }
   hipLaunchKernelGGL(fill_kernel, dim3((B + 255) / 256), dim3(256), 0, h->s, h->ones, (long long)B, 1.0f);
   if (hipStreamSynchronize(h->s) != hipSuccess) {
     mlp_free(h);
@@ -288,7 +290,7 @@ extern "C" int xsh_mlp_create(int nin, int nh, int batch_max, const float* theta
 }
 
 extern "C" void xsh_mlp_destroy(xsh_mlp* h) {
-  if (h && h->s) hipStreamSynchronize(h->s);
+  if (h && h->s) (void)hipStreamSynchronize(h->s);
   mlp_free(h);
 }
 
@@ -376,9 +378,9 @@ extern "C" int xsh_mlp_train(xsh_mlp* h, int n, const float* x, const float* t, 
   if (!h || !x || !t || !order) return mfail(XSH_EINVAL, "NULL argument");
   if (n < 1 || epochs < 1 || batch < 1 || batch > h->bmax) return mfail(XSH_EINVAL, "bad training sizes");
   if (h->nall < n) {
-    if (h->Xall) hipFree(h->Xall);
-    if (h->Tall) hipFree(h->Tall);
-    if (h->idx) hipFree(h->idx);
+    if (h->Xall) (void)hipFree(h->Xall);
+    if (h->Tall) (void)hipFree(h->Tall);
+    if (h->idx) (void)hipFree(h->idx);
     h->Xall = h->Tall = nullptr;
     h->idx = nullptr;
     if (hipMalloc(&h->Xall, (size_t)n * h->nin * sizeof(float)) != hipSuccess ||
@@ -387,7 +389,7 @@ extern "C" int xsh_mlp_train(xsh_mlp* h, int n, const float* x, const float* t, 
       return mfail(XSH_ENOMEM, "training set");
     h->nall = n;
   } else {
-    if (h->idx) hipFree(h->idx);
+    if (h->idx) (void)hipFree(h->idx);
     if (hipMalloc(&h->idx, (size_t)n * epochs * sizeof(int)) != hipSuccess) return mfail(XSH_ENOMEM, "order");
   }
   MCHK(hipMemcpyAsync(h->Xall, x, (size_t)n * h->nin * sizeof(float), hipMemcpyHostToDevice, h->s));
@@ -429,7 +431,7 @@ extern "C" int xsh_mlp_set_translations(xsh_mlp* h, int nmodes, const int* kxy, 
   if (!h) return mfail(XSH_EINVAL, "NULL handle");
   if (h->kxy) {
     MCHK(hipStreamSynchronize(h->s));
-    hipFree(h->kxy);
+    (void)hipFree(h->kxy);
     h->kxy = nullptr;
   }
   if (nmodes == 0) return XSH_OK;
