@@ -97,3 +97,22 @@ def test_refined_lattice_and_defaults():
     assert q["method"] == "MP"
     q = preferences(dict(dimensions=2, gpu=dict(deriv="zero")))
     assert q["points"] == [51, 35] and q["method"] == "RK4" and q["checks"] == [1, 0]
+
+
+def test_batch_instances_generate_valid_descriptors():
+    """The uneven batch workload: 64 instances over every lattice family, with
+    the drawn parameters inside their stated ranges and descriptors that pass
+    the host-side form checks."""
+    from xspde_matlab_amd.configs import BATCH_1D, BATCH_2D, BATCH_E, batch_instances
+    from xspde_matlab_amd.xsim import make_desc, preferences
+    ps = batch_instances(count=64, seed=0)
+    shapes = {tuple(p["points"][1:]) for p in ps}
+    assert shapes == {(n,) for n in BATCH_1D} | {(n, n) for n in BATCH_2D}
+    for p in ps:
+        g = p["gpu"]
+        gamma, gnl = -g["u"], -g["g"].imag
+        assert 0.01 <= gamma <= 0.5 and 0.1 <= gnl <= 2.0
+        assert p["ensembles"][0] * p["ensembles"][1] in BATCH_E
+        assert (p["points"][0] - 1) * p["steps"][0] == 500
+        d = make_desc(preferences(p))
+        assert d.dimensions == len(p["points"]) and d.method == 0

@@ -183,7 +183,8 @@ def test_cgl2d_neumann_path(prec, tol, n):
     """Config 3-NN: 2D CGL with Neumann ends (DCT-I both dimensions)."""
     p = shrink(cgl2d(prec, n=n), E=2, e2=2, points=3, steps=2)
     p["boundaries"] = [[-1, -1], [-1, -1]]
-    p["gpu"]["observe"] = ["abs2_ave", "abs2"]
+    # corr is the periodic FFT correlation of the field whatever the boundaries
+    p["gpu"]["observe"] = ["abs2_ave", "abs2", "corr"]
     _path_case(p, tol)
 
 

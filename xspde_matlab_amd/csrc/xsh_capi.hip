@@ -53,8 +53,10 @@ struct xsh_handle {
   long long obs_off[XSH_MAX_OBS];
   int obs_ns[XSH_MAX_OBS];
   size_t esize;
-  void* d_tables;   // px[2], py[2], twx, twy, transform matrices, half-length twiddles
-  size_t tab_off[11];
+  void* d_tables;   // px[2], py[2], twx, twy, transform matrices, half-length twiddles,
+                    // matrix-core operand copies of the real matrices
+  size_t tab_off[15];
+  bool has_tmp[4];
   void* d_work;      // streaming kernels: field planes per workgroup
   int work_planes;
   int btype[2];      // boundary type per space dimension (0 periodic)
@@ -262,14 +264,9 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     }
     h->threads = h->kern->threads;
     h->tpb = h->kern->tpb;
-    // spectral observables and noise modes use the lattice FFT of the kernel
-    bool spectral_obs = d.nmodes > 0;
-    for (int o = 0; o < d.nobserve; ++o)
-      spectral_obs = spectral_obs || d.observe[o] == XSH_OBS_CORR || d.observe[o] == XSH_OBS_LOWK;
-    if (spectral_obs && h->mat) {
-      delete h;
-      return fail(XSH_EUNSUPPORTED, "corr/lowk observables and noise modes need a periodic lattice with an FFT kernel");
-    }
+    // corr, lowk and the noise modes are periodic FFTs of the field whatever the
+    // boundaries (the dense-transform kernels have power-of-two lattices and the
+    // FFT tables too); the streaming kernel computes corr only
     if (h->kern->stream) {
       bool lowk = false;
       for (int o = 0; o < d.nobserve; ++o) lowk = lowk || d.observe[o] == XSH_OBS_LOWK;
@@ -374,6 +371,25 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   }
   h->tab_off[10] = total;
   total += ((twh.size() * h->esize + 255) / 256) * 256;
+  // FP32 real matrices of 2D lattices with N % 32 == 0: the same entries in the
+  // operand order of the 32x32x2 matrix-core instruction, Tp[kb][kk/2][lane] =
+  // T[32 kb + lane%32][kk + lane/32] (used by kernels that run those passes on MFMA)
+  std::vector<std::vector<double>> mpk(4);
+  for (int i = 0; i < 4; ++i) {
+    const int k = i % 2, n = npt[k];
+    h->has_tmp[i] = h->mat && d.dimensions == 3 && d.precision == XSH_FP32 && !cplx[k] && n % 32 == 0 &&
+                    mats[i].size() == (size_t)n * n;
+    if (!h->has_tmp[i]) continue;
+    mpk[i].resize((size_t)n * n);
+    for (int kb = 0; kb < n / 32; ++kb)
+      for (int kk = 0; kk < n; kk += 2)
+        for (int lane = 0; lane < 64; ++lane)
+          mpk[i][((size_t)kb * (n / 2) + kk / 2) * 64 + lane] = mats[i][(size_t)(32 * kb + lane % 32) * n + kk + lane / 32];
+  }
+  for (int i = 0; i < 4; ++i) {
+    h->tab_off[11 + i] = total;
+    total += ((mpk[i].size() * sizeof(float) + 255) / 256) * 256;
+  }
   std::vector<char> host(total + 256);
   for (int i = 0; i < 6; ++i) {
     if (d.precision == XSH_FP64)
@@ -391,6 +407,8 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     pack<double>(host, h->tab_off[10], twh);
   else
     pack<float>(host, h->tab_off[10], twh);
+  for (int i = 0; i < 4; ++i)
+    if (h->has_tmp[i]) pack_real<float>(host, h->tab_off[11 + i], mpk[i]);
   auto cleanup = [&](int code) {
     xsh_destroy(h);
     return code;
@@ -538,6 +556,7 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
   }
   for (int i = 0; i < 4; ++i) A.tm[i] = t + h->tab_off[6 + i];
   A.twh = t + h->tab_off[10];
+  for (int i = 0; i < 4; ++i) A.tmp[i] = h->has_tmp[i] ? t + h->tab_off[11 + i] : nullptr;
   A.work = h->d_work;
   A.work_planes = h->work_planes;
   A.traj = h->d_traj;

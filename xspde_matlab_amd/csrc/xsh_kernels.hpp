@@ -309,7 +309,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   auto propagate = [&](C (&f)[PPT]) __attribute__((always_inline)) {
     if (A.has_linear) {
       if constexpr (MAT)
-        Lat::spectral_mat(L, f, A.tm, A.cplx[0], A.cplx[1], prop);
+        Lat::spectral_mat(L, f, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop);
       else
         Lat::spectral(L, f, twx, twy, prop);
     }
@@ -619,7 +619,7 @@ __global__ __launch_bounds__(NT) void probe_propagate(const PathArgs* __restrict
   const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
                        NY > 1 ? static_cast<const C*>(nci ? A.py[1] : A.py[0]) : nullptr};
   if constexpr (MAT)
-    Lat::spectral_mat(L, f, A.tm, A.cplx[0], A.cplx[1], prop);
+    Lat::spectral_mat(L, f, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop);
   else
     Lat::spectral(L, f, static_cast<const C*>(A.twx), static_cast<const C*>(A.twy), prop);
 #pragma unroll

@@ -18,6 +18,7 @@
 // (RK4.m:13-37).  Observables are reduced in FP64 and accumulated with
 // FP64 atomics into the per-path ensemble means (xdata.m:192).
 #pragma once
+#include <type_traits>
 #include "xsh_common.hpp"
 #include "../../include/xspde_hip.h"
 
@@ -51,6 +52,8 @@ struct PathArgs {
   int mat;                 // 1: dense per-dimension transforms (non-periodic lattice)
   int cplx[2];             // per dimension: 1 if its matrix is complex (periodic DFT)
   const void* tm[4];       // forward x, forward y, inverse x, inverse y (row-major N x N)
+  const void* tmp[4];      // FP32 real matrices with N % 32 == 0 in MFMA operand order
+                           // Tp[kb][kk/2][lane] = T[32 kb + lane%32][kk + lane/32], else NULL
   int dirichlet;           // any Dirichlet end (xsetbound.m:102-116)
   double* traj;            // optional [paths][E][npts][nobs] per-trajectory lattice means
   int nmodes;              // Fourier modes of the initial noise to keep
@@ -369,8 +372,68 @@ struct Lattice {
     });
   }
 
+  // FP32 real transforms of a whole 2D plane on the matrix cores: along DIM,
+  // OUT[k][line] = sum_n T[k][n] IN[n][line] is a GEMM with the lines' real and
+  // imaginary parts as columns.  v_mfma_f32_32x32x2f32 (lane l supplies
+  // A[l%32][l/32] and B[l/32][l%32] and receives D[8(r/4) + 4(l/32) + r%4][l%32]
+  // in register r): each wave owns PW pairs of 32x32 output tiles (real and
+  // imaginary columns of the same 32 lines, one 32-row block of k), reads T in
+  // the packed operand order from memory and the plane from LDS, and writes
+  // op(X, line, k) back after every wave has read its inputs.
+  template <int DIM>
+  struct MfmaPlan {
+    static constexpr int N = Line<DIM>::N, NL = Line<DIM>::NL, NW = NT / 64;
+    static constexpr int KB = N / 32, LB = NL / 32, P = KB * LB;
+    static constexpr int PW = P >= NW && P % NW == 0 ? P / NW : 0;
+    static constexpr bool ok = std::is_same<R, float>::value && NY > 1 && TPB == 1 && N % 32 == 0 &&
+                               NL % 32 == 0 && PW > 0 && LB % PW == 0;
+  };
   template <int DIM, class Op>
-  __device__ static __forceinline__ void mat_any(C* L, const void* T, bool cplx, const Op& op) {
+  __device__ static __forceinline__ void mat_mfma(C* L, const float* __restrict__ Tp, const Op& op) {
+    using Pl = MfmaPlan<DIM>;
+    constexpr int N = Pl::N, PW = Pl::PW, LB = Pl::LB;
+    typedef float v16f __attribute__((ext_vector_type(16)));
+    const int lane = threadIdx.x & 63;
+    const int p0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * PW;
+    const int kb = p0 / LB;
+    v16f acc[2 * PW];
+    static_for<0, 2 * PW>([&](auto T) __attribute__((always_inline)) {
+      acc[decltype(T)::value] = v16f{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    });
+    const float* __restrict__ Ta = Tp + (size_t)kb * (N / 2) * 64 + lane;
+#pragma unroll 4
+    for (int kk = 0; kk < N; kk += 2) {
+      const float a = Ta[(kk / 2) * 64];
+      const int n = kk + (lane >> 5);
+      static_for<0, PW>([&](auto T) __attribute__((always_inline)) {
+        constexpr int t = decltype(T)::value;
+        const int ln = ((p0 + t) % LB) * 32 + (lane & 31);
+        const C x = L[DIM == 0 ? ln * RS + n : n * RS + ln];
+        acc[2 * t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, x.x, acc[2 * t], 0, 0, 0);
+        acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, x.y, acc[2 * t + 1], 0, 0, 0);
+      });
+    }
+    __syncthreads();
+    static_for<0, PW>([&](auto T) __attribute__((always_inline)) {
+      constexpr int t = decltype(T)::value;
+      const int ln = ((p0 + t) % LB) * 32 + (lane & 31);
+      static_for<0, 16>([&](auto Rg) __attribute__((always_inline)) {
+        constexpr int r = decltype(Rg)::value;
+        const int k = kb * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+        const C X = op(C{acc[2 * t][r], acc[2 * t + 1][r]}, ln, k);
+        L[DIM == 0 ? ln * RS + k : k * RS + ln] = X;
+      });
+    });
+  }
+
+  template <int DIM, class Op>
+  __device__ static __forceinline__ void mat_any(C* L, const void* T, const void* Tp, bool cplx, const Op& op) {
+    if constexpr (MfmaPlan<DIM>::ok) {
+      if (!cplx && Tp) {
+        mat_mfma<DIM>(L, static_cast<const float*>(Tp), op);
+        return;
+      }
+    }
     if (cplx)
       mat<DIM, true>(L, static_cast<const R*>(T), op);
     else
@@ -380,24 +443,27 @@ struct Lattice {
   // a -> T^-1 op(T a) with dense per-dimension transforms (boundary types of
   // xtransform.m:208-245); op receives natural-order (kx, ky).
   template <class Op>
-  __device__ static __forceinline__ void spectral_mat(C* L, C (&f)[PPT], const void* const* Tm, int cplx_x,
-                                                      int cplx_y, const Op& op) {
+  __device__ static __forceinline__ void spectral_mat(C* L, C (&f)[PPT], const void* const* Tm,
+                                                      const void* const* Tp, int cplx_x, int cplx_y,
+                                                      const Op& op) {
     auto id = [](C X, int, int) __attribute__((always_inline)) { return X; };
     __syncthreads();
     store(L, f);
     __syncthreads();
     if constexpr (NY == 1) {
-      mat_any<0>(L, Tm[0], cplx_x, [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, k, 0); });
+      mat_any<0>(L, Tm[0], Tp[0], cplx_x,
+                 [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, k, 0); });
       __syncthreads();
-      mat_any<0>(L, Tm[2], cplx_x, id);
+      mat_any<0>(L, Tm[2], Tp[2], cplx_x, id);
     } else {
-      mat_any<0>(L, Tm[0], cplx_x, id);
+      mat_any<0>(L, Tm[0], Tp[0], cplx_x, id);
       __syncthreads();
-      mat_any<1>(L, Tm[1], cplx_y, [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, l % NX, k); });
+      mat_any<1>(L, Tm[1], Tp[1], cplx_y,
+                 [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, l % NX, k); });
       __syncthreads();
-      mat_any<1>(L, Tm[3], cplx_y, id);
+      mat_any<1>(L, Tm[3], Tp[3], cplx_y, id);
       __syncthreads();
-      mat_any<0>(L, Tm[2], cplx_x, id);
+      mat_any<0>(L, Tm[2], Tp[2], cplx_x, id);
     }
     __syncthreads();
     load(L, f);
