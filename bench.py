@@ -274,33 +274,37 @@ def bench_tw(args):
     print(json.dumps(line), flush=True)
 
 
-def bench_batch(args):
-    """64 uneven GP instances (1D 32..512, 2D 32^2..128^2, E 256..4096, 500 RK4IP
-    steps, FP32) in one graph launch, autotuned, against the same instances
-    launched one after another on their own handles."""
+def _batch_time(ps, args, serial):
     import numpy as np
-    from xspde_matlab_amd import Problem
-    from xspde_matlab_amd.configs import batch_instances
     from xspde_matlab_amd.xsim import BatchProblem
 
-    ps = batch_instances(count=64, seed=0, steps=500)
-    log("bench batch: creating + autotuning 64 instances")
-    (bp, t_tune) = (lambda t0: (BatchProblem(ps, autotune=True), time.perf_counter() - t0))(time.perf_counter())
+    t0 = time.perf_counter()
+    bp = BatchProblem(ps, autotune=True, serial=serial)
+    t_tune = time.perf_counter() - t0
     total = sum(info.traj_steps for info in bp.infos)
-    log("bench batch: %d trajectory-steps per run, tuning %.1f s, plan %s" % (total, t_tune, bp.plan()))
     for _ in range(args.warmup):
         bp.run(1)
         bp.synchronize()
-    ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         bp.run(1)
-        ms.append(bp.synchronize())
+        bp.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
     finite = all(bool(np.isfinite(bp.read_averages(i)).all()) for i in range(len(ps)))
     plan = bp.plan()
     bp.close()
-    log("bench batch: %.1f ms per batch run; now one launch per instance" % (1e3 * wall))
+    return total, wall, t_tune, finite, plan
+
+
+def _batch_run(ps, args):
+    """Autotuned graph runs of the instances ps (families concurrent, then
+    serial) against one launch per instance; the faster graph is reported."""
+    from xspde_matlab_amd import Problem
+
+    total, wall_c, t_tune, finite, plan = _batch_time(ps, args, serial=False)
+    _, wall_s, _, finite_s, _ = _batch_time(ps, args, serial=True)
+    wall = min(wall_c, wall_s)
+    finite = finite and finite_s
     probs = [Problem(p) for p in ps]
     for pr in probs:                       # warm every kernel once
         pr.handle.run_ensemble(1)
@@ -313,16 +317,33 @@ def bench_batch(args):
     seq = time.perf_counter() - t0
     for pr in probs:
         pr.close()
+    return dict(traj_steps_per_run=total, batch_seconds=wall, concurrent_seconds=wall_c, serial_seconds=wall_s,
+                sequential_seconds=seq, speedup_vs_sequential=seq / wall, tuning_seconds=t_tune, plan=plan,
+                finite=finite)
+
+
+def bench_batch(args):
+    """64 uneven GP instances (1D 32..512, 2D 32^2..128^2, E 256..4096, 500 RK4IP
+    steps, FP32) in one graph launch, autotuned, against the same instances
+    launched one after another on their own handles; plus a batch of small
+    lattices only (1D, 32^2), where one launch per instance leaves the GPU idle."""
+    from xspde_matlab_amd.configs import batch_instances
+
+    log("bench batch: 64 instances, all lattice families")
+    full = _batch_run(batch_instances(count=64, seed=0, steps=500), args)
+    log("bench batch: %.2f s per run, sequential %.2f s" % (full["batch_seconds"], full["sequential_seconds"]))
+    log("bench batch: 64 small-lattice instances")
+    small = _batch_run(batch_instances(count=64, seed=1, steps=500, sizes_2d=(32,)), args)
+    log("bench batch: small %.3f s per run, sequential %.3f s" % (small["batch_seconds"], small["sequential_seconds"]))
     line = {
         "metric": "trajectory-steps/s (batched uneven ensembles, one graph launch)",
-        "value": total / wall, "unit": "trajectory-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * wall, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "value": full["traj_steps_per_run"] / full["batch_seconds"], "unit": "trajectory-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * full["batch_seconds"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "64 GP instances, 1D {32..512} / 2D {32^2..128^2} by seed, gamma U[0.01,0.5], "
                                "g U[0.1,2], E {256..4096}, 500 RK4IP steps, FP32, autotuned",
-                   "traj_steps_per_run": total, "sequential_seconds": seq, "batch_seconds": wall,
-                   "speedup_vs_sequential": seq / wall, "tuning_seconds": t_tune, "plan": plan,
-                   "finite": finite},
+                   "full": full,
+                   "small_lattices": dict(small, rate=small["traj_steps_per_run"] / small["batch_seconds"])},
     }
     print(json.dumps(line), flush=True)
 

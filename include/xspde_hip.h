@@ -187,6 +187,8 @@ typedef struct xsh_batch xsh_batch;
 #define XSH_BATCH_AUTOTUNE 1  /* time every kernel variant (threads, trajectories
                                  per workgroup, first FFT radix) and persistent
                                  grid per family on a shortened run; keep the fastest */
+#define XSH_BATCH_SERIAL 2    /* run the families one after another inside the graph
+                                 (default: concurrently on forked streams) */
 int xsh_batch_create(int n, const xsh_desc* descs, int flags, xsh_batch** out);
 void xsh_batch_destroy(xsh_batch* b);
 /* All instances, parallel ensemble npar, asynchronously (one graph launch). */

@@ -28,13 +28,13 @@ def _single(p):
     return av
 
 
-@pytest.mark.parametrize("autotune,tol", [(False, 1e-12), (True, 2e-5)])
-def test_batch_matches_single_runs(autotune, tol):
+@pytest.mark.parametrize("autotune,serial,tol", [(False, False, 1e-12), (True, False, 2e-5), (False, True, 1e-12)])
+def test_batch_matches_single_runs(autotune, serial, tol):
     ps = _small_batch()
     sizes = sorted({tuple(p["points"][1:]) for p in ps})
     print("lattices", sizes)
     assert len(sizes) >= 3
-    bp = BatchProblem(ps, autotune=autotune)
+    bp = BatchProblem(ps, autotune=autotune, serial=serial)
     print("plan", bp.plan())
     for rep in range(2):               # replay the captured graph
         bp.run(1)

@@ -30,6 +30,7 @@ EXPORTS = ["xsh_version", "xsh_last_error", "xsh_create", "xsh_destroy", "xsh_qu
            "xsh_batch_create", "xsh_batch_destroy", "xsh_batch_run", "xsh_batch_synchronize",
            "xsh_batch_read_averages", "xsh_batch_query", "xsh_batch_plan"]
 XSH_BATCH_AUTOTUNE = 1
+XSH_BATCH_SERIAL = 2
 
 
 class Desc(C.Structure):
@@ -280,13 +281,14 @@ class MLP:
 class Batch:
     """Many instances in one graph launch (xsh_batch_*)."""
 
-    def __init__(self, descs, autotune=False):
+    def __init__(self, descs, autotune=False, serial=False):
         lib = load()
         for d in descs:
             d.struct_size = C.sizeof(Desc)
         arr = (Desc * len(descs))(*descs)
         b = C.c_void_p()
-        check(lib.xsh_batch_create(len(descs), arr, XSH_BATCH_AUTOTUNE if autotune else 0, C.byref(b)))
+        flags = (XSH_BATCH_AUTOTUNE if autotune else 0) | (XSH_BATCH_SERIAL if serial else 0)
+        check(lib.xsh_batch_create(len(descs), arr, flags, C.byref(b)))
         self._b, self.descs = b, list(descs)
         self.infos = []
         for i in range(len(descs)):

@@ -74,7 +74,8 @@ BATCH_2D = (32, 64, 128)
 BATCH_E = (256, 512, 1024, 2048, 4096)
 
 
-def batch_instances(count=64, seed=0, steps=500, precision="fp32", points=11):
+def batch_instances(count=64, seed=0, steps=500, precision="fp32", points=11, sizes_1d=BATCH_1D,
+                    sizes_2d=BATCH_2D):
     """Uneven batch: `count` independent GP instances, lattice drawn by seed from
     1D {32..512} and 2D {32^2, 64^2, 128^2}, damping gamma ~ U[0.01, 0.5],
     nonlinearity g ~ U[0.1, 2], ensemble size from {256..4096}, `steps` RK4IP
@@ -82,7 +83,7 @@ def batch_instances(count=64, seed=0, steps=500, precision="fp32", points=11):
         da = [i/2 lap a - i g |a|^2 a - gamma a] dt + sqrt(gamma) dW."""
     import numpy as np
     rng = np.random.default_rng(seed)
-    sizes = [(n,) for n in BATCH_1D] + [(n, n) for n in BATCH_2D]
+    sizes = [(n,) for n in sizes_1d] + [(n, n) for n in sizes_2d]
     per = steps // (points - 1)
     out = []
     for i in range(count):

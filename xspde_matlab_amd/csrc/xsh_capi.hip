@@ -760,6 +760,7 @@ struct xsh_batch {
   hipGraphExec_t exec;
   xsh::PathArgs* h_args;  // pinned staging, one per instance
   bool ran;
+  bool serial;            // families one after another instead of concurrently
 };
 
 static void batch_free(xsh_batch* b) {
@@ -879,12 +880,17 @@ static int batch_capture(xsh_batch* b) {
     const long long n = (long long)h->e2 * h->nchk * h->avlen * 2;
     hipLaunchKernelGGL(zero_doubles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->s0, h->d_av, n);
   }
-  e = hipEventRecord(b->fork, b->s0);
-  for (auto& g : b->g) {
-    if (e == hipSuccess) e = hipStreamWaitEvent(g.s, b->fork, 0);
-    if (e == hipSuccess) e = g.kern->launch(g.d_args, g.d_wstart, (int)g.inst.size(), g.nwork, g.grid, g.s);
-    if (e == hipSuccess) e = hipEventRecord(g.join, g.s);
-    if (e == hipSuccess) e = hipStreamWaitEvent(b->s0, g.join, 0);
+  if (b->serial) {
+    for (auto& g : b->g)
+      if (e == hipSuccess) e = g.kern->launch(g.d_args, g.d_wstart, (int)g.inst.size(), g.nwork, g.grid, b->s0);
+  } else {
+    e = hipEventRecord(b->fork, b->s0);
+    for (auto& g : b->g) {
+      if (e == hipSuccess) e = hipStreamWaitEvent(g.s, b->fork, 0);
+      if (e == hipSuccess) e = g.kern->launch(g.d_args, g.d_wstart, (int)g.inst.size(), g.nwork, g.grid, g.s);
+      if (e == hipSuccess) e = hipEventRecord(g.join, g.s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(b->s0, g.join, 0);
+    }
   }
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(b->s0, &graph);
@@ -902,6 +908,7 @@ extern "C" int xsh_batch_create(int n, const xsh_desc* descs, int flags, xsh_bat
   *out = nullptr;
   if (n < 1) return fail(XSH_EINVAL, "a batch needs at least one instance");
   auto* b = new xsh_batch();
+  b->serial = (flags & XSH_BATCH_SERIAL) != 0;
   b->h.assign(n, nullptr);
   for (int i = 0; i < n; ++i) {
     if (descs[i].dimensions < 2) {

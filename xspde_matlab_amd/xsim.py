@@ -306,9 +306,9 @@ class BatchProblem:
     """Many independent projects in one graph launch (uneven lattices, ensembles
     and drift parameters); each keeps the averages of its own Problem."""
 
-    def __init__(self, problems, autotune=False):
+    def __init__(self, problems, autotune=False, serial=False):
         self.qs = [preferences(p) for p in problems]
-        self.batch = _lib.Batch([make_desc(q) for q in self.qs], autotune=autotune)
+        self.batch = _lib.Batch([make_desc(q) for q in self.qs], autotune=autotune, serial=serial)
         self.infos = self.batch.infos
 
     def run(self, npar=1):
