@@ -58,9 +58,11 @@ enum {
   XSH_OBS_ABS2_INT = 4, /* sum |a|^2 dV                */
   XSH_OBS_ABS4_AVE = 5, /* lattice mean |a|^4          */
   XSH_OBS_CORR = 6,     /* (1/Ns) sum_x a*(x) a(x+r), per r (periodic) */
-  XSH_OBS_LOWK = 7      /* sum over the nonzero noise modes k of (|a_k|^2 - 1/2),
+  XSH_OBS_LOWK = 7,     /* sum over the nonzero noise modes k of (|a_k|^2 - 1/2),
                            a_k = sqrt(dv/Ns) sum_x a(x) e^{-ik.x}: occupation of the
                            lowest lattice modes (periodic, nmodes > 0) */
+  XSH_OBS_ABS2K = 8     /* |a_k|^2 per k-space site, a_k = fft(a) prod_d dx_d/sqrt(2 pi)
+                           (the observable |a|^2 with p.transforms set, xnft.m), FFT order */
 };
 #define XSH_MAX_OBS 8
 #define XSH_MAX_MODES 32

@@ -142,6 +142,10 @@ def make_observe(name, g=None):
             r = sum(np.abs(A[0, kx, ky]) ** 2 - 0.5 for kx, ky in modes if (kx, ky) != (0, 0))
             return np.asarray(r)[None, :]
         return f
+    if name == "abs2k":
+        # |a_k|^2 per k site: the xdata k-space path (xnft.m) applies the fft and kfact
+        # because apply_forms flags every space dimension in p.transforms
+        return lambda a, p: (np.conj(a) * a)[0:1]
     if name == "a":
         return lambda a, p: a[0:1]
     if name == "abs2":
@@ -177,5 +181,9 @@ def apply_forms(p_in):
     p["deriv"] = make_deriv(g)
     p["linear"] = make_linear(g)
     p["initial"] = make_initial(g)
-    p["observe"] = [make_observe(o, g) for o in g.get("observe", ["a"])]
+    names = g.get("observe", ["a"])
+    p["observe"] = [make_observe(o, g) for o in names]
+    if "abs2k" in names:
+        nd = p["dimensions"]
+        p["transforms"] = [[0] + [1 if o == "abs2k" else 0] * (nd - 1) for o in names]
     return p

@@ -212,3 +212,25 @@ def test_stream_kernel_persistent_grid(method):
     m = np.mean(np.abs(a) ** 2)
     print("ensemble |a|^2 %.15g, from final fields %.15g" % (av[0][0, -1, 0].real, m))
     assert abs(av[0][0, -1, 0].real - m) < 1e-12 * m
+
+
+@pytest.mark.parametrize("case", ["nls1d-fp64-256", "nls1d-fp32-64", "cgl2d-fp64-64", "cgl2d-fp32-128",
+                                  "cgl2d-fp64-128", "cgl2d-nn-fp32-128"])
+def test_kspace_observable(case):
+    """|a_k|^2 per k site (the observable |a|^2 with p.transforms on every space
+    dimension, xnft.m): accumulated inside the transform's middle operation, for
+    partial trajectory groups (20 trajectories, 16 per workgroup in 1D), the
+    streaming FP64 kernel and a Neumann lattice (the FFT of the field whatever
+    the boundaries)."""
+    kind, *rest = case.split("-")
+    if kind == "nls1d":
+        prec, n = rest[0], int(rest[1])
+        p = shrink(nls1d(prec, nx=n), E=20, e2=2, points=3, steps=2)
+    else:
+        nn = rest[0] == "nn"
+        prec, n = rest[-2], int(rest[-1])
+        p = shrink(cgl2d(prec, n=n), E=3, e2=2, points=3, steps=2)
+        if nn:
+            p["boundaries"] = [[-1, -1], [-1, -1]]
+    p["gpu"]["observe"] = ["abs2k", "abs2_ave"]
+    _path_case(p, 1e-10 if prec == "fp64" else 5e-5)

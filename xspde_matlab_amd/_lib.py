@@ -18,7 +18,7 @@ FP32, FP64 = 0, 1
 RK4, MP = 0, 1
 DERIV = {"zero": 0, "kubo": 1, "gp": 2}
 INITIAL = {"const": 0, "sech": 1, "gauss": 2, "phase": 3, "coherent": 4}
-OBSERVE = {"a": 0, "abs2": 1, "a_ave": 2, "abs2_ave": 3, "abs2_int": 4, "abs4_ave": 5, "corr": 6, "lowk": 7}
+OBSERVE = {"a": 0, "abs2": 1, "a_ave": 2, "abs2_ave": 3, "abs2_int": 4, "abs4_ave": 5, "corr": 6, "lowk": 7, "abs2k": 8}
 STATUS = {0: "XSH_OK", 1: "XSH_EINVAL", 2: "XSH_EUNSUPPORTED", 3: "XSH_ENOMEM", 4: "XSH_EDEVICE",
           5: "XSH_ESTATE"}
 

@@ -107,7 +107,7 @@ static int validate(const xsh_desc* d) {
   if (d->nobserve < 1 || d->nobserve > XSH_MAX_OBS) return fail(XSH_EINVAL, "nobserve must be 1..%d", XSH_MAX_OBS);
   for (int o = 0; o < d->nobserve; ++o) {
     if (d->observe[o] == XSH_OBS_LOWK && d->nmodes < 1) return fail(XSH_EINVAL, "observe form lowk needs noise modes");
-    if (d->observe[o] < XSH_OBS_A || d->observe[o] > XSH_OBS_LOWK) return fail(XSH_EUNSUPPORTED, "unknown observe form %d", d->observe[o]);
+    if (d->observe[o] < XSH_OBS_A || d->observe[o] > XSH_OBS_ABS2K) return fail(XSH_EUNSUPPORTED, "unknown observe form %d", d->observe[o]);
   }
   const int sdim = d->dimensions - 1;
   const int n[2] = {d->nx, d->ny};
@@ -286,7 +286,7 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   long long off = 0;
   for (int o = 0; o < d.nobserve; ++o) {
     const int kind = d.observe[o];
-    const bool site = kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR;
+    const bool site = kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR || kind == XSH_OBS_ABS2K;
     h->obs_ns[o] = site ? h->ns : 1;
     h->obs_off[o] = off;
     off += (long long)d.points * h->obs_ns[o];
@@ -439,7 +439,8 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   if (h->kern && h->kern->stream) {
     // one workspace per resident workgroup of the persistent grid
     bool corr = false;
-    for (int o = 0; o < d.nobserve; ++o) corr = corr || d.observe[o] == XSH_OBS_CORR;
+    for (int o = 0; o < d.nobserve; ++o)
+      corr = corr || d.observe[o] == XSH_OBS_CORR || d.observe[o] == XSH_OBS_ABS2K;
     h->work_planes = corr ? 3 : 2;
     const long long blocks = std::min<long long>((long long)paths * h->groups, xsh::kStreamMaxBlocks);
     const size_t wbytes = (size_t)blocks * h->work_planes * h->ns * h->esize;
@@ -564,6 +565,8 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
   A.fmap = h->d_fmap;
   A.feat = h->d_feat;
   A.feat_scale = std::sqrt(h->dv / h->ns);
+  A.kscale = 1.0;
+  for (int k = 0; k < d.dimensions - 1; ++k) A.kscale *= h->dx[k] * h->dx[k] / (2.0 * M_PI);
   return A;
 }
 

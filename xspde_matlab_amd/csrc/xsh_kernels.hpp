@@ -470,10 +470,11 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   };
 
   // one transform instance for the observable-side ops (corr, noise modes) of f1
-  auto obs_transform = [&](int mode) __attribute__((always_inline)) {
+  auto obs_transform = [&](int mode, double* acc) __attribute__((always_inline)) {
     const R fs = (R)A.feat_scale;
     ObsOp<R> op{mode, mode == 0 ? R(1.0 / ((double)NS * (double)NS)) : (mode == 2 ? fs * fs : fs), NX, A.fmap,
-                A.feat + ((long long)pidx * A.E + group * TPB) * 2 * A.nmodes};
+                A.feat + ((long long)pidx * A.E + group * TPB) * 2 * A.nmodes, acc, A.kscale * A.inv_E,
+                group * TPB, A.E};
     Lat::spectral(L, f1, twx, twy, op);
   };
 
@@ -481,17 +482,21 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   if (A.nmodes > 0) {
     const C alpha = C{(R)A.ip[0], (R)A.ip[1]};
     XSH_FOR(k, 0, PPT) f1[k] = R(2) * (f0[k] - alpha); XSH_END
-    obs_transform(1);
+    obs_transform(1, nullptr);
   }
 
   auto observe = [&](int n) __attribute__((always_inline)) {
     for (int o = 0; o < A.nobs; ++o) {
       const int kind = A.obs[o];
       double* dst = av + 2 * A.obs_off[o];
-      if (kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR) {
+      if (kind == XSH_OBS_ABS2K) {
+        // k-space |a_k|^2 (xnft.m): accumulated by the transform's middle operation
+        XSH_FOR(k, 0, PPT) f1[k] = f0[k]; XSH_END
+        obs_transform(3, dst + 2 * (long long)n * NS);
+      } else if (kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR) {
         if (kind == XSH_OBS_CORR) {
           XSH_FOR(k, 0, PPT) f1[k] = f0[k]; XSH_END
-          obs_transform(0);
+          obs_transform(0, nullptr);
         }
         XSH_FOR(i, 0, CPT)
           const int pt = Lat::point(i, 0);
@@ -514,7 +519,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
       } else {
         if (kind == XSH_OBS_LOWK) {
           XSH_FOR(k, 0, PPT) f1[k] = f0[k]; XSH_END
-          obs_transform(2);
+          obs_transform(2, nullptr);
         }
         double re = 0.0, im = 0.0;
         XSH_FOR(i, 0, CPT)

@@ -60,6 +60,7 @@ struct PathArgs {
   const int* fmap;         // [nx*ny] mode index of each FFT position, -1 if not kept
   float* feat;             // [paths][E][2*nmodes]
   double feat_scale;       // sqrt(dv / nsites): unit-variance Re/Im
+  double kscale;           // prod_d kfact_d^2, kfact = dx / sqrt(2 pi) (xpreferences.m:167)
   int dir_lo[2], dir_hi[2];// per dimension: lower / upper end is Dirichlet
   void* work;              // streaming kernels: per-workgroup field planes in device memory
   int work_planes;         // planes per workgroup (2, or 3 with the corr observable)
@@ -485,7 +486,7 @@ struct Lattice {
     if constexpr (NY == 1) {
       dif1<0>(L, twx);
       __syncthreads();
-      mid<0>(L, twx, [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, k, 0); });
+      mid<0>(L, twx, [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, k, 0, l); });
       __syncthreads();
       ditb<0>(L);
     } else {
@@ -495,7 +496,9 @@ struct Lattice {
       __syncthreads();
       dif1<1>(L, twy);
       __syncthreads();
-      mid<1>(L, twy, [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, kx_of_pos(l % NX), k); });
+      mid<1>(L, twy, [&](C X, int l, int k) __attribute__((always_inline)) {
+        return op(X, kx_of_pos(l % NX), k, l / NX);
+      });
       __syncthreads();
       ditb<1>(L);
       __syncthreads();
@@ -513,7 +516,8 @@ template <class R>
 struct PropOp {
   const cpx<R>* px;
   const cpx<R>* py;
-  __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int kx, int ky) const {
+  // (kx, ky): frequency indices; p: trajectory within the workgroup
+  __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int kx, int ky, int p = 0) const {
     cpx<R> r = X * px[kx];
     return py ? r * py[ky] : r;
   }
@@ -523,15 +527,24 @@ struct PropOp {
 //   MODES: record selected modes X(kx,ky) * scale as features, pass X through
 //   LOWK : |X|^2 * scale - 1/2 on the selected modes other than k = 0, else 0;
 //          the unnormalised inverse transform then holds their sum at x = 0
+//   ABS2K: add |X|^2 * kscale to the k-space site kx + nx ky of acc (trajectories
+//          p of the workgroup with traj0 + p < E), pass X through
 template <class R>
 struct ObsOp {
-  int mode;  // 0 CORR, 1 MODES, 2 LOWK
+  int mode;  // 0 CORR, 1 MODES, 2 LOWK, 3 ABS2K
   R scale;
   int nx;
   const int* fmap;
   float* feat;
-  __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int kx, int ky) const {
+  double* acc;
+  double kscale;
+  int traj0, E;
+  __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int kx, int ky, int p = 0) const {
     if (mode == 0) return cpx<R>{norm2(X) * scale, R(0)};
+    if (mode == 3) {
+      if (traj0 + p < E) unsafeAtomicAdd(acc + 2 * (long long)(kx + nx * ky), (double)norm2(X) * kscale);
+      return X;
+    }
     const int fi = fmap[kx + nx * ky];
     if (mode == 2) return cpx<R>{(fi >= 0 && (kx | ky)) ? norm2(X) * scale - R(0.5) : R(0), R(0)};
     if (fi >= 0) {

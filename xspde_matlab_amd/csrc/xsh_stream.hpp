@@ -267,7 +267,16 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
       for (int o = 0; o < A.nobs; ++o) {
         const int kind = A.obs[o];
         double* dst = av + 2 * A.obs_off[o];
-        if (kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR) {
+        if (kind == XSH_OBS_ABS2K) {
+#pragma unroll 1
+          for (int i = 0; i < PAIRS; ++i) {
+            const int s = S::site(i);
+            F2[s] = F0[s];
+            F2[s + 1] = F0[s + 1];
+          }
+          const ObsOp<R> op{3, R(0), NX, nullptr, nullptr, dst + 2 * (long long)n * NS, A.kscale * A.inv_E, traj, A.E};
+          S::propagate(L, F2, twx, twy, twh, op);
+        } else if (kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR) {
           const C* src = F0;
           if (kind == XSH_OBS_CORR) {
 #pragma unroll 1
@@ -276,7 +285,7 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
               F2[s] = F0[s];
               F2[s + 1] = F0[s + 1];
             }
-            const ObsOp<R> op{0, R(1.0 / ((double)NS * (double)NS)), NX, nullptr, nullptr};
+            const ObsOp<R> op{0, R(1.0 / ((double)NS * (double)NS)), NX, nullptr, nullptr, nullptr, 0.0, 0, 0};
             S::propagate(L, F2, twx, twy, twh, op);
             src = F2;
           }
