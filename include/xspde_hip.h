@@ -105,6 +105,9 @@ typedef struct {
   int traj_out;         /* 1: keep lattice-mean observables per trajectory    */
   int nmodes;           /* Fourier modes of the initial noise kept per trajectory */
   int modes[XSH_MAX_MODES][2]; /* (kx, ky) FFT indices of those modes          */
+  double boundval[2][2][2]; /* [dim][lower,upper][re,im]: constant boundary values
+                               (p.boundval, spdpreferences.m:26-43): the field at
+                               Dirichlet ends, its derivative at Neumann ends     */
 } xsh_desc;
 
 typedef struct xsh_handle xsh_handle;

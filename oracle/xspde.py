@@ -18,7 +18,11 @@ xpropfactor           Applications/SPDE/xpropfactor.m:140-175       linear-only 
                                                                     Batchtest rows
 xprop / xtransform    xprop.m:88-117, xtransform.m:196-245          transforms brute force,
                                                                     Batchtest rows
-xsetbound             xsetbound.m:92-121 (homogeneous values)       Batchtest #21 (NN)
+xsetbound             xsetbound.m:92-121 (constant boundval)        Batchtest #21 (NN),
+                                                                    heat closed forms
+xpatch, da            xpatch.m:24-62, xpropfactor.m:40-51           inhomogeneous heat
+                                                                    closed forms (DD, NN,
+                                                                    DN, ND)
 RK4                   Applications/SDE/RK4.m:7-42                   Batchtest #4, #8, #14,
                                                                     #16, #21
 MP                    Applications/SDE/MP.m:53-70                   order test (dt vs dt/2),
@@ -175,6 +179,16 @@ def xpreferences(p_in):
         shape[d] = n
         p.xs.append(np.broadcast_to(xc.reshape(shape), sh).copy())
         p.ks.append(np.broadcast_to(kcp.reshape(shape), sh).copy())
+    # constant boundary values p.boundval (spdpreferences.m:26-43, xboundfun default 0):
+    # per space dimension (lower, upper) - field values at Dirichlet ends,
+    # derivatives at Neumann (Robin) ends
+    bv = p.get("boundval", None)
+    p.bvals = []
+    for d in range(1, nd):
+        v = (0.0, 0.0)
+        if bv is not None and len(bv) > d - 1 and bv[d - 1] is not None:
+            v = (complex(bv[d - 1][0]), complex(bv[d - 1][1]))
+        p.bvals.append(v)
     names = ["x", "y", "z"]
     for i, (xg, kg) in enumerate(zip(p.xs, p.ks)):
         p[names[i]] = xg
@@ -207,28 +221,66 @@ def xpreferences(p_in):
 
 # ------------------------------------------------------------------ propagator
 def xpropfactor(p):
-    """propagator = exp(L * dtr / ipsteps) (xpropfactor.m:143-156)."""
+    """propagator = exp(L * dtr / ipsteps) (xpropfactor.m:143-156); also the
+    Neumann patch correction da_d = -(L2 - L1) dt range_d^2 / pi^2 from the first
+    two k values of each non-periodic dimension (xpropfactor.m:40-51), into p.da."""
+    p.da = [0.0] * len(p.btypes)
     if p.linear is None or (not callable(p.linear) and p.linear == 0):
         return None
-    L = p.linear(p)
-    return np.exp(np.asarray(L) * (p.dtr / p.ipsteps))
+    L = np.asarray(p.linear(p))
+    dt = p.dtr / p.ipsteps
+    if p.setboundaries:
+        Lb = np.broadcast_to(L, (1,) + p.space + (1,))
+        for d in range(1, len(p.btypes) + 1):
+            i1 = [0] * Lb.ndim
+            i2 = list(i1)
+            i2[d] = 1
+            p.da[d - 1] = -(Lb[tuple(i2)] - Lb[tuple(i1)]) * dt / np.pi ** 2 * p.ranges[d] ** 2
+    return np.exp(L * dt)
+
+
+def xpatch(a, p):
+    """Boundary patch (xpatch.m:24-62): returns (a - pa, pa + lp) where pa is the
+    per-dimension polynomial matching the boundary values and lp the Neumann
+    (type -3) correction db * da_d."""
+    pa = np.zeros(a.shape, dtype=complex)
+    lp = 0.0
+    for d, (t, _) in enumerate(p.btypes, start=1):
+        if t == 0 or a.shape[d] <= 1:
+            continue
+        b1, b2 = p.bvals[d - 1]
+        rng = p.ranges[d]
+        x = p.xs[d - 1] - p.origins[d]
+        if t == -3:
+            db = (b2 - b1) / rng
+            pa = pa + b1 * x + 0.5 * x ** 2 * db
+            lp = lp + db * p.da[d - 1]
+        elif t == -1:
+            pa = pa + b1 + x * b2
+        elif t == 1:
+            pa = pa + b2 + (x - rng) * b1
+        elif t == 3:
+            pa = pa + b1 + x * (b2 - b1) / rng
+    return a - pa, pa + lp
 
 
 def xsetbound(a, p):
-    """Homogeneous boundary values: Dirichlet ends set to the default boundval 0
-    (xsetbound.m:100-116; xboundfun default zeros, spdpreferences.m:84-93)."""
+    """Dirichlet ends set to their boundary values, dimension by dimension (later
+    dimensions win at corners) (xsetbound.m:92-121; p.boundval constants,
+    default 0, spdpreferences.m:26-43,75-93)."""
     if not p.setboundaries:
         return a
-    a = a.copy()
+    a = np.array(a, dtype=complex)
     for d, (t, b) in enumerate(p.btypes, start=1):
+        lo, hi = p.bvals[d - 1]
         if b[0] == 1:
             idx = [slice(None)] * a.ndim
             idx[d] = 0
-            a[tuple(idx)] = 0
+            a[tuple(idx)] = lo
         if b[1] == 1:
             idx = [slice(None)] * a.ndim
             idx[d] = -1
-            a[tuple(idx)] = 0
+            a[tuple(idx)] = hi
     return a
 
 
@@ -241,11 +293,16 @@ def xtransform(a, p, inverse):
 
 
 def xprop(a, p, prop):
-    """One interaction-picture propagation (xprop.m:88-117), homogeneous patches."""
+    """One interaction-picture propagation (xprop.m:17-47): subtract the boundary
+    patch, transform, propagate, inverse transform, add the patch back."""
     if prop is not None:
-        a = xtransform(np.asarray(a, complex), p, 0)
+        a = np.asarray(a, complex)
+        pat = 0.0
+        if p.setboundaries:
+            a, pat = xpatch(a, p)
+        a = xtransform(a, p, 0)
         a = prop * a
-        a = xtransform(a, p, 1)
+        a = xtransform(a, p, 1) + pat
     if p.setboundaries:
         a = xsetbound(a, p)
     return a

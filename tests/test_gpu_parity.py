@@ -234,3 +234,29 @@ def test_kspace_observable(case):
             p["boundaries"] = [[-1, -1], [-1, -1]]
     p["gpu"]["observe"] = ["abs2k", "abs2_ave"]
     _path_case(p, 1e-10 if prec == "fp64" else 5e-5)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 5e-5)])
+@pytest.mark.parametrize("bnd", ["NN", "DD", "DN", "ND"])
+def test_nls1d_boundary_values(prec, tol, bnd):
+    """Constant non-zero boundary values (p.boundval): the boundary patch around
+    every propagation, the Neumann correction and the Dirichlet values, against
+    the oracle (whose patches reproduce the inhomogeneous heat-equation closed
+    forms)."""
+    code = {"NN": [-1, -1], "DD": [1, 1], "DN": [1, -1], "ND": [-1, 1]}[bnd]
+    p = shrink(nls1d(prec, nx=128), E=20, e2=2, points=3, steps=2)
+    p["boundaries"] = [code]
+    p["boundval"] = [[0.3 + 0.1j, -0.2 + 0.05j]]
+    p["gpu"]["observe"] = ["abs2", "a"]
+    _path_case(p, tol)
+
+
+@pytest.mark.parametrize("prec,tol,n", [("fp64", 1e-10, 64), ("fp32", 5e-5, 128)])
+def test_cgl2d_boundary_values(prec, tol, n):
+    """2D: Neumann in x with derivative values, Dirichlet in y with field values."""
+    p = shrink(cgl2d(prec, n=n), E=2, e2=2, points=3, steps=2)
+    p["boundaries"] = [[-1, -1], [1, 1]]
+    p["boundval"] = [[0.02, -0.01 + 0.005j], [0.05 - 0.02j, 0.03]]
+    # the lattice mean of a is a near-cancelling sum (random phases): FP64 only
+    p["gpu"]["observe"] = ["abs2_ave", "abs2"] + (["a_ave"] if prec == "fp64" else [])
+    _path_case(p, tol)

@@ -84,3 +84,33 @@ def test_heat_equation_exact_per_boundary(name, method):
              compare=[lambda p: exact(p.x, p.t)])
     err = X.xsim(p)[0]
     assert err[0] < 1e-12
+
+
+# inhomogeneous constant boundary values (p.boundval): steady part + decaying mode
+#   DD  u(0) = A, u(pi) = B:        A + (B - A) x / pi + sin(x) e^-t
+#   NN  u'(0) = A, u'(pi) = B:      A x + (B - A) x^2 / (2 pi) + (B - A) t / pi + cos(x) e^-t
+#   DN  u(0) = A, u'(pi) = B:       A + B x + sin(x/2) e^(-t/4)
+#   ND  u'(0) = A, u(pi) = B:       B + A (x - pi) + cos(x/2) e^(-t/4)
+HEAT_BV = {
+    "DD": ([1, 1], lambda x, t, A, B: A + (B - A) * x / np.pi + np.sin(x) * np.exp(-t)),
+    "NN": ([-1, -1], lambda x, t, A, B: A * x + (B - A) * x ** 2 / (2 * np.pi) + (B - A) * t / np.pi
+           + np.cos(x) * np.exp(-t)),
+    "DN": ([1, -1], lambda x, t, A, B: A + B * x + np.sin(x / 2) * np.exp(-t / 4)),
+    "ND": ([-1, 1], lambda x, t, A, B: B + A * (x - np.pi) + np.cos(x / 2) * np.exp(-t / 4)),
+}
+
+
+@pytest.mark.parametrize("name", list(HEAT_BV))
+@pytest.mark.parametrize("method", ["MP", "RK4"])
+def test_heat_equation_inhomogeneous_boundaries(name, method):
+    """Boundary patches (xpatch.m), the Neumann correction da (xpropfactor.m:40-51)
+    and Dirichlet values (xsetbound.m) against closed forms with non-zero
+    boundary values: the linear interaction picture stays exact."""
+    b, exact = HEAT_BV[name]
+    A, B = 0.7, -0.4
+    p = dict(dimensions=2, points=[11, 51], ranges=[4, np.pi], origins=[0, 0], method=method,
+             boundaries=[b], boundval=[[A, B]], initial=lambda v, p: exact(p.x, 0.0, A, B),
+             linear=lambda p: p.Dx ** 2, compare=[lambda p: exact(p.x, p.t, A, B)])
+    err = X.xsim(p)[0]
+    print(name, method, err[0])
+    assert err[0] < 1e-10

@@ -46,6 +46,7 @@ class Desc(C.Structure):
         ("has_linear", C.c_int), ("initial", C.c_int), ("init_par", C.c_double * 6),
         ("nobserve", C.c_int), ("observe", C.c_int * XSH_MAX_OBS), ("seq", C.c_int), ("inst", C.c_int),
         ("traj_out", C.c_int), ("nmodes", C.c_int), ("modes", (C.c_int * 2) * XSH_MAX_MODES),
+        ("boundval", ((C.c_double * 2) * 2) * 2),
     ]
 
 

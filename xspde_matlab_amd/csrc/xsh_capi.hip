@@ -55,8 +55,10 @@ struct xsh_handle {
   size_t esize;
   void* d_tables;   // px[2], py[2], twx, twy, transform matrices, half-length twiddles,
                     // matrix-core operand copies of the real matrices
-  size_t tab_off[15];
+  size_t tab_off[16];
   bool has_tmp[4];
+  bool has_patch;      // non-zero boundary values on a non-periodic dimension
+  double lp[2][2];     // [nc] Neumann patch correction (complex)
   void* d_work;      // streaming kernels: field planes per workgroup
   int work_planes;
   int btype[2];      // boundary type per space dimension (0 periodic)
@@ -390,6 +392,52 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     h->tab_off[11 + i] = total;
     total += ((mpk[i].size() * sizeof(float) + 255) / 256) * 256;
   }
+  // Boundary patch for constant boundary values (xpatch.m:24-62): pa(x) summed
+  // over the non-periodic dimensions, and the Neumann correction lp = db * da with
+  // da = -(L(k1) - L(k0)) dt range^2 / pi^2 from the first two k values of the
+  // dimension (xpropfactor.m:40-51), per check (dt = dtr/2).
+  std::vector<cd> patch;
+  h->has_patch = false;
+  for (int k = 0; k < d.dimensions - 1; ++k)
+    for (int side = 0; side < 2; ++side)
+      if (h->btype[k] != 0 && (d.boundval[k][side][0] != 0.0 || d.boundval[k][side][1] != 0.0)) h->has_patch = true;
+  if (h->has_patch) {
+    patch.assign(h->ns, cd(0.0, 0.0));
+    for (int nci = 0; nci < 2; ++nci) h->lp[nci][0] = h->lp[nci][1] = 0.0;
+    for (int k = 0; k < d.dimensions - 1; ++k) {
+      const int t = h->btype[k], n = npt[k];
+      if (t == 0) continue;
+      const cd b1(d.boundval[k][0][0], d.boundval[k][0][1]), b2(d.boundval[k][1][0], d.boundval[k][1][1]);
+      const double rng = d.ranges[k + 1];
+      for (int site = 0; site < h->ns; ++site) {
+        const int m = k == 0 ? site % h->nx : site / h->nx;
+        const double x = m * h->dx[k];
+        cd pa(0.0, 0.0);
+        switch (t) {
+          case -3: pa = b1 * x + 0.5 * x * x * (b2 - b1) / rng; break;
+          case -1: pa = b1 + x * b2; break;
+          case 1: pa = b2 + (x - rng) * b1; break;
+          case 3: pa = b1 + x * (b2 - b1) / rng; break;
+        }
+        patch[site] += pa;
+      }
+      if (t == -3) {
+        const double dkp = M_PI / ((n - 1) * h->dx[k]);
+        const double k0 = 0.0, k1 = dkp;  // NN k-grid: m * dkp (spdpreferences.m:56-57)
+        const cd l1(d.lin_l1[k][0], d.lin_l1[k][1]), l2(d.lin_l2[k][0], d.lin_l2[k][1]);
+        auto f = [&](double kk) { return l1 * cd(0, kk) + l2 * cd(-kk * kk, 0); };
+        const cd db = (b2 - b1) / rng;
+        for (int nci = 0; nci < 2; ++nci) {
+          const double dtp = h->dtr[nci] / 2.0;
+          const cd da = d.has_linear ? -(f(k1) - f(k0)) * dtp * rng * rng / (M_PI * M_PI) : cd(0, 0);
+          h->lp[nci][0] += (db * da).real();
+          h->lp[nci][1] += (db * da).imag();
+        }
+      }
+    }
+  }
+  h->tab_off[15] = total;
+  total += ((patch.size() * h->esize + 255) / 256) * 256;
   std::vector<char> host(total + 256);
   for (int i = 0; i < 6; ++i) {
     if (d.precision == XSH_FP64)
@@ -409,6 +457,12 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     pack<float>(host, h->tab_off[10], twh);
   for (int i = 0; i < 4; ++i)
     if (h->has_tmp[i]) pack_real<float>(host, h->tab_off[11 + i], mpk[i]);
+  if (h->has_patch) {
+    if (d.precision == XSH_FP64)
+      pack<double>(host, h->tab_off[15], patch);
+    else
+      pack<float>(host, h->tab_off[15], patch);
+  }
   auto cleanup = [&](int code) {
     xsh_destroy(h);
     return code;
@@ -554,9 +608,15 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
     A.dir_lo[k] = k < d.dimensions - 1 && d.boundaries[k][0] == 1;
     A.dir_hi[k] = k < d.dimensions - 1 && d.boundaries[k][1] == 1;
     if (A.dir_lo[k] || A.dir_hi[k]) A.dirichlet = 1;
+    for (int side = 0; side < 2; ++side)
+      for (int c = 0; c < 2; ++c) A.dval[k][side][c] = k < d.dimensions - 1 ? d.boundval[k][side][c] : 0.0;
   }
   for (int i = 0; i < 4; ++i) A.tm[i] = t + h->tab_off[6 + i];
   A.twh = t + h->tab_off[10];
+  A.has_patch = h->has_patch;
+  A.patch = h->has_patch ? t + h->tab_off[15] : nullptr;
+  for (int nci = 0; nci < 2; ++nci)
+    for (int c = 0; c < 2; ++c) A.lp[nci][c] = h->has_patch ? h->lp[nci][c] : 0.0;
   for (int i = 0; i < 4; ++i) A.tmp[i] = h->has_tmp[i] ? t + h->tab_off[11 + i] : nullptr;
   A.work = h->d_work;
   A.work_planes = h->work_planes;

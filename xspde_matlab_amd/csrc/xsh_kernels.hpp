@@ -295,23 +295,50 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     return (ix == 0 && A.dir_lo[0]) || (ix == NX - 1 && A.dir_hi[0]) ||
            (NY > 1 && ((iy == 0 && A.dir_lo[1]) || (iy == NY - 1 && A.dir_hi[1])));
   };
+  // value of a Dirichlet site: x ends, then y ends (xsetbound.m sets dimension
+  // by dimension, so y values win at corners)
+  auto dir_value = [&](int i, int v) __attribute__((always_inline)) -> C {
+    const int site = Lat::point(i, v) % NS, iy = site / NX, ix = site % NX;
+    int dd = -1, sd = 0;
+    if (ix == 0 && A.dir_lo[0]) dd = 0, sd = 0;
+    if (ix == NX - 1 && A.dir_hi[0]) dd = 0, sd = 1;
+    if (NY > 1 && iy == 0 && A.dir_lo[1]) dd = 1, sd = 0;
+    if (NY > 1 && iy == NY - 1 && A.dir_hi[1]) dd = 1, sd = 1;
+    return dd < 0 ? C{R(0), R(0)} : C{(R)A.dval[dd][sd][0], (R)A.dval[dd][sd][1]};
+  };
   // (only dense-transform lattices can have Dirichlet ends)
   auto setbound = [&](C (&f)[PPT]) __attribute__((always_inline)) {
     if constexpr (!MAT) return;
     if (!A.dirichlet) return;
     XSH_FOR(i, 0, CPT)
       XSH_FOR(v, 0, 2)
-        if (dir_site(i, v)) f[2 * i + v] = C{R(0), R(0)};
+        if (dir_site(i, v)) f[2 * i + v] = dir_value(i, v);
+      XSH_END
+    XSH_END
+  };
+  // constant boundary values: a -> a - pa before the transforms, + pa + lp after
+  // (xprop.m:17-47 with xpatch.m:24-62)
+  auto patch = [&](C (&f)[PPT], R sign, bool with_lp) __attribute__((always_inline)) {
+    if constexpr (!MAT) return;
+    if (!A.has_patch) return;
+    const C* pa = static_cast<const C*>(A.patch);
+    const C lp = with_lp ? C{(R)A.lp[nci][0], (R)A.lp[nci][1]} : C{R(0), R(0)};
+    XSH_FOR(i, 0, CPT)
+      XSH_FOR(v, 0, 2)
+        f[2 * i + v] = f[2 * i + v] + sign * pa[Lat::point(i, v) % NS] + lp;
       XSH_END
     XSH_END
   };
   // xprop.m:88-117: transform, propagator, inverse transform, then setbound
   auto propagate = [&](C (&f)[PPT]) __attribute__((always_inline)) {
     if (A.has_linear) {
-      if constexpr (MAT)
+      if constexpr (MAT) {
+        patch(f, R(-1), false);
         Lat::spectral_mat(L, f, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop);
-      else
+        patch(f, R(1), true);
+      } else {
         Lat::spectral(L, f, twx, twy, prop);
+      }
     }
     setbound(f);
   };
@@ -405,9 +432,9 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
           const C A0 = av, A1 = bv;
           const bool dz = MAT && A.dirichlet && dir_site(i, v);
           C a2 = A0 + hdt * D(A1, w);
-          if (dz) a2 = C{R(0), R(0)};
+          if (dz) a2 = dir_value(i, v);
           C a3 = A0 + (R(2) * hdt) * D(a2, w);
-          if (dz) a3 = C{R(0), R(0)};
+          if (dz) a3 = dir_value(i, v);
           bv = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
           av = a3;
         };
@@ -426,7 +453,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
           C a = a0;
           for (int it = 0; it < A.iterations; ++it) {
             a = a0 + hdt * D(a, w);
-            if (dz) a = C{R(0), R(0)};
+            if (dz) a = dir_value(i, v);
           }
           xv = R(2) * a - a0;
         };

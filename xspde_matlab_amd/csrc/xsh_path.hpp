@@ -62,6 +62,10 @@ struct PathArgs {
   double feat_scale;       // sqrt(dv / nsites): unit-variance Re/Im
   double kscale;           // prod_d kfact_d^2, kfact = dx / sqrt(2 pi) (xpreferences.m:167)
   int dir_lo[2], dir_hi[2];// per dimension: lower / upper end is Dirichlet
+  double dval[2][2][2];    // [dim][lower,upper][re,im] Dirichlet boundary values
+  int has_patch;           // non-zero boundary values: patch around every propagation
+  const void* patch;       // [nx*ny] complex R: boundary patch pa (xpatch.m:24-62)
+  double lp[2][2];         // [nc][re,im]: Neumann correction db * da added after propagation
   void* work;              // streaming kernels: per-workgroup field planes in device memory
   int work_planes;         // planes per workgroup (2, or 3 with the corr observable)
   const void* twh;         // streaming kernels: W_{ny/2}^m, m < ny/2

@@ -158,6 +158,13 @@ def make_desc(q):
         if name not in _lib.OBSERVE:
             raise GpuFormError("unknown observe form %r" % (name,))
         d.observe[o] = _lib.OBSERVE[name]
+    bv = q.get("boundval") or []
+    for k in range(min(len(bv), 2)):
+        if bv[k] is None:
+            continue
+        for side in range(2):
+            v = complex(bv[k][side])
+            d.boundval[k][side][0], d.boundval[k][side][1] = v.real, v.imag
     d.seq = int(g.get("seq", 0))
     d.inst = int(g.get("inst", 0))
     d.traj_out = 1 if g.get("traj_out") else 0
