@@ -108,6 +108,8 @@ typedef struct {
   double boundval[2][2][2]; /* [dim][lower,upper][re,im]: constant boundary values
                                (p.boundval, spdpreferences.m:26-43): the field at
                                Dirichlet ends, its derivative at Neumann ends     */
+  int raw;              /* 1: keep every trajectory's field at every stored point
+                           (p.raw, xpath.m:73-78), read with xsh_read_raw      */
 } xsh_desc;
 
 typedef struct xsh_handle xsh_handle;
@@ -170,6 +172,11 @@ int xsh_read_averages(xsh_handle* h, double* av_out);
  *   feat [ensembles(2)][nchk][ensembles(1)][2*nmodes] Re, Im of the chosen
  *        Fourier modes of the initial noise, scaled to unit variance (NULL to skip) */
 int xsh_read_trajectories(xsh_handle* h, double* obs, float* feat);
+
+/* Raw trajectories of the last xsh_run_ensemble (p.raw, xpath.m:73-78, stored by
+ * the reference in raw{j}{nc,np}): [ensembles(2)][nchk][points][ensembles(1)][nx*ny]
+ * complex (interleaved doubles, x fastest); the descriptor must set raw = 1. */
+int xsh_read_raw(xsh_handle* h, double* out);
 
 /* Test hooks (no reference counterpart; used by the parity tests). */
 /* Standard normals of one draw: out[t][L], L < count, for trajectories

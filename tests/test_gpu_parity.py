@@ -260,3 +260,44 @@ def test_cgl2d_boundary_values(prec, tol, n):
     # the lattice mean of a is a near-cancelling sum (random phases): FP64 only
     p["gpu"]["observe"] = ["abs2_ave", "abs2"] + (["a_ave"] if prec == "fp64" else [])
     _path_case(p, tol)
+
+
+def _oracle_snapshots(q, j, nc):
+    """The oracle path's fields at every stored point (xpath.m:73-78)."""
+    snaps = []
+    fs = list(q.observe)
+    q2 = q.copy()
+    q2.observe = [lambda a, p, _f=fs[0]: (snaps.append(np.array(a)), _f(a, p))[1]] + fs[1:]
+    X.run_path(q2, j, nc)
+    return snaps
+
+
+@pytest.mark.parametrize("case,tol", [("cgl2d-fp64-32", 1e-10), ("nls1d-fp32-64", 5e-5), ("cgl2d-fp64-128", 1e-10)])
+def test_raw_fields(case, tol):
+    """p.raw: every trajectory's field at every stored point, for every (j, nc)
+    path of the ensemble, against the oracle's fields at its observation points
+    (register kernels, a partial 1D trajectory group, the streaming kernel)."""
+    kind, prec, n = case.split("-")
+    n = int(n)
+    if kind == "nls1d":
+        p = shrink(nls1d(prec, nx=n), E=20, e2=2, points=3, steps=2)
+    else:
+        p = shrink(cgl2d(prec, n=n), E=3, e2=2, points=3, steps=2)
+    p["raw"] = True
+    prob = Problem(p)
+    prob.handle.run_ensemble(1)
+    prob.handle.synchronize()
+    raw = prob.handle.read_raw()                       # [e2, nchk, points, E, ns]
+    prob.close()
+    q = X.xpreferences(oracle_problem(p))
+    for j in (1, 2):
+        for nc in (1, 2):
+            snaps = _oracle_snapshots(q, j, nc)
+            assert len(snaps) == p["points"][0]
+            for t, a in enumerate(snaps):
+                ref = np.moveaxis(a[0], -1, 0)
+                ref = ref.reshape(ref.shape[0], -1, order="F")
+                got = raw[j - 1, nc - 1, t]
+                err = np.abs(got - ref).max() / np.abs(ref).max()
+                assert err < tol, (j, nc, t, err)
+    print("%s raw fields: %d paths x %d points match" % (case, 4, p["points"][0]))

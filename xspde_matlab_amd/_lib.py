@@ -23,7 +23,7 @@ STATUS = {0: "XSH_OK", 1: "XSH_EINVAL", 2: "XSH_EUNSUPPORTED", 3: "XSH_ENOMEM", 
           5: "XSH_ESTATE"}
 
 EXPORTS = ["xsh_version", "xsh_last_error", "xsh_create", "xsh_destroy", "xsh_query", "xsh_run_path",
-           "xsh_run_ensemble", "xsh_synchronize", "xsh_read_averages", "xsh_read_trajectories",
+           "xsh_run_ensemble", "xsh_synchronize", "xsh_read_averages", "xsh_read_trajectories", "xsh_read_raw",
            "xsh_probe_normals", "xsh_probe_propagate", "xsh_mlp_create", "xsh_mlp_destroy",
            "xsh_mlp_last_error", "xsh_mlp_get_params", "xsh_mlp_forward", "xsh_mlp_loss_grad",
            "xsh_mlp_train", "xsh_mlp_set_translations", "xsh_mlp_gaussian_moments",
@@ -46,7 +46,7 @@ class Desc(C.Structure):
         ("has_linear", C.c_int), ("initial", C.c_int), ("init_par", C.c_double * 6),
         ("nobserve", C.c_int), ("observe", C.c_int * XSH_MAX_OBS), ("seq", C.c_int), ("inst", C.c_int),
         ("traj_out", C.c_int), ("nmodes", C.c_int), ("modes", (C.c_int * 2) * XSH_MAX_MODES),
-        ("boundval", ((C.c_double * 2) * 2) * 2),
+        ("boundval", ((C.c_double * 2) * 2) * 2), ("raw", C.c_int),
     ]
 
 
@@ -91,6 +91,7 @@ def load():
     lib.xsh_probe_propagate.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
     fp = C.POINTER(C.c_float)
     lib.xsh_read_trajectories.argtypes = [C.c_void_p, dp, fp]
+    lib.xsh_read_raw.argtypes = [C.c_void_p, dp]
     lib.xsh_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, fp, C.POINTER(C.c_void_p)]
     lib.xsh_mlp_destroy.argtypes = [C.c_void_p]
     lib.xsh_mlp_destroy.restype = None
@@ -191,6 +192,14 @@ class Handle:
         if feat is not None:
             feat = feat.reshape(e2, nchk, E, 2 * d.nmodes)
         return obs, feat
+
+    def read_raw(self):
+        """Raw fields [e2, nchk, points, E, nx*ny] complex of the last run_ensemble."""
+        d = self.desc
+        e2, nchk, E = d.ensembles[1], self.info.nchk, d.ensembles[0]
+        out = np.zeros(2 * e2 * nchk * d.points * E * self.info.nsites)
+        check(load().xsh_read_raw(self._h, _ptr(out)))
+        return (out[0::2] + 1j * out[1::2]).reshape(e2, nchk, d.points, E, self.info.nsites)
 
     def probe_normals(self, nsp, tag, draw, traj0, ntraj, count):
         out = np.zeros(ntraj * count)

@@ -68,6 +68,7 @@ struct xsh_handle {
   double* d_path_av; // [avlen] complex (single path)
   double* d_a;       // [E][ns] complex (single path in/out)
   double* d_traj;    // [e2*nchk][E][points][nobs] per-trajectory lattice means
+  double* d_raw;     // [e2*nchk][points][E][ns] complex raw fields
   float* d_feat;     // [e2*nchk][E][2*nmodes] noise modes
   int* d_fmap;       // [nx*ny] mode index per FFT position
   hipStream_t stream;
@@ -129,6 +130,8 @@ static int validate(const xsh_desc* d) {
   }
   if (!(d->ranges[0] > 0) && d->points > 1) return fail(XSH_EINVAL, "ranges(1) must be > 0");
   if (d->traj_out != 0 && d->traj_out != 1) return fail(XSH_EINVAL, "traj_out must be 0 or 1");
+  if (d->raw != 0 && d->raw != 1) return fail(XSH_EINVAL, "raw must be 0 or 1");
+  if (d->raw && d->dimensions < 2) return fail(XSH_EUNSUPPORTED, "raw fields are kept for lattice problems");
   if (d->nmodes < 0 || d->nmodes > XSH_MAX_MODES) return fail(XSH_EINVAL, "nmodes must be 0..%d", XSH_MAX_MODES);
   if (d->nmodes > 0) {
     if (d->initial != XSH_INIT_COHERENT) return fail(XSH_EUNSUPPORTED, "noise modes need the coherent initial form");
@@ -481,6 +484,11 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   if (d.traj_out &&
       hipMalloc(&h->d_traj, paths * h->E * d.points * d.nobserve * sizeof(double)) != hipSuccess)
     return cleanup(fail(XSH_ENOMEM, "hipMalloc per-trajectory observables"));
+  if (d.raw) {
+    const size_t rbytes = paths * d.points * (size_t)h->E * h->ns * 16;
+    if (hipMalloc(&h->d_raw, rbytes) != hipSuccess)
+      return cleanup(fail(XSH_ENOMEM, "hipMalloc raw fields (%zu bytes)", rbytes));
+  }
   if (d.nmodes) {
     if (hipMalloc(&h->d_feat, paths * h->E * 2 * d.nmodes * sizeof(float)) != hipSuccess ||
         hipMalloc(&h->d_fmap, (size_t)h->ns * sizeof(int)) != hipSuccess)
@@ -520,6 +528,7 @@ extern "C" void xsh_destroy(xsh_handle* h) {
   if (h->d_path_av) (void)hipFree(h->d_path_av);
   if (h->d_a) (void)hipFree(h->d_a);
   if (h->d_traj) (void)hipFree(h->d_traj);
+  if (h->d_raw) (void)hipFree(h->d_raw);
   if (h->d_feat) (void)hipFree(h->d_feat);
   if (h->d_fmap) (void)hipFree(h->d_fmap);
   if (h->d_work) (void)hipFree(h->d_work);
@@ -621,6 +630,7 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
   A.work = h->d_work;
   A.work_planes = h->work_planes;
   A.traj = h->d_traj;
+  A.raw = h->d_raw;
   A.nmodes = d.nmodes;
   A.fmap = h->d_fmap;
   A.feat = h->d_feat;
@@ -667,6 +677,7 @@ extern "C" int xsh_run_path(xsh_handle* h, int npar, int j, int nc, const double
   A.nc_only = nc;
   A.av = h->d_path_av;
   A.traj = nullptr;  // per-trajectory outputs belong to xsh_run_ensemble
+  A.raw = nullptr;
   A.nmodes = 0;
   const size_t fbytes = (size_t)h->E * h->ns * 16;
   HIPCHK(hipMemsetAsync(h->d_path_av, 0, (size_t)h->avlen * 16, h->stream));
@@ -713,6 +724,16 @@ extern "C" int xsh_read_averages(xsh_handle* h, double* av_out) {
   if (!h || !av_out) return fail(XSH_EINVAL, "NULL argument");
   if (!h->ran) return fail(XSH_ESTATE, "xsh_run_ensemble has not run");
   HIPCHK(hipMemcpyAsync(av_out, h->d_av, (size_t)h->e2 * h->nchk * h->avlen * 16, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return XSH_OK;
+}
+
+extern "C" int xsh_read_raw(xsh_handle* h, double* out) {
+  if (!h || !out) return fail(XSH_EINVAL, "NULL argument");
+  if (!h->ran) return fail(XSH_ESTATE, "xsh_run_ensemble has not run");
+  if (!h->d_raw) return fail(XSH_ESTATE, "descriptor did not ask for raw fields");
+  const size_t rbytes = (size_t)h->e2 * h->nchk * h->d.points * (size_t)h->E * h->ns * 16;
+  HIPCHK(hipMemcpyAsync(out, h->d_raw, rbytes, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   return XSH_OK;
 }

@@ -513,6 +513,20 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   }
 
   auto observe = [&](int n) __attribute__((always_inline)) {
+    if (A.raw) {  // xpath.m:73-78: the fields themselves at this point
+      XSH_FOR(i, 0, CPT)
+        const int pt = Lat::point(i, 0);
+        const int p = pt / NS, s = pt % NS;
+        const int traj = group * TPB + p;
+        if (traj < A.E) {
+          XSH_FOR(v, 0, 2)
+            const long long g = (((long long)pidx * A.npts + n) * A.E + traj) * NS + s + v;
+            A.raw[2 * g] = (double)f0[2 * i + v].x;
+            A.raw[2 * g + 1] = (double)f0[2 * i + v].y;
+          XSH_END
+        }
+      XSH_END
+    }
     for (int o = 0; o < A.nobs; ++o) {
       const int kind = A.obs[o];
       double* dst = av + 2 * A.obs_off[o];

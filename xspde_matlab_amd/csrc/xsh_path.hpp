@@ -56,6 +56,7 @@ struct PathArgs {
                            // Tp[kb][kk/2][lane] = T[32 kb + lane%32][kk + lane/32], else NULL
   int dirichlet;           // any Dirichlet end (xsetbound.m:102-116)
   double* traj;            // optional [paths][E][npts][nobs] per-trajectory lattice means
+  double* raw;             // optional [paths][npts][E][ns] complex double fields
   int nmodes;              // Fourier modes of the initial noise to keep
   const int* fmap;         // [nx*ny] mode index of each FFT position, -1 if not kept
   float* feat;             // [paths][E][2*nmodes]

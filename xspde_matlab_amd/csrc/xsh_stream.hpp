@@ -264,6 +264,18 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
 
     auto observe = [&](int n) __attribute__((always_inline)) {
       __syncthreads();
+      if (A.raw) {  // xpath.m:73-78
+#pragma unroll 1
+        for (int i = 0; i < PAIRS; ++i) {
+          const int s = S::site(i);
+#pragma unroll
+          for (int v = 0; v < 2; ++v) {
+            const long long g = (((long long)pidx * A.npts + n) * A.E + traj) * NS + s + v;
+            A.raw[2 * g] = (double)F0[s + v].x;
+            A.raw[2 * g + 1] = (double)F0[s + v].y;
+          }
+        }
+      }
       for (int o = 0; o < A.nobs; ++o) {
         const int kind = A.obs[o];
         double* dst = av + 2 * A.obs_off[o];

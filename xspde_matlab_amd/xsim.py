@@ -168,6 +168,7 @@ def make_desc(q):
     d.seq = int(g.get("seq", 0))
     d.inst = int(g.get("inst", 0))
     d.traj_out = 1 if g.get("traj_out") else 0
+    d.raw = 1 if q.get("raw") else 0
     modes = g.get("modes")
     if modes:
         if isinstance(modes, int):
