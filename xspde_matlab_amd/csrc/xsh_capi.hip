@@ -376,13 +376,13 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   }
   h->tab_off[10] = total;
   total += ((twh.size() * h->esize + 255) / 256) * 256;
-  // FP32 real matrices of 2D lattices with N % 32 == 0: the same entries in the
+  // FP32 real matrices with N % 32 == 0: the same entries in the
   // operand order of the 32x32x2 matrix-core instruction, Tp[kb][kk/2][lane] =
   // T[32 kb + lane%32][kk + lane/32] (used by kernels that run those passes on MFMA)
   std::vector<std::vector<double>> mpk(4);
   for (int i = 0; i < 4; ++i) {
     const int k = i % 2, n = npt[k];
-    h->has_tmp[i] = h->mat && d.dimensions == 3 && d.precision == XSH_FP32 && !cplx[k] && n % 32 == 0 &&
+    h->has_tmp[i] = h->mat && k < d.dimensions - 1 && d.precision == XSH_FP32 && !cplx[k] && n % 32 == 0 &&
                     mats[i].size() == (size_t)n * n;
     if (!h->has_tmp[i]) continue;
     mpk[i].resize((size_t)n * n);

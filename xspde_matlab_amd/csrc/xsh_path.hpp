@@ -378,7 +378,7 @@ struct Lattice {
     });
   }
 
-  // FP32 real transforms of a whole 2D plane on the matrix cores: along DIM,
+  // FP32 real transforms of the lattice block on the matrix cores: along DIM,
   // OUT[k][line] = sum_n T[k][n] IN[n][line] is a GEMM with the lines' real and
   // imaginary parts as columns.  v_mfma_f32_32x32x2f32 (lane l supplies
   // A[l%32][l/32] and B[l/32][l%32] and receives D[8(r/4) + 4(l/32) + r%4][l%32]
@@ -391,8 +391,10 @@ struct Lattice {
     static constexpr int N = Line<DIM>::N, NL = Line<DIM>::NL, NW = NT / 64;
     static constexpr int KB = N / 32, LB = NL / 32, P = KB * LB;
     static constexpr int PW = P >= NW && P % NW == 0 ? P / NW : 0;
-    static constexpr bool ok = std::is_same<R, float>::value && NY > 1 && TPB == 1 && N % 32 == 0 &&
-                               NL % 32 == 0 && PW > 0 && LB % PW == 0;
+    // 2D planes (lines = rows or columns of one trajectory) and 1D blocks
+    // (lines = the TPB trajectories of the workgroup) alike
+    static constexpr bool ok = std::is_same<R, float>::value && (NY > 1 ? TPB == 1 : DIM == 0) &&
+                               N % 32 == 0 && NL % 32 == 0 && PW > 0 && LB % PW == 0;
   };
   template <int DIM, class Op>
   __device__ static __forceinline__ void mat_mfma(C* L, const float* __restrict__ Tp, const Op& op) {
