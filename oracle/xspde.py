@@ -18,11 +18,13 @@ xpropfactor           Applications/SPDE/xpropfactor.m:140-175       linear-only 
                                                                     Batchtest rows
 xprop / xtransform    xprop.m:88-117, xtransform.m:196-245          transforms brute force,
                                                                     Batchtest rows
-xsetbound             xsetbound.m:92-121 (constant boundval)        Batchtest #21 (NN),
-                                                                    heat closed forms
+xsetbound             xsetbound.m:92-121 (boundval constants or     Batchtest #21 (NN),
+                      boundfun(a, c, d, p) at p.t)                  #42 (Peregrine), heat
+                                                                    closed forms
 xpatch, da            xpatch.m:24-62, xpropfactor.m:40-51           inhomogeneous heat
                                                                     closed forms (DD, NN,
-                                                                    DN, ND)
+                                                                    DN, ND), Batchtest #42
+                                                                    (time-dependent values)
 RK4                   Applications/SDE/RK4.m:7-42                   Batchtest #4, #8, #14,
                                                                     #16, #21
 MP                    Applications/SDE/MP.m:53-70                   order test (dt vs dt/2),
@@ -239,6 +241,40 @@ def xpropfactor(p):
     return np.exp(L * dt)
 
 
+def boundary_line(v, a_shape, d):
+    """One boundary value as an array over the lattice with axis d of length 1
+    (xboundfun.m's [1, .., 1 at d, .., E] shape): a scalar, a vector over the
+    other space dimension, or an array of the lattice's space shape (values
+    taken at the first site along d)."""
+    v = np.asarray(v, dtype=complex)
+    nsp = len(a_shape) - 2
+    shape = [1] * len(a_shape)
+    if v.size == 1:
+        return v.reshape(shape)
+    space = tuple(a_shape[1:1 + nsp])
+    if v.size == int(np.prod(space)):
+        v = np.take(v.reshape(space), [0], axis=d - 1)
+    other = [k for k in range(1, 1 + nsp) if k != d]
+    if len(other) != 1 or v.size != a_shape[other[0]]:
+        raise ValueError("boundary value of size %d does not fit dimension %d" % (v.size, d))
+    shape[other[0]] = v.size
+    return v.reshape(shape)
+
+
+def boundary_values(a, p, d):
+    """(lower, upper) boundary values of space dimension d at time p.t:
+    p.boundfun(a, c, d, p) (reference numbering: c = 1, d = 2, 3 for x, y;
+    xpatch.m:37, xsetbound.m:39) when given, else the constants p.boundval
+    (spdpreferences.m:26-43; default 0)."""
+    f = p.get("boundfun", None)
+    if f is None:
+        lo, hi = p.bvals[d - 1]
+    else:
+        b = f(a, 1, d + 1, p)
+        lo, hi = b[0], b[1]
+    return boundary_line(lo, a.shape, d), boundary_line(hi, a.shape, d)
+
+
 def xpatch(a, p):
     """Boundary patch (xpatch.m:24-62): returns (a - pa, pa + lp) where pa is the
     per-dimension polynomial matching the boundary values and lp the Neumann
@@ -248,7 +284,7 @@ def xpatch(a, p):
     for d, (t, _) in enumerate(p.btypes, start=1):
         if t == 0 or a.shape[d] <= 1:
             continue
-        b1, b2 = p.bvals[d - 1]
+        b1, b2 = boundary_values(a, p, d)
         rng = p.ranges[d]
         x = p.xs[d - 1] - p.origins[d]
         if t == -3:
@@ -266,21 +302,23 @@ def xpatch(a, p):
 
 def xsetbound(a, p):
     """Dirichlet ends set to their boundary values, dimension by dimension (later
-    dimensions win at corners) (xsetbound.m:92-121; p.boundval constants,
-    default 0, spdpreferences.m:26-43,75-93)."""
+    dimensions win at corners) (xsetbound.m:92-121): p.boundfun at p.t, else the
+    p.boundval constants (default 0, spdpreferences.m:26-43,75-93)."""
     if not p.setboundaries:
         return a
     a = np.array(a, dtype=complex)
     for d, (t, b) in enumerate(p.btypes, start=1):
-        lo, hi = p.bvals[d - 1]
+        if t == 0:
+            continue
+        lo, hi = boundary_values(a, p, d)
         if b[0] == 1:
             idx = [slice(None)] * a.ndim
-            idx[d] = 0
-            a[tuple(idx)] = lo
+            idx[d] = slice(0, 1)
+            a[tuple(idx)] = np.broadcast_to(lo, a[tuple(idx)].shape)
         if b[1] == 1:
             idx = [slice(None)] * a.ndim
-            idx[d] = -1
-            a[tuple(idx)] = hi
+            idx[d] = slice(a.shape[d] - 1, a.shape[d])
+            a[tuple(idx)] = np.broadcast_to(hi, a[tuple(idx)].shape)
     return a
 
 

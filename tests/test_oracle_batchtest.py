@@ -114,3 +114,45 @@ def test_heat_equation_inhomogeneous_boundaries(name, method):
     err = X.xsim(p)[0]
     print(name, method, err[0])
     assert err[0] < 1e-10
+
+
+# Examples/SPDE_2D/Peregrine.m (Batchtest #42): NLS with time-dependent boundary
+# values of the Peregrine solution, one field per boundary type (DD, NN, DN, ND),
+# MP, checks and order-2 extrapolation.  The reference integrates the four types
+# as four uncoupled field components of one run; here each is its own one-field
+# run and the four outputs share one xsim statistics pass, which is the same
+# error vector (the fields do not couple and ensembles = 1).
+def peregrine(x, t):
+    return np.exp(1j * t) * (4 * (1 + 2j * t) / (1 + 4 * (t ** 2 + x ** 2)) - 1)
+
+
+def peregrine_dx(x, t):
+    return -8 * x * np.exp(1j * t) * (4 * (1 + 2j * t) / (1 + 4 * (t ** 2 + x ** 2)) ** 2)
+
+
+PEREGRINE_ENDS = {  # boundaries, (lower, upper) value at time t (Peregrine.m:45-50)
+    "DD": ([1, 1], lambda x0, t: (peregrine(x0, t), peregrine(x0, t))),
+    "NN": ([-1, -1], lambda x0, t: (peregrine_dx(x0, t), -peregrine_dx(x0, t))),
+    "DN": ([1, -1], lambda x0, t: (peregrine(x0, t), -peregrine_dx(x0, t))),
+    "ND": ([-1, 1], lambda x0, t: (peregrine_dx(x0, t), peregrine(x0, t))),
+}
+
+
+def peregrine_problem(name, nobs=1):
+    b, ends = PEREGRINE_ENDS[name]
+    sol = lambda p: np.abs(peregrine(p.x, p.t) ** 2)
+    return dict(dimensions=2, noises=1, ranges=[10, 10], origins=[-5, -5], points=[51, 161], method="MP",
+                order=2, steps=20, boundaries=[b], boundfun=lambda a, c, d, p: ends(p.origins[1], p.t),
+                initial=lambda v, p: peregrine(p.x, p.origins[0]),
+                deriv=lambda a, w, p: 1j * a * (np.conj(a) * a), linear=lambda p: 0.5j * p.Dx ** 2,
+                observe=[lambda a, p: a] * nobs,
+                output=[lambda o, p, _k=k: np.abs(o[_k]) ** 2 for k in range(nobs)], compare=[sol] * nobs)
+
+
+def test_batchtest_peregrine_time_dependent_boundaries():
+    data = [X.xensemble(1, X.xpreferences(peregrine_problem(name)))[0] for name in PEREGRINE_ENDS]
+    q = X.xpreferences(peregrine_problem("DD", nobs=4))
+    q._avshape = X._probe_shapes(q)
+    err = X.xsim_stats(q, data)[0]
+    print("Peregrine error %.7e (Batchtest 4.588574e-04)" % err[0])
+    assert 4.588574e-04 / 1.001 < err[0] < 4.588574e-04 * 1.001
