@@ -110,6 +110,17 @@ typedef struct {
                                Dirichlet ends, its derivative at Neumann ends     */
   int raw;              /* 1: keep every trajectory's field at every stored point
                            (p.raw, xpath.m:73-78), read with xsh_read_raw      */
+  /* Boundary values that change in time or along the boundary (p.boundfun,
+     called by xpatch.m:37 and xsetbound.m:39 at the method's p.t), tabulated by
+     the caller: boundtab[level][dim][lower,upper][b][re,im] with
+     b < (dimensions == 3 ? max(nx, ny) : 1) running along the other space
+     dimension (y for the x ends, x for the y ends).  Level m is the time
+     origins[0] + m * dtr / 2 with dtr the fine step ranges[0] / ((points - 1) *
+     steps * (1 + checks)): boundlevels is 1 (fixed values) or
+     2 * (points * steps - 1) * (1 + checks) + 1 (one level per half fine step of
+     the path; coarse steps use the even levels).  NULL: the boundval constants. */
+  const double* boundtab;
+  int boundlevels;
 } xsh_desc;
 
 typedef struct xsh_handle xsh_handle;

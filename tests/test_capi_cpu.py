@@ -2,6 +2,8 @@
 entry point, and rejects invalid descriptors before touching a device
 (the analogue of the reference's error() aborts, xtransform.m:225-226)."""
 import ctypes as C
+
+import numpy as np
 import os
 import re
 
@@ -116,3 +118,47 @@ def test_batch_instances_generate_valid_descriptors():
         assert (p["points"][0] - 1) * p["steps"][0] == 500
         d = make_desc(preferences(p))
         assert d.dimensions == len(p["points"]) and d.method == 0
+
+
+def test_boundary_table_levels_and_times():
+    """p.boundfun tabulated per half fine step: level m holds the values at
+    origins(1) + m dtr/2 (dtr the fine step), 2 n (1 + checks) + 1 levels for n
+    method calls per path; values that do not change in time give one level."""
+    from xspde_matlab_amd.xsim import boundary_table
+    p = configs.shrink(configs.nls1d(), E=16, e2=1, points=3, steps=2)
+    p["boundaries"] = [[1, -1]]
+    p["origins"] = [0.5, -10.0]
+    p["boundfun"] = lambda a, c, d, p: (np.exp(1j * p.t), p.t ** 2)
+    q = preferences(p)
+    tab = boundary_table(q)
+    calls = 3 * 2 - 1
+    assert tab.shape == (2 * calls * 2 + 1, 2, 2, 1)
+    dtr = q["dt"] / (2 * 2)
+    t = 0.5 + np.arange(tab.shape[0]) * dtr / 2
+    assert np.allclose(tab[:, 0, 0, 0], np.exp(1j * t)) and np.allclose(tab[:, 0, 1, 0], t ** 2)
+    assert not tab[:, 1].any()
+    p["boundfun"] = lambda a, c, d, p: (0.25, -1j)
+    tab = boundary_table(preferences(p))
+    assert tab.shape == (1, 2, 2, 1) and tab[0, 0, 0, 0] == 0.25 and tab[0, 0, 1, 0] == -1j
+    d = make_desc(preferences(p))
+    assert d.boundlevels == 1
+    d.boundlevels = 7
+    rc, msg = _create(d)
+    assert rc == 1 and "boundlevels" in msg
+
+
+def test_boundary_table_lines_2d():
+    """2D: the x ends' values run along y and the y ends' along x, whether given
+    as vectors, lattice-shaped arrays or scalars."""
+    from xspde_matlab_amd.xsim import boundary_table
+    p = configs.shrink(configs.cgl2d(n=32), E=2, e2=1, points=2, steps=1)
+    p["boundaries"] = [[-1, -1], [1, 1]]
+    p["boundfun"] = lambda a, c, d, p: ((np.cos(p.y), 2.0) if d == 2 else (np.arange(32) * p.t, np.sin(p.x)))
+    q = preferences(p)
+    tab = boundary_table(q)
+    y = q["origins"][2] + np.arange(32) * q["ranges"][2] / 31
+    x = q["origins"][1] + np.arange(32) * q["ranges"][1] / 31
+    assert np.allclose(tab[3, 0, 0], np.cos(y)) and np.allclose(tab[3, 0, 1], 2.0)
+    assert np.allclose(tab[3, 1, 1], np.sin(x))
+    t3 = q["origins"][0] + 3 * q["dt"] / 2 / 2
+    assert np.allclose(tab[3, 1, 0], np.arange(32) * t3)

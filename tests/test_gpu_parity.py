@@ -262,6 +262,115 @@ def test_cgl2d_boundary_values(prec, tol, n):
     _path_case(p, tol)
 
 
+def _ends_1d(a, c, d, p):
+    """Time-dependent boundary values of the x ends (p.boundfun)."""
+    return (0.3 * np.exp(2j * p.t) + 0.1 * p.t, (-0.2 + 0.05j) * np.cos(3 * p.t))
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 5e-5)])
+@pytest.mark.parametrize("method", ["RK4", "MP"])
+@pytest.mark.parametrize("bnd", ["NN", "DD", "DN", "ND"])
+def test_nls1d_time_dependent_boundaries(prec, tol, method, bnd):
+    """p.boundfun values changing in time: tabulated per half fine step on the
+    host, read by the patch, the Neumann correction and the Dirichlet ends at the
+    method's p.t of each stage (start, midpoint, end; coarse steps at every
+    other level), against the oracle (pinned on Batchtest #42)."""
+    p = shrink(nls1d(prec, nx=128), E=20, e2=2, points=3, steps=2)
+    p["method"] = method
+    p["boundaries"] = [BOUNDS[bnd]]
+    p["boundfun"] = _ends_1d
+    p["gpu"]["observe"] = ["abs2", "a"]
+    _path_case(p, tol)
+
+
+def _ends_2d(a, c, d, p):
+    """x ends vary along y, y ends along x, both in time."""
+    if d == 2:
+        return (0.02 * np.cos(p.y) * np.exp(1j * p.t), 0.01 * np.sin(p.y + p.t))
+    return (0.05 * np.exp(-p.t) * np.cos(0.5 * p.x), 0.03 + 0.01j * p.t)
+
+
+@pytest.mark.parametrize("prec,tol,n", [("fp64", 1e-10, 64), ("fp32", 5e-5, 128)])
+def test_cgl2d_time_dependent_boundaries(prec, tol, n):
+    """2D: values along the boundary lines and in time (Neumann x, Dirichlet y)."""
+    p = shrink(cgl2d(prec, n=n), E=2, e2=2, points=3, steps=2)
+    p["boundaries"] = [[-1, -1], [1, 1]]
+    p["boundfun"] = _ends_2d
+    p["gpu"]["observe"] = ["abs2_ave", "abs2"] + (["a_ave"] if prec == "fp64" else [])
+    _path_case(p, tol)
+
+
+def _peregrine(x, t):
+    return np.exp(1j * t) * (4 * (1 + 2j * t) / (1 + 4 * (t ** 2 + x ** 2)) - 1)
+
+
+def _peregrine_dx(x, t):
+    return -8 * x * np.exp(1j * t) * (4 * (1 + 2j * t) / (1 + 4 * (t ** 2 + x ** 2)) ** 2)
+
+
+_PEREGRINE_ENDS = {
+    "DD": lambda x0, t: (_peregrine(x0, t), _peregrine(x0, t)),
+    "NN": lambda x0, t: (_peregrine_dx(x0, t), -_peregrine_dx(x0, t)),
+    "DN": lambda x0, t: (_peregrine(x0, t), -_peregrine_dx(x0, t)),
+    "ND": lambda x0, t: (_peregrine_dx(x0, t), _peregrine(x0, t)),
+}
+
+
+@pytest.mark.parametrize("prec,points,tol,etol", [("fp64", 51, 1e-8, 1e-4), ("fp32", 11, 2e-4, None)])
+def test_peregrine_time_dependent_boundaries(prec, points, tol, etol):
+    """Examples/SPDE_2D/Peregrine.m on a 128-site lattice (the GPU lattices are
+    even; the oracle reproduces the reference's 161-site Batchtest #42 value):
+    NLS with the Peregrine solution's values at both ends for each boundary type,
+    MP with checks and order-2 extrapolation.  Every (nc) path against the oracle,
+    and the xsim error vector (step error, difference from the exact solution)
+    of the GPU averages against the oracle's.  The Peregrine wave sits on a
+    modulationally unstable plane-wave background, so rounding differences grow
+    about as e^t (a 1e-15 change of the initial field reaches 7e-12 at t = 5 in
+    the oracle): FP64 runs the reference's whole t = -5..5 (3057 steps) at 1e-8,
+    FP32 the first two time units (t = -5..-3, 657 steps) at 2e-4.  The error
+    vector is compared in FP64 (to 1e-4 of each entry); FP32 rounding is of the
+    size of the short run's errors themselves."""
+    ends = {"DD": [1, 1], "NN": [-1, -1], "DN": [1, -1], "ND": [-1, 1]}
+    datas_g, datas_r = [], []
+    for name, b in ends.items():
+        f = _PEREGRINE_ENDS[name]
+        p = dict(name="Peregrine-" + name, dimensions=2, noises=1, ranges=[0.2 * (points - 1), 10.0],
+                 origins=[-5.0, -5.0],
+                 points=[points, 128], method="MP", order=2, steps=20, checks=1, ensembles=[1, 1, 1],
+                 boundaries=[b], boundfun=lambda a, c, d, p, _f=f: _f(p.origins[1], p.t),
+                 gpu=dict(deriv="gp", g=1j, s=0.0, linear=dict(l2=0.5j), initial="const", c0=0.0,
+                          observe=["a"], precision=prec))
+        q = X.xpreferences(oracle_problem(p))
+        q.output = [lambda o, p: np.abs(o[0]) ** 2]
+        q.compare = [lambda p: np.abs(_peregrine(p.x, p.t)) ** 2]
+        x = q.xc[1]
+        a0 = _peregrine(x, -5.0)
+        prob = Problem(p)
+        gpu_av = {}
+        for nc in (1, 2):
+            av_g, a_g = prob.xpath(nc, npar=1, j=1, a0=a0[None, :], want_final=True)
+            a_r, av_r = X.run_path(q, 1, nc, a0=a0.reshape(1, -1, 1))
+            _cmp(a_g.reshape(-1), a_r.reshape(-1), tol, "Peregrine %s final field nc=%d" % (name, nc))
+            _cmp(av_g[0], av_r[0], tol, "Peregrine %s <a> nc=%d" % (name, nc))
+            gpu_av[nc] = av_g
+        prob.close()
+        datas_g.append(X.xensemble(1, q, path_fn=lambda p_, j, nc, npar: gpu_av[nc])[0])
+        datas_r.append(X.xensemble(1, q, path_fn=lambda p_, j, nc, npar, _q=q, _a0=a0:
+                                   X.run_path(_q, 1, nc, a0=_a0.reshape(1, -1, 1))[1])[0])
+    q4 = X.xpreferences(dict(oracle_problem(p), observe=[lambda a, p: a] * 4))
+    q4.output = [lambda o, p, _k=k: np.abs(o[_k]) ** 2 for k in range(4)]
+    q4.compare = [lambda p: np.abs(_peregrine(p.x, p.t)) ** 2] * 4
+    q4._avshape = X._probe_shapes(q4)
+    eg = X.xsim_stats(q4, datas_g)[0]
+    er = X.xsim_stats(q4, datas_r)[0]
+    print("Peregrine 128 sites: error gpu %.7e oracle %.7e; step %.3e / %.3e; difference %.3e / %.3e"
+          % (eg[0], er[0], eg[1], er[1], eg[3], er[3]), flush=True)
+    if etol is None:
+        return
+    assert abs(eg[0] - er[0]) <= etol * er[0]
+    assert abs(eg[3] - er[3]) <= etol * er[3]
+
+
 def _oracle_snapshots(q, j, nc):
     """The oracle path's fields at every stored point (xpath.m:73-78)."""
     snaps = []

@@ -47,7 +47,16 @@ class Desc(C.Structure):
         ("nobserve", C.c_int), ("observe", C.c_int * XSH_MAX_OBS), ("seq", C.c_int), ("inst", C.c_int),
         ("traj_out", C.c_int), ("nmodes", C.c_int), ("modes", (C.c_int * 2) * XSH_MAX_MODES),
         ("boundval", ((C.c_double * 2) * 2) * 2), ("raw", C.c_int),
+        ("boundtab", C.POINTER(C.c_double)), ("boundlevels", C.c_int),
     ]
+
+    def set_boundary_table(self, tab):
+        """Attach a boundary-value table [levels][2][2][nb] complex (kept alive
+        with the descriptor until xsh_create has copied it)."""
+        tab = np.ascontiguousarray(tab, dtype=np.complex128)
+        self._boundtab = tab
+        self.boundtab = tab.ctypes.data_as(C.POINTER(C.c_double))
+        self.boundlevels = tab.shape[0]
 
 
 class Info(C.Structure):

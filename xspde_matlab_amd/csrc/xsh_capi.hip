@@ -55,10 +55,13 @@ struct xsh_handle {
   size_t esize;
   void* d_tables;   // px[2], py[2], twx, twy, transform matrices, half-length twiddles,
                     // matrix-core operand copies of the real matrices
-  size_t tab_off[16];
+  size_t tab_off[17];
   bool has_tmp[4];
   bool has_patch;      // non-zero boundary values on a non-periodic dimension
-  double lp[2][2];     // [nc] Neumann patch correction (complex)
+  int bval_level;      // boundary-value table: elements per time level (0: one level)
+  int bval_nb;         // values per boundary line
+  int bshape_n;        // entries per patch shape function
+  double lpf[2][2][2]; // [nc][dim] Neumann correction per unit of b_upper - b_lower
   void* d_work;      // streaming kernels: field planes per workgroup
   int work_planes;
   int btype[2];      // boundary type per space dimension (0 periodic)
@@ -132,6 +135,15 @@ static int validate(const xsh_desc* d) {
   if (d->traj_out != 0 && d->traj_out != 1) return fail(XSH_EINVAL, "traj_out must be 0 or 1");
   if (d->raw != 0 && d->raw != 1) return fail(XSH_EINVAL, "raw must be 0 or 1");
   if (d->raw && d->dimensions < 2) return fail(XSH_EUNSUPPORTED, "raw fields are kept for lattice problems");
+  if (d->boundtab) {
+    const long long calls = d->points > 1 ? (long long)d->points * d->steps - 1 : d->steps;
+    const long long full = 2 * calls * (1 + d->checks) + 1;
+    if (d->boundlevels != 1 && d->boundlevels != full)
+      return fail(XSH_EINVAL, "boundlevels %d: must be 1 or %lld (one per half fine step)", d->boundlevels, full);
+    const long long nb = d->dimensions == 3 ? std::max(n[0], n[1]) : 1;
+    if ((long long)d->boundlevels * 4 * nb * 16 > (1LL << 31))
+      return fail(XSH_EUNSUPPORTED, "boundary-value table beyond 2 GiB");
+  }
   if (d->nmodes < 0 || d->nmodes > XSH_MAX_MODES) return fail(XSH_EINVAL, "nmodes must be 0..%d", XSH_MAX_MODES);
   if (d->nmodes > 0) {
     if (d->initial != XSH_INIT_COHERENT) return fail(XSH_EUNSUPPORTED, "noise modes need the coherent initial form");
@@ -395,52 +407,68 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     h->tab_off[11 + i] = total;
     total += ((mpk[i].size() * sizeof(float) + 255) / 256) * 256;
   }
-  // Boundary patch for constant boundary values (xpatch.m:24-62): pa(x) summed
-  // over the non-periodic dimensions, and the Neumann correction lp = db * da with
-  // da = -(L(k1) - L(k0)) dt range^2 / pi^2 from the first two k values of the
-  // dimension (xpropfactor.m:40-51), per check (dt = dtr/2).
-  std::vector<cd> patch;
+  // Boundary values (the boundval constants, or the caller's boundfun table per
+  // time level) and the patch of xpatch.m:24-62: pa(x, y) = sum over the
+  // non-periodic dimensions of b_lower s0(x_d) + b_upper s1(x_d) with the shape
+  // functions of each boundary type, and the Neumann correction
+  // lp = (b_upper - b_lower) / range * da, da = -(L(k1) - L(k0)) dt range^2 / pi^2
+  // from the first two k values of the dimension (xpropfactor.m:40-51), per
+  // check (dt = dtr/2).
+  const int nsh = std::max(h->nx, h->ny);
+  const int nbv = d.dimensions == 3 ? nsh : 1;
+  const int nlev = d.boundtab ? d.boundlevels : 1;
+  std::vector<double> bshape((size_t)4 * nsh, 0.0);
+  std::vector<cd> bvals((size_t)nlev * 4 * nbv, cd(0.0, 0.0));
   h->has_patch = false;
-  for (int k = 0; k < d.dimensions - 1; ++k)
-    for (int side = 0; side < 2; ++side)
-      if (h->btype[k] != 0 && (d.boundval[k][side][0] != 0.0 || d.boundval[k][side][1] != 0.0)) h->has_patch = true;
-  if (h->has_patch) {
-    patch.assign(h->ns, cd(0.0, 0.0));
-    for (int nci = 0; nci < 2; ++nci) h->lp[nci][0] = h->lp[nci][1] = 0.0;
-    for (int k = 0; k < d.dimensions - 1; ++k) {
-      const int t = h->btype[k], n = npt[k];
-      if (t == 0) continue;
-      const cd b1(d.boundval[k][0][0], d.boundval[k][0][1]), b2(d.boundval[k][1][0], d.boundval[k][1][1]);
-      const double rng = d.ranges[k + 1];
-      for (int site = 0; site < h->ns; ++site) {
-        const int m = k == 0 ? site % h->nx : site / h->nx;
-        const double x = m * h->dx[k];
-        cd pa(0.0, 0.0);
-        switch (t) {
-          case -3: pa = b1 * x + 0.5 * x * x * (b2 - b1) / rng; break;
-          case -1: pa = b1 + x * b2; break;
-          case 1: pa = b2 + (x - rng) * b1; break;
-          case 3: pa = b1 + x * (b2 - b1) / rng; break;
-        }
-        patch[site] += pa;
+  for (int nci = 0; nci < 2; ++nci)
+    for (int k = 0; k < 2; ++k) h->lpf[nci][k][0] = h->lpf[nci][k][1] = 0.0;
+  for (int k = 0; k < d.dimensions - 1; ++k) {
+    const int t = h->btype[k], n = npt[k];
+    if (t == 0) continue;
+    const double rng = d.ranges[k + 1];
+    for (int m = 0; m < n; ++m) {
+      const double x = m * h->dx[k];
+      double s0 = 0.0, s1 = 0.0;
+      switch (t) {
+        case -3: s0 = x - 0.5 * x * x / rng; s1 = 0.5 * x * x / rng; break;
+        case -1: s0 = 1.0; s1 = x; break;
+        case 1: s0 = x - rng; s1 = 1.0; break;
+        case 3: s0 = 1.0 - x / rng; s1 = x / rng; break;
       }
-      if (t == -3) {
-        const double dkp = M_PI / ((n - 1) * h->dx[k]);
-        const double k0 = 0.0, k1 = dkp;  // NN k-grid: m * dkp (spdpreferences.m:56-57)
-        const cd l1(d.lin_l1[k][0], d.lin_l1[k][1]), l2(d.lin_l2[k][0], d.lin_l2[k][1]);
-        auto f = [&](double kk) { return l1 * cd(0, kk) + l2 * cd(-kk * kk, 0); };
-        const cd db = (b2 - b1) / rng;
-        for (int nci = 0; nci < 2; ++nci) {
-          const double dtp = h->dtr[nci] / 2.0;
-          const cd da = d.has_linear ? -(f(k1) - f(k0)) * dtp * rng * rng / (M_PI * M_PI) : cd(0, 0);
-          h->lp[nci][0] += (db * da).real();
-          h->lp[nci][1] += (db * da).imag();
+      bshape[(size_t)(2 * k) * nsh + m] = s0;
+      bshape[(size_t)(2 * k + 1) * nsh + m] = s1;
+    }
+    const int nother = d.dimensions == 3 ? npt[1 - k] : 1;
+    for (int lev = 0; lev < nlev; ++lev)
+      for (int side = 0; side < 2; ++side)
+        for (int o = 0; o < nother; ++o) {
+          const size_t at = (((size_t)lev * 2 + k) * 2 + side) * nbv + o;
+          const cd v = d.boundtab ? cd(d.boundtab[2 * at], d.boundtab[2 * at + 1])
+                                  : cd(d.boundval[k][side][0], d.boundval[k][side][1]);
+          bvals[at] = v;
+          if (v != cd(0.0, 0.0)) h->has_patch = true;
         }
+    if (t == -3 && d.has_linear) {
+      const double dkp = M_PI / ((n - 1) * h->dx[k]);
+      const double k0 = 0.0, k1 = dkp;  // NN k-grid: m * dkp (spdpreferences.m:56-57)
+      const cd l1(d.lin_l1[k][0], d.lin_l1[k][1]), l2(d.lin_l2[k][0], d.lin_l2[k][1]);
+      auto f = [&](double kk) { return l1 * cd(0, kk) + l2 * cd(-kk * kk, 0); };
+      for (int nci = 0; nci < 2; ++nci) {
+        const double dtp = h->dtr[nci] / 2.0;
+        const cd da = -(f(k1) - f(k0)) * dtp * rng * rng / (M_PI * M_PI);
+        h->lpf[nci][k][0] = (da / rng).real();
+        h->lpf[nci][k][1] = (da / rng).imag();
       }
     }
   }
+  h->d.boundtab = nullptr;  // copied: the caller's table is not referenced again
+  h->bval_level = nlev > 1 ? 4 * nbv : 0;
+  h->bval_nb = nbv;
+  h->bshape_n = nsh;
   h->tab_off[15] = total;
-  total += ((patch.size() * h->esize + 255) / 256) * 256;
+  total += ((bshape.size() * h->esize / 2 + 255) / 256) * 256;
+  h->tab_off[16] = total;
+  total += ((bvals.size() * h->esize + 255) / 256) * 256;
   std::vector<char> host(total + 256);
   for (int i = 0; i < 6; ++i) {
     if (d.precision == XSH_FP64)
@@ -460,11 +488,12 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     pack<float>(host, h->tab_off[10], twh);
   for (int i = 0; i < 4; ++i)
     if (h->has_tmp[i]) pack_real<float>(host, h->tab_off[11 + i], mpk[i]);
-  if (h->has_patch) {
-    if (d.precision == XSH_FP64)
-      pack<double>(host, h->tab_off[15], patch);
-    else
-      pack<float>(host, h->tab_off[15], patch);
+  if (d.precision == XSH_FP64) {
+    pack_real<double>(host, h->tab_off[15], bshape);
+    pack<double>(host, h->tab_off[16], bvals);
+  } else {
+    pack_real<float>(host, h->tab_off[15], bshape);
+    pack<float>(host, h->tab_off[16], bvals);
   }
   auto cleanup = [&](int code) {
     xsh_destroy(h);
@@ -617,15 +646,18 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
     A.dir_lo[k] = k < d.dimensions - 1 && d.boundaries[k][0] == 1;
     A.dir_hi[k] = k < d.dimensions - 1 && d.boundaries[k][1] == 1;
     if (A.dir_lo[k] || A.dir_hi[k]) A.dirichlet = 1;
-    for (int side = 0; side < 2; ++side)
-      for (int c = 0; c < 2; ++c) A.dval[k][side][c] = k < d.dimensions - 1 ? d.boundval[k][side][c] : 0.0;
   }
   for (int i = 0; i < 4; ++i) A.tm[i] = t + h->tab_off[6 + i];
   A.twh = t + h->tab_off[10];
   A.has_patch = h->has_patch;
-  A.patch = h->has_patch ? t + h->tab_off[15] : nullptr;
+  A.bshape = t + h->tab_off[15];
+  A.bshape_n = h->bshape_n;
+  A.bval = t + h->tab_off[16];
+  A.bval_level = h->bval_level;
+  A.bval_nb = h->bval_nb;
   for (int nci = 0; nci < 2; ++nci)
-    for (int c = 0; c < 2; ++c) A.lp[nci][c] = h->has_patch ? h->lp[nci][c] : 0.0;
+    for (int k = 0; k < 2; ++k)
+      for (int c = 0; c < 2; ++c) A.lpf[nci][k][c] = h->lpf[nci][k][c];
   for (int i = 0; i < 4; ++i) A.tmp[i] = h->has_tmp[i] ? t + h->tab_off[11 + i] : nullptr;
   A.work = h->d_work;
   A.work_planes = h->work_planes;

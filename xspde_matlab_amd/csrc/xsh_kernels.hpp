@@ -295,52 +295,65 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     return (ix == 0 && A.dir_lo[0]) || (ix == NX - 1 && A.dir_hi[0]) ||
            (NY > 1 && ((iy == 0 && A.dir_lo[1]) || (iy == NY - 1 && A.dir_hi[1])));
   };
+  // Boundary value of dimension dd, side sd at time level lev, at coordinate
+  // index o along the other dimension (p.boundfun, xpatch.m:37, xsetbound.m:39)
+  auto bvalue = [&](int lev, int dd, int sd, int o) __attribute__((always_inline)) -> C {
+    const C* bv = static_cast<const C*>(A.bval);
+    return bv[(long long)lev * A.bval_level + (dd * 2 + sd) * A.bval_nb + o];
+  };
   // value of a Dirichlet site: x ends, then y ends (xsetbound.m sets dimension
   // by dimension, so y values win at corners)
-  auto dir_value = [&](int i, int v) __attribute__((always_inline)) -> C {
+  auto dir_value = [&](int i, int v, int lev) __attribute__((always_inline)) -> C {
     const int site = Lat::point(i, v) % NS, iy = site / NX, ix = site % NX;
     int dd = -1, sd = 0;
     if (ix == 0 && A.dir_lo[0]) dd = 0, sd = 0;
     if (ix == NX - 1 && A.dir_hi[0]) dd = 0, sd = 1;
     if (NY > 1 && iy == 0 && A.dir_lo[1]) dd = 1, sd = 0;
     if (NY > 1 && iy == NY - 1 && A.dir_hi[1]) dd = 1, sd = 1;
-    return dd < 0 ? C{R(0), R(0)} : C{(R)A.dval[dd][sd][0], (R)A.dval[dd][sd][1]};
+    return dd < 0 || !A.has_patch ? C{R(0), R(0)} : bvalue(lev, dd, sd, dd == 0 ? iy : ix);
   };
   // (only dense-transform lattices can have Dirichlet ends)
-  auto setbound = [&](C (&f)[PPT]) __attribute__((always_inline)) {
+  auto setbound = [&](C (&f)[PPT], int lev) __attribute__((always_inline)) {
     if constexpr (!MAT) return;
     if (!A.dirichlet) return;
     XSH_FOR(i, 0, CPT)
       XSH_FOR(v, 0, 2)
-        if (dir_site(i, v)) f[2 * i + v] = dir_value(i, v);
+        if (dir_site(i, v)) f[2 * i + v] = dir_value(i, v, lev);
       XSH_END
     XSH_END
   };
-  // constant boundary values: a -> a - pa before the transforms, + pa + lp after
-  // (xprop.m:17-47 with xpatch.m:24-62)
-  auto patch = [&](C (&f)[PPT], R sign, bool with_lp) __attribute__((always_inline)) {
+  // non-zero boundary values: a -> a - pa before the transforms, + pa + lp
+  // after (xprop.m:17-47 with xpatch.m:24-62), pa and lp at time level lev
+  auto patch = [&](C (&f)[PPT], R sign, bool with_lp, int lev) __attribute__((always_inline)) {
     if constexpr (!MAT) return;
     if (!A.has_patch) return;
-    const C* pa = static_cast<const C*>(A.patch);
-    const C lp = with_lp ? C{(R)A.lp[nci][0], (R)A.lp[nci][1]} : C{R(0), R(0)};
+    const R* sh = static_cast<const R*>(A.bshape);
     XSH_FOR(i, 0, CPT)
       XSH_FOR(v, 0, 2)
-        f[2 * i + v] = f[2 * i + v] + sign * pa[Lat::point(i, v) % NS] + lp;
+        const int site = Lat::point(i, v) % NS, iy = site / NX, ix = site % NX;
+        C acc = C{R(0), R(0)};
+        XSH_FOR(dd, 0, (NY > 1 ? 2 : 1))
+          const int m = dd == 0 ? ix : iy, o = dd == 0 ? iy : ix;
+          const C b0 = bvalue(lev, dd, 0, o), b1 = bvalue(lev, dd, 1, o);
+          acc = acc + sign * (sh[(dd * 2) * A.bshape_n + m] * b0 + sh[(dd * 2 + 1) * A.bshape_n + m] * b1);
+          if (with_lp) acc = acc + C{(R)A.lpf[nci][dd][0], (R)A.lpf[nci][dd][1]} * (b1 - b0);
+        XSH_END
+        f[2 * i + v] = f[2 * i + v] + acc;
       XSH_END
     XSH_END
   };
   // xprop.m:88-117: transform, propagator, inverse transform, then setbound
-  auto propagate = [&](C (&f)[PPT]) __attribute__((always_inline)) {
+  auto propagate = [&](C (&f)[PPT], int lev) __attribute__((always_inline)) {
     if (A.has_linear) {
       if constexpr (MAT) {
-        patch(f, R(-1), false);
+        patch(f, R(-1), false, lev);
         Lat::spectral_mat(L, f, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop);
-        patch(f, R(1), true);
+        patch(f, R(1), true, lev);
       } else {
         Lat::spectral(L, f, twx, twy, prop);
       }
     }
-    setbound(f);
+    setbound(f, lev);
   };
 
   // All propagations of a step go through one code site that transforms f1
@@ -404,11 +417,14 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     }
   };
 
-  auto method_step = [&](uint32_t q, bool coarse) __attribute__((always_inline)) {
+  // lev: time level of the step's start in half fine steps; a coarse step
+  // spans twice the levels of a fine one
+  auto method_step = [&](uint32_t q, bool coarse, int lev) __attribute__((always_inline)) {
     const bool rk4 = A.method == XSH_RK4;
+    const int hl = coarse ? 2 : 1;
     if (rk4) {
       // RK4.m:9-14: a1 = a + D(a) dt, into f1 (a stays in f0)
-      setbound(f0);
+      setbound(f0, lev);
       chunks2(f0, f1, [&](int i, C& a0, C& a1, C& b0, C& b1) __attribute__((always_inline)) {
         C w0, w1;
         noise_chunk(i, q, coarse, w0, w1);
@@ -432,9 +448,9 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
           const C A0 = av, A1 = bv;
           const bool dz = MAT && A.dirichlet && dir_site(i, v);
           C a2 = A0 + hdt * D(A1, w);
-          if (dz) a2 = dir_value(i, v);
+          if (dz) a2 = dir_value(i, v, lev + hl);
           C a3 = A0 + (R(2) * hdt) * D(a2, w);
-          if (dz) a3 = dir_value(i, v);
+          if (dz) a3 = dir_value(i, v, lev + hl);
           bv = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
           av = a3;
         };
@@ -453,7 +469,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
           C a = a0;
           for (int it = 0; it < A.iterations; ++it) {
             a = a0 + hdt * D(a, w);
-            if (dz) a = dir_value(i, v);
+            if (dz) a = dir_value(i, v, lev + hl);
           }
           xv = R(2) * a - a0;
         };
@@ -465,7 +481,8 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     auto swap_free_stage2 = [&]() __attribute__((always_inline)) { rk4_stage2(f0, f1); };
     auto one = [&](int sp) __attribute__((always_inline)) {
       if (!rk4 && sp == 0) swap01();
-      propagate(f1);
+      // RK4: P a1, P a at the start, P a3, P R at the end; MP: start, end
+      propagate(f1, (rk4 ? sp >= 2 : sp == 1) ? lev + 2 * hl : lev);
       if (rk4 ? (sp % 2 == 0) : (sp == 1)) swap01();
       if (rk4 && sp == 1) rk4_stage2(f1, f0);
       else if (!rk4 && sp == 0) mp_stage(f1);
@@ -474,15 +491,15 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
       for (int sp = 0; sp < nprop; ++sp) one(sp);
     } else if (rk4) {
       // the same sequence written out, with no role swaps
-      propagate(f0);
-      propagate(f1);
+      propagate(f0, lev);
+      propagate(f1, lev);
       swap_free_stage2();
-      propagate(f0);  // P a3
-      propagate(f1);  // P R
+      propagate(f0, lev + 2 * hl);  // P a3
+      propagate(f1, lev + 2 * hl);  // P R
     } else {
-      propagate(f0);
+      propagate(f0, lev);
       mp_stage(f0);
-      propagate(f0);
+      propagate(f0, lev + 2 * hl);
     }
     if (rk4) {
       // RK4.m:33-41: a = P R + D(P a3) dt/3 with P a3 in f0, P R in f1
@@ -492,7 +509,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
         a0 = b0 + (hdt / R(3)) * D(a0, w0);
         a1 = b1 + (hdt / R(3)) * D(a1, w1);
       });
-      setbound(f0);
+      setbound(f0, lev + 2 * hl);
     }
   };
 
@@ -597,14 +614,19 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
 
   // ---- the time loop of xpath.m:29-86
   uint32_t q = 0;
+  int lev = 0;
   for (int n = 0; n < A.npts; ++n) {
     for (int st = 0; st < A.steps; ++st) {
       if (n > 0 || st > 0 || A.npts == 1) {
         if (A.checks && nci == 1) {
-          method_step(q, true);
+          method_step(q, true, lev);
           q += 2;
+          lev += 4;
         } else {
-          for (int c = 0; c <= A.checks; ++c) method_step(q++, false);
+          for (int c = 0; c <= A.checks; ++c) {
+            method_step(q++, false, lev);
+            lev += 2;
+          }
         }
       }
       if (st == 0) observe(n);

@@ -63,10 +63,16 @@ struct PathArgs {
   double feat_scale;       // sqrt(dv / nsites): unit-variance Re/Im
   double kscale;           // prod_d kfact_d^2, kfact = dx / sqrt(2 pi) (xpreferences.m:167)
   int dir_lo[2], dir_hi[2];// per dimension: lower / upper end is Dirichlet
-  double dval[2][2][2];    // [dim][lower,upper][re,im] Dirichlet boundary values
   int has_patch;           // non-zero boundary values: patch around every propagation
-  const void* patch;       // [nx*ny] complex R: boundary patch pa (xpatch.m:24-62)
-  double lp[2][2];         // [nc][re,im]: Neumann correction db * da added after propagation
+  const void* bval;        // complex R [level][dim][lower,upper][nb]: boundary values
+                           // along the other dimension, level m at t0 + m dtr_fine / 2
+  int bval_level;          // elements per level (0: one level for the whole path)
+  int bval_nb;             // values per boundary line
+  const void* bshape;      // real R [dim][lower,upper][n]: patch shape functions,
+                           // pa(x, y) = sum_dim,side b(side, other) * shape(side, x_dim)
+  int bshape_n;            // entries per shape function (max(nx, ny))
+  double lpf[2][2][2];     // [nc][dim][re,im]: Neumann correction per unit of
+                           // b(upper) - b(lower), da / range (xpropfactor.m:40-51)
   void* work;              // streaming kernels: per-workgroup field planes in device memory
   int work_planes;         // planes per workgroup (2, or 3 with the corr observable)
   const void* twh;         // streaming kernels: W_{ny/2}^m, m < ny/2

@@ -165,6 +165,9 @@ def make_desc(q):
         for side in range(2):
             v = complex(bv[k][side])
             d.boundval[k][side][0], d.boundval[k][side][1] = v.real, v.imag
+    tab = boundary_table(q)
+    if tab is not None:
+        d.set_boundary_table(tab)
     d.seq = int(g.get("seq", 0))
     d.inst = int(g.get("inst", 0))
     d.traj_out = 1 if g.get("traj_out") else 0
@@ -179,6 +182,82 @@ def make_desc(q):
         for m, (kx, ky) in enumerate(modes):
             d.modes[m][0], d.modes[m][1] = int(kx), int(ky)
     return d
+
+
+class Params(dict):
+    """The project's parameters with attribute access (the reference's struct p),
+    as handed to p.boundfun."""
+
+    def __getattr__(self, k):
+        try:
+            return self[k]
+        except KeyError as e:
+            raise AttributeError(k) from e
+
+    def __setattr__(self, k, v):
+        self[k] = v
+
+
+def _boundary_line(v, space, k):
+    """One boundary value of space dimension k (0: x, 1: y) as the vector of its
+    values along the other dimension: a scalar, such a vector, or an array of the
+    lattice's space shape (values taken at the first site along k)."""
+    v = np.asarray(v, dtype=complex)
+    nother = space[1 - k] if len(space) == 2 else 1
+    if v.size == 1:
+        return np.full(nother, v.reshape(-1)[0])
+    if v.size == int(np.prod(space)):
+        v = np.take(v.reshape(space), 0, axis=k)
+    if v.size != nother:
+        raise GpuFormError("boundary value of size %d does not fit dimension %d" % (v.size, k + 2))
+    return v.reshape(-1)
+
+
+def boundary_table(q):
+    """p.boundfun tabulated for the kernels (xpatch.m:37 and xsetbound.m:39 call it
+    at the method's p.t): values [level][dim][lower, upper][b] at the times
+    origins(1) + m dtr/2, m = 0 .. 2 n (1 + checks), with dtr the fine step and
+    n the method calls of one path.  One level when the values do not change in
+    time.  boundfun(a, c, d, p) is called with a = None: values that depend on
+    the fields cannot be tabulated, and such projects stay on the CPU path."""
+    f = q.get("boundfun")
+    if f is None:
+        return None
+    if not callable(f):
+        raise GpuFormError("boundfun must be a function of (a, c, d, p)")
+    nd = q["dimensions"]
+    bnd = q.get("boundaries") or []
+    dims = [k for k in range(nd - 1) if k < len(bnd) and bnd[k] is not None and any(int(b) for b in bnd[k])]
+    if not dims:
+        return None
+    space = tuple(q["points"][1:nd])
+    nb = max(space) if nd == 3 else 1
+    npts, steps, checks = q["points"][0], q["steps"][0], q["checks"][0]
+    calls = npts * steps - 1 if npts > 1 else steps
+    levels = 2 * calls * (1 + checks) + 1
+    dtr = q["dt"] / (steps * (1 + checks))
+    pp = Params(q)
+    dx = [q["ranges"][d] / max(1, q["points"][d] - 1) for d in range(nd)]
+    names = ["x", "y"]
+    for k in range(nd - 1):
+        shape = [1] * (nd + 1)
+        shape[k + 1] = space[k]
+        grid = q["origins"][k + 1] + np.arange(space[k]) * dx[k + 1]
+        pp[names[k]] = np.broadcast_to(grid.reshape(shape), (1,) + space + (1,))
+    tab = np.zeros((levels, 2, 2, nb), complex)
+    for m in range(levels):
+        pp.t = q["origins"][0] + m * dtr / 2
+        for k in dims:
+            try:
+                b = f(None, 1, k + 2, pp)
+            except (TypeError, AttributeError, IndexError) as e:
+                raise GpuFormError("boundfun depends on the fields (%s): not tabulated for the GPU" % e)
+            for side in range(2):
+                line = _boundary_line(b[side], space, k)
+                tab[m, k, side, :line.size] = line
+    if np.array_equal(tab, np.broadcast_to(tab[:1], tab.shape)):
+        tab = tab[:1]
+    return tab
 
 
 def lowest_modes(nx, ny, count):
