@@ -49,10 +49,12 @@ def make_deriv(g):
         s = float(g.get("s", 0.0))
 
         def deriv(a, w, p):
+            # noise components: the Gaussian cell, then the k-space filtered cell
+            nw = p.noises + p.get("knoises", 0)
             d = gg * (np.conj(a) * a) * a + u * a
-            if s != 0.0 and p.noises >= 2:
+            if s != 0.0 and nw >= 2:
                 d = d + s * (w[0:1] + 1j * w[1:2])
-            elif s != 0.0 and p.noises == 1:
+            elif s != 0.0 and nw == 1:
                 d = d + s * w[0:1]
             return d
         return deriv

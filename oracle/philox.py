@@ -33,6 +33,8 @@ MASK = np.uint64(0xFFFFFFFF)
 TAG_NOISE = 0
 TAG_GAUSS_INIT = 1
 TAG_UNIFORM_INIT = 2
+TAG_KNOISE = 3          # k-space filtered noises (xnoise.m:39-50)
+TAG_KRAND_INIT = 4      # k-space filtered initial randoms (xrandom.m:24-29)
 
 
 def philox4x32_10(c0, c1, c2, c3, k0, k1):

@@ -29,9 +29,11 @@ RK4                   Applications/SDE/RK4.m:7-42                   Batchtest #4
                                                                     #16, #21
 MP                    Applications/SDE/MP.m:53-70                   order test (dt vs dt/2),
                                                                     Kubo closed form
-xnoise / xrandom      Simulator/xnoise.m:133-145, xrandom.m:185-200 moments, KS test;
-                                                                    MATLAB stream itself:
-                                                                    parity unpinned
+xnoise / xrandom      Simulator/xnoise.m:26-60, xrandom.m:18-34     moments, KS test;
+  (incl. knoises,     xift.m                                        filtered-noise covariance
+  krandoms)                                                         closed form; MATLAB
+                                                                    stream itself: parity
+                                                                    unpinned
 xpath                 Simulator/xpath.m:16-86                       Batchtest rows, checks
                                                                     pairing test
 xdata                 Simulator/xdata.m:129-192 (mean branch)       Batchtest rows
@@ -93,11 +95,20 @@ def xpreferences(p_in):
     p.dimensions = nd
     p.fields = int(p.get("fields", 1))
     noises = p.get("noises", None)
-    p.noises = p.fields if noises is None else int(noises)
-    if p.get("inrandoms", None) is None and p.get("urandoms", None) is None:
-        p.inrandoms = p.noises                                          # xpreferences.m:74-76
+    p.knoises = int(p.get("knoises", 0) or 0)
+    p.krandoms = int(p.get("krandoms", 0) or 0)
+    if noises is None and p.get("knoises", None) is None and p.get("unoises", None) is None:
+        p.noises = p.fields                                             # xpreferences.m:65-70
+    else:
+        p.noises = int(noises or 0)
+    if p.get("inrandoms", None) is None and p.get("urandoms", None) is None and \
+            p.get("krandoms", None) is None:
+        p.inrandoms = p.noises                                          # xpreferences.m:71-76
     else:
         p.inrandoms = int(p.get("inrandoms", 0) or 0)
+    # k-space filters, default identity (xpreferences.m:122-123,419-425)
+    p.nfilter = p.get("nfilter", None) or (lambda w, p: w)
+    p.rfilter = p.get("rfilter", None) or (lambda w, p: w)
     p.urandoms = int(p.get("urandoms", 0) or 0)
     p.ensembles = list(p.get("ensembles", [1, 1, 1])) + [1] * 3
     p.ensembles = [int(e) for e in p.ensembles[:3]]
@@ -399,22 +410,42 @@ def _site_major_to_cells(x, comps, space, E):
     return np.ascontiguousarray(np.transpose(x, perm))
 
 
+def xift(a, p):
+    """Inverse FFT over the space dimensions (xift.m:8-11)."""
+    for d in range(1, p.dimensions):
+        a = np.fft.ifft(a, axis=d)
+    return a
+
+
 def noise_draw(p, nsp, q, seq=0, inst=0):
-    """One xnoise Gaussian draw (xnoise.m:137-139): N(0,1)*sqrt(nscale*noisefactor)."""
+    """One xnoise draw: the Gaussian cell N(0,1)*sqrt(nscale*noisefactor)
+    (xnoise.m:29-37), then the k-space cell xift(nfilter(z))*sqrt(nscale*knoisefactor)
+    with z ~ N(0,1) per k site in FFT order (xnoise.m:39-50); components stacked."""
     E = p.ensembles[0]
     ns = max(1, p.nspace)
     n = philox.normals(nsp, philox.TAG_NOISE, q, np.arange(E), ns * p.noises, seq, inst)
-    return _site_major_to_cells(n, p.noises, p.space, E) * math.sqrt(p.nscale * p.noisefactor)
+    w = _site_major_to_cells(n, p.noises, p.space, E) * math.sqrt(p.nscale * p.noisefactor)
+    if p.knoises:
+        z = philox.normals(nsp, philox.TAG_KNOISE, q, np.arange(E), ns * p.knoises, seq, inst)
+        zk = p.nfilter(_site_major_to_cells(z, p.knoises, p.space, E), p)
+        wk = xift(zk, p) * math.sqrt(p.nscale * p.knoisefactor)
+        w = np.concatenate([w.astype(complex), wk], axis=0) if p.noises else wk
+    return w
 
 
 def xrandom(p, nsp, seq=0, inst=0):
-    """Initial randoms (xrandom.m:187-200): Gaussians of variance 1/dv, then uniforms."""
+    """Initial randoms (xrandom.m:18-34): Gaussians of variance 1/dv, k-space
+    filtered randoms sqrt(knoisefactor) xift(rfilter(z)), then uniforms."""
     E = p.ensembles[0]
     ns = max(1, p.nspace)
     v = []
     if p.inrandoms:
         n = philox.normals(nsp, philox.TAG_GAUSS_INIT, 0, np.arange(E), ns * p.inrandoms, seq, inst)
         v.append(_site_major_to_cells(n, p.inrandoms, p.space, E) * math.sqrt(p.noisefactor))
+    if p.krandoms:                                                      # xrandom.m:24-29
+        z = philox.normals(nsp, philox.TAG_KRAND_INIT, 0, np.arange(E), ns * p.krandoms, seq, inst)
+        zk = p.rfilter(_site_major_to_cells(z, p.krandoms, p.space, E), p)
+        v.append(math.sqrt(p.knoisefactor) * xift(zk, p))
     if p.urandoms:
         u = philox.uniforms(nsp, philox.TAG_UNIFORM_INIT, 0, np.arange(E), ns * p.urandoms, seq, inst)
         v.append(_site_major_to_cells(u, p.urandoms, p.space, E) * p.noisefactor)
