@@ -121,6 +121,16 @@ typedef struct {
      the path; coarse steps use the even levels).  NULL: the boundval constants. */
   const double* boundtab;
   int boundlevels;
+  /* k-space filtered noises and randoms (p.knoises, p.krandoms; xnoise.m:39-50,
+     xrandom.m:24-29): knoises 0..2, krandoms 0 or 2, with the filters p.nfilter,
+     p.rfilter as complex tables nfilter[knoises][nx*ny], rfilter[krandoms][nx*ny]
+     of the multiplier at every k site in FFT order (site kx + nx*ky).  The
+     drift's noise term (w1 + i w2, or w1) and the initial forms' randoms
+     (v1 + i v2) then come from these cells (noises, inrandoms 0).  Periodic
+     lattices on the register-resident kernels; not in batches. */
+  int knoises, krandoms;
+  const double* nfilter;
+  const double* rfilter;
 } xsh_desc;
 
 typedef struct xsh_handle xsh_handle;

@@ -371,6 +371,52 @@ def test_peregrine_time_dependent_boundaries(prec, points, tol, etol):
     assert abs(eg[3] - er[3]) <= etol * er[3]
 
 
+def _kfilter_n(w, p):
+    """A filter that is neither real nor symmetric in k (xnoise.m:44)."""
+    k = p.kx if p.dimensions == 2 else p.kx - 0.5 * p.ky
+    return w * np.exp(-0.5 * (k - 0.3) ** 2) * (1 + 0.2j * k)
+
+
+def _kfilter_r(v, p):
+    k = p.kx if p.dimensions == 2 else p.kx + p.ky
+    return v * np.exp(-0.2 * k ** 2) * (0.8 - 0.3j * np.tanh(k))
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 5e-5)])
+@pytest.mark.parametrize("method", ["RK4", "MP"])
+@pytest.mark.parametrize("nx", [64, 256])
+def test_nls1d_kspace_noise(prec, tol, method, nx):
+    """knoises/krandoms (xnoise.m:39-50, xrandom.m:24-29): filtered k-space
+    normals at every k site, inverse transformed inside the kernel once per step
+    (coarse steps: mean of the two fine draws) and once for the initial state,
+    against the oracle (pinned on the filtered-noise covariances)."""
+    p = shrink(nls1d(prec, nx=nx), E=20, e2=2, points=3, steps=2)
+    p["method"] = method
+    p.update(noises=0, knoises=2, inrandoms=0, krandoms=2, nfilter=_kfilter_n, rfilter=_kfilter_r)
+    p["gpu"]["observe"] = ["abs2", "a"]
+    _path_case(p, tol)
+
+
+@pytest.mark.parametrize("prec,tol,n,ini", [("fp64", 1e-10, 64, "phase"), ("fp32", 5e-5, 128, "phase"),
+                                            ("fp64", 1e-10, 32, "coherent"), ("fp32", 5e-5, 64, "coherent")])
+def test_cgl2d_kspace_noise(prec, tol, n, ini):
+    """2D: the k sites of both dimensions in the transforms' frequency order."""
+    p = shrink(cgl2d(prec, n=n), E=3, e2=2, points=3, steps=2)
+    p.update(noises=0, knoises=2, nfilter=_kfilter_n)
+    if ini == "coherent":
+        p.update(inrandoms=0, krandoms=2, rfilter=_kfilter_r, urandoms=0)
+        p["gpu"] = dict(p["gpu"], initial="coherent", alpha=0.3 + 0.1j)
+    p["gpu"]["observe"] = ["abs2_ave", "abs2"] + (["a_ave"] if prec == "fp64" else [])
+    _path_case(p, tol)
+
+
+def test_kspace_noise_one_component():
+    """knoises = 1: the drift's noise is s w1 with w1 complex (xift of a filtered real field)."""
+    p = shrink(nls1d("fp64", nx=128), E=20, e2=1, points=3, steps=2)
+    p.update(noises=0, knoises=1, nfilter=_kfilter_n)
+    _path_case(p, 1e-10, nc_list=(1, 2), j=1)
+
+
 def _oracle_snapshots(q, j, nc):
     """The oracle path's fields at every stored point (xpath.m:73-78)."""
     snaps = []

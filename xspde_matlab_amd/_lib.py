@@ -48,7 +48,20 @@ class Desc(C.Structure):
         ("traj_out", C.c_int), ("nmodes", C.c_int), ("modes", (C.c_int * 2) * XSH_MAX_MODES),
         ("boundval", ((C.c_double * 2) * 2) * 2), ("raw", C.c_int),
         ("boundtab", C.POINTER(C.c_double)), ("boundlevels", C.c_int),
+        ("knoises", C.c_int), ("krandoms", C.c_int),
+        ("nfilter", C.POINTER(C.c_double)), ("rfilter", C.POINTER(C.c_double)),
     ]
+
+    def set_filters(self, nfilter=None, rfilter=None):
+        """Attach k-space filter tables [components][nx*ny] complex (FFT order)."""
+        if nfilter is not None:
+            self._nfilter = np.ascontiguousarray(nfilter, dtype=np.complex128)
+            self.knoises = self._nfilter.shape[0]
+            self.nfilter = self._nfilter.ctypes.data_as(C.POINTER(C.c_double))
+        if rfilter is not None:
+            self._rfilter = np.ascontiguousarray(rfilter, dtype=np.complex128)
+            self.krandoms = self._rfilter.shape[0]
+            self.rfilter = self._rfilter.ctypes.data_as(C.POINTER(C.c_double))
 
     def set_boundary_table(self, tab):
         """Attach a boundary-value table [levels][2][2][nb] complex (kept alive

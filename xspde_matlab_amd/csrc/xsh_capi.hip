@@ -55,7 +55,7 @@ struct xsh_handle {
   size_t esize;
   void* d_tables;   // px[2], py[2], twx, twy, transform matrices, half-length twiddles,
                     // matrix-core operand copies of the real matrices
-  size_t tab_off[17];
+  size_t tab_off[19];
   bool has_tmp[4];
   bool has_patch;      // non-zero boundary values on a non-periodic dimension
   int bval_level;      // boundary-value table: elements per time level (0: one level)
@@ -63,6 +63,7 @@ struct xsh_handle {
   int bshape_n;        // entries per patch shape function
   double lpf[2][2][2]; // [nc][dim] Neumann correction per unit of b_upper - b_lower
   void* d_work;      // streaming kernels: field planes per workgroup
+  void* d_kwork;     // k-space noise: x-space noise plane per trajectory of a launch
   int work_planes;
   int btype[2];      // boundary type per space dimension (0 periodic)
   int mat;           // dense per-dimension transforms in use
@@ -105,8 +106,19 @@ static int validate(const xsh_desc* d) {
   if (d->deriv < XSH_DERIV_ZERO || d->deriv > XSH_DERIV_GP) return fail(XSH_EUNSUPPORTED, "unknown drift form %d", d->deriv);
   if (d->deriv == XSH_DERIV_KUBO && d->noises < 1) return fail(XSH_EINVAL, "Kubo drift needs noises >= 1");
   if (d->initial < XSH_INIT_CONST || d->initial > XSH_INIT_COHERENT) return fail(XSH_EUNSUPPORTED, "unknown initial form %d", d->initial);
-  if ((d->initial == XSH_INIT_COHERENT || (d->initial == XSH_INIT_SECH && d->init_par[2] != 0.0)) && d->inrandoms < 2)
-    return fail(XSH_EINVAL, "initial form needs inrandoms >= 2");
+  if (d->knoises < 0 || d->knoises > 2) return fail(XSH_EUNSUPPORTED, "knoises %d: built-in forms use 0..2", d->knoises);
+  if (d->krandoms != 0 && d->krandoms != 2)
+    return fail(XSH_EUNSUPPORTED, "krandoms %d: the initial forms read v1 + i v2 (0 or 2)", d->krandoms);
+  if (d->knoises && d->noises)
+    return fail(XSH_EUNSUPPORTED, "Gaussian and k-space noise cells together are not a built-in drift form");
+  if (d->krandoms && d->inrandoms)
+    return fail(XSH_EUNSUPPORTED, "Gaussian and k-space random cells together are not a built-in initial form");
+  if ((d->knoises || d->krandoms) && d->dimensions < 2) return fail(XSH_EINVAL, "k-space noises need a lattice");
+  if ((d->knoises && !d->nfilter) || (d->krandoms && !d->rfilter))
+    return fail(XSH_EINVAL, "k-space noises need their filter table");
+  if ((d->initial == XSH_INIT_COHERENT || (d->initial == XSH_INIT_SECH && d->init_par[2] != 0.0)) &&
+      d->inrandoms < 2 && d->krandoms < 2)
+    return fail(XSH_EINVAL, "initial form needs inrandoms >= 2 (or krandoms = 2)");
   if (d->initial == XSH_INIT_PHASE && d->urandoms < 1) return fail(XSH_EINVAL, "phase initial form needs urandoms >= 1");
   if (d->initial == XSH_INIT_SECH && !(d->init_par[1] > 0)) return fail(XSH_EINVAL, "sech width must be > 0");
   if (d->initial == XSH_INIT_GAUSS && !(d->init_par[1] > 0)) return fail(XSH_EINVAL, "gauss width must be > 0");
@@ -268,7 +280,7 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
       h->btype[k] = d.boundaries[k][0] + 2 * d.boundaries[k][1];
       if (h->btype[k] != 0) h->mat = 1;
     }
-    h->kern = xsh::find_kernel(d.precision, h->nx, h->ny, h->mat);
+    h->kern = xsh::find_kernel(d.precision, h->nx, h->ny, h->mat, d.knoises || d.krandoms);
     if (!h->kern && !h->mat) h->kern = xsh::find_stream_kernel(d.precision, h->nx, h->ny);
     if (!h->kern && !h->mat) {  // periodic lattice without an FFT kernel: dense DFT
       h->mat = 1;
@@ -281,6 +293,11 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     }
     h->threads = h->kern->threads;
     h->tpb = h->kern->tpb;
+    if ((d.knoises || d.krandoms) && (h->mat || !h->kern->kn)) {
+      delete h;
+      return fail(XSH_EUNSUPPORTED, "k-space noises run on periodic lattices with register-resident fields "
+                                    "(not the %s kernel)", h->mat ? "dense-transform" : "streaming");
+    }
     // corr, lowk and the noise modes are periodic FFTs of the field whatever the
     // boundaries (the dense-transform kernels have power-of-two lattices and the
     // FFT tables too); the streaming kernel computes corr only
@@ -469,6 +486,20 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   total += ((bshape.size() * h->esize / 2 + 255) / 256) * 256;
   h->tab_off[16] = total;
   total += ((bvals.size() * h->esize + 255) / 256) * 256;
+  // k-space filters with the scales of xnoise.m:42 and xrandom.m:27 and the
+  // inverse transform's 1/N folded in (knoisefactor = N / dv, xpreferences.m:142)
+  std::vector<cd> kfil((size_t)d.knoises * h->ns), rfil((size_t)d.krandoms * h->ns);
+  {
+    const double knf = (double)h->ns / h->dv;
+    const double kns = std::sqrt(h->nscale * knf) / h->ns, krs = std::sqrt(knf) / h->ns;
+    for (size_t i = 0; i < kfil.size(); ++i) kfil[i] = cd(d.nfilter[2 * i], d.nfilter[2 * i + 1]) * kns;
+    for (size_t i = 0; i < rfil.size(); ++i) rfil[i] = cd(d.rfilter[2 * i], d.rfilter[2 * i + 1]) * krs;
+  }
+  h->d.nfilter = h->d.rfilter = nullptr;  // copied
+  h->tab_off[17] = total;
+  total += ((kfil.size() * h->esize + 255) / 256) * 256;
+  h->tab_off[18] = total;
+  total += ((rfil.size() * h->esize + 255) / 256) * 256;
   std::vector<char> host(total + 256);
   for (int i = 0; i < 6; ++i) {
     if (d.precision == XSH_FP64)
@@ -491,9 +522,13 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   if (d.precision == XSH_FP64) {
     pack_real<double>(host, h->tab_off[15], bshape);
     pack<double>(host, h->tab_off[16], bvals);
+    pack<double>(host, h->tab_off[17], kfil);
+    pack<double>(host, h->tab_off[18], rfil);
   } else {
     pack_real<float>(host, h->tab_off[15], bshape);
     pack<float>(host, h->tab_off[16], bvals);
+    pack<float>(host, h->tab_off[17], kfil);
+    pack<float>(host, h->tab_off[18], rfil);
   }
   auto cleanup = [&](int code) {
     xsh_destroy(h);
@@ -538,6 +573,12 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     if (hipMalloc(&h->d_work, wbytes) != hipSuccess)
       return cleanup(fail(XSH_ENOMEM, "hipMalloc streaming workspace (%zu bytes)", wbytes));
   }
+  if (d.knoises) {
+    // the largest launch (every (j, nc) path of an ensemble) has one workgroup per item
+    const size_t kbytes = paths * h->groups * (size_t)h->tpb * h->ns * h->esize;
+    if (hipMalloc(&h->d_kwork, kbytes) != hipSuccess)
+      return cleanup(fail(XSH_ENOMEM, "hipMalloc k-space noise planes (%zu bytes)", kbytes));
+  }
   if (hipStreamCreate(&h->stream) != hipSuccess) return cleanup(fail(XSH_EDEVICE, "hipStreamCreate"));
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_args, hipEventDisableTiming) != hipSuccess)
@@ -558,6 +599,7 @@ extern "C" void xsh_destroy(xsh_handle* h) {
   if (h->d_a) (void)hipFree(h->d_a);
   if (h->d_traj) (void)hipFree(h->d_traj);
   if (h->d_raw) (void)hipFree(h->d_raw);
+  if (h->d_kwork) (void)hipFree(h->d_kwork);
   if (h->d_feat) (void)hipFree(h->d_feat);
   if (h->d_fmap) (void)hipFree(h->d_fmap);
   if (h->d_work) (void)hipFree(h->d_work);
@@ -619,6 +661,11 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
   A.noises = d.noises;
   A.inrandoms = d.inrandoms;
   A.urandoms = d.urandoms;
+  A.knoises = d.knoises;
+  A.krandoms = d.krandoms;
+  A.kn_filter = static_cast<char*>(h->d_tables) + h->tab_off[17];
+  A.kr_filter = static_cast<char*>(h->d_tables) + h->tab_off[18];
+  A.kn_work = h->d_kwork;
   A.noise_scale = std::sqrt(h->nscale * h->noisefactor);
   A.init_scale = std::sqrt(h->noisefactor);
   A.deriv = d.deriv;
@@ -1030,6 +1077,10 @@ extern "C" int xsh_batch_create(int n, const xsh_desc* descs, int flags, xsh_bat
     if (descs[i].dimensions < 2) {
       batch_free(b);
       return fail(XSH_EUNSUPPORTED, "instance %d: batches hold lattice problems (dimensions >= 2)", i);
+    }
+    if (descs[i].knoises || descs[i].krandoms) {
+      batch_free(b);
+      return fail(XSH_EUNSUPPORTED, "instance %d: k-space noises are not in batches", i);
     }
     int rc = xsh_create(&descs[i], &b->h[i]);
     if (rc) {

@@ -136,7 +136,7 @@ __device__ __forceinline__ u4 philox10(u4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 
-enum : uint32_t { TAG_NOISE = 0, TAG_GAUSS_INIT = 1, TAG_UNIFORM_INIT = 2 };
+enum : uint32_t { TAG_NOISE = 0, TAG_GAUSS_INIT = 1, TAG_UNIFORM_INIT = 2, TAG_KNOISE = 3, TAG_KRAND_INIT = 4 };
 
 __device__ __forceinline__ uint32_t key1(uint32_t tag, uint32_t seq) {
   return (tag & 0xFFu) | (seq << 8);

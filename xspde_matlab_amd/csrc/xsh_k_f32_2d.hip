@@ -5,7 +5,7 @@ namespace xsh {
 
 static const KernelEntry kEntries[] = {
     XSH_ENTRY(float, 32, 32, 256, 1), XSH_ENTRY(float, 64, 64, 512, 1),
-    XSH_ENTRY(float, 128, 128, 1024, 1),
+    XSH_FAST_ENTRY(float, 128, 128, 1024, 1), XSH_ENTRY(float, 128, 128, 1024, 1),
 };
 
 const KernelEntry* table_f32_2d(int* n) {

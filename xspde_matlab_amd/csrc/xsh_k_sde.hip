@@ -23,13 +23,14 @@ int kernel_variants(int precision, int nx, int ny, int mat, const KernelEntry** 
   return k;
 }
 
-const KernelEntry* find_kernel(int precision, int nx, int ny, int mat) {
+const KernelEntry* find_kernel(int precision, int nx, int ny, int mat, int kn) {
   const KernelEntry* (*tables[])(int*) = {table_f32_1d, table_f32_2d, table_f64, table_mat_f32, table_mat_f64};
   for (auto tab : tables) {
     int n = 0;
     const KernelEntry* e = tab(&n);
     for (int i = 0; i < n; ++i)
-      if (e[i].precision == precision && e[i].nx == nx && e[i].ny == ny && e[i].mat == mat) return &e[i];
+      if (e[i].precision == precision && e[i].nx == nx && e[i].ny == ny && e[i].mat == mat && (!kn || e[i].kn))
+        return &e[i];
   }
   return nullptr;
 }

@@ -60,6 +60,20 @@ __device__ __forceinline__ void draw_pair(const PathArgs& A, uint32_t nsp, uint3
   }
 }
 
+// the ncomp (1 or 2) normals of k site ks in a k-space draw (counter layout
+// of every other stream: L = ks * ncomp + comp, lane L & 3)
+template <class R>
+__device__ __forceinline__ void knormals(const PathArgs& A, uint32_t nsp, uint32_t tag, uint32_t q,
+                                         uint32_t traj, int ks, int ncomp, R& z0, R& z1) {
+  R n[4];
+  const uint32_t L = (uint32_t)(ks * ncomp);
+  normals4<R>(u4{L >> 2, q, traj, (uint32_t)A.inst}, nsp, key1(tag, (uint32_t)A.seq), n);
+  const uint32_t lane = L & 3u;
+  // selects, not a runtime index: n stays in registers
+  z0 = lane == 0 ? n[0] : lane == 1 ? n[1] : lane == 2 ? n[2] : n[3];
+  z1 = ncomp < 2 ? R(0) : (lane == 0 ? n[1] : n[3]);
+}
+
 // one normal / uniform of the initial randoms (xrandom.m:187-200)
 template <class R>
 __device__ __forceinline__ R init_normal(const PathArgs& A, uint32_t nsp, uint32_t traj, int L) {
@@ -79,8 +93,9 @@ __device__ __forceinline__ R init_uniform(const PathArgs& A, uint32_t nsp, uint3
 
 // built-in initial forms (user-level p.initial handles, xensemble.m:60)
 template <class R>
+// kv: the k-space filtered randoms v1 + i v2 at this site (used when krandoms > 0)
 __device__ __forceinline__ cpx<R> initial_value(const PathArgs& A, uint32_t nsp, uint32_t traj, int site,
-                                                int ix, int iy) {
+                                                int ix, int iy, cpx<R> kv = cpx<R>{R(0), R(0)}) {
   const double x = A.x0 + ix * A.dx, y = A.y0 + iy * A.dy;
   const R isc = (R)A.init_scale;
   switch (A.initial) {
@@ -88,6 +103,7 @@ __device__ __forceinline__ cpx<R> initial_value(const PathArgs& A, uint32_t nsp,
       cpx<R> a = {(R)(A.ip[0] / cosh(x / A.ip[1])), R(0)};
       if (A.ip[2] != 0.0) {
         const R e = (R)(A.ip[2] * 0.70710678118654752440);
+        if (A.krandoms) return cpx<R>{a.x + e * kv.x, a.y + e * kv.y};
         const R v0 = isc * init_normal<R>(A, nsp, traj, site * A.inrandoms);
         const R v1 = isc * init_normal<R>(A, nsp, traj, site * A.inrandoms + 1);
         a.x += e * v0;
@@ -109,6 +125,7 @@ __device__ __forceinline__ cpx<R> initial_value(const PathArgs& A, uint32_t nsp,
       return cpx<R>{(R)A.ip[0] * c, (R)A.ip[0] * s};
     }
     case XSH_INIT_COHERENT: {
+      if (A.krandoms) return cpx<R>{(R)A.ip[0] + R(0.5) * kv.x, (R)A.ip[1] + R(0.5) * kv.y};
       const R v0 = isc * init_normal<R>(A, nsp, traj, site * A.inrandoms);
       const R v1 = isc * init_normal<R>(A, nsp, traj, site * A.inrandoms + 1);
       return cpx<R>{(R)A.ip[0] + R(0.5) * v0, (R)A.ip[1] + R(0.5) * v1};
@@ -229,7 +246,7 @@ __device__ __forceinline__ int find_instance(const int* __restrict__ wstart, int
 }
 
 // One work item (a group of TPB trajectories of one path) of one instance.
-template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX>
+template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX, bool KN>
 __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __restrict__ L, double* red) {
   using Lat = Lattice<R, NX, NY, NT, TPB, RX>;
   using C = cpx<R>;
@@ -254,6 +271,57 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     constexpr int VAR = decltype(VAR##_)::value;
 #define XSH_END });
 
+  // k-space filtered noise (xnoise.m:39-50) and randoms (xrandom.m:24-29) of
+  // the ncomp components: Z(k) = F0(k) z0(k) + i F1(k) z1(k) from the counter
+  // normals of every k site, written to LDS in the forward stages' frequency
+  // order and inverse transformed there, so that LDS then holds w1 + i w2 (one
+  // component: w1), the combination the drift and initial forms use.  Coarse
+  // steps use the mean of draws q and q + 1 (linear: the mean of the k-space
+  // values).  Runtime loops and no register arrays: the rest of the kernel's
+  // register budget is not touched.
+  auto kfield = [&](const C* __restrict__ F, int ncomp, uint32_t tag, uint32_t q, bool coarse)
+      __attribute__((always_inline)) {
+    __syncthreads();
+#pragma unroll 1
+    for (int i = 0; i < CPT; ++i) {
+      const int pt = Lat::point(i, 0);
+      const int p = pt / NS, s0 = pt - p * NS;
+      const uint32_t traj = (uint32_t)(group * TPB + p);
+      const int a = Lat::lds_of(pt);
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        const int ks = Lat::k_site(s0 + v);
+        R z0, z1;
+        knormals<R>(A, nsp, tag, q, traj, ks, ncomp, z0, z1);
+        if (coarse) {
+          R u0, u1;
+          knormals<R>(A, nsp, tag, q + 1, traj, ks, ncomp, u0, u1);
+          z0 = R(0.5) * (z0 + u0);
+          z1 = R(0.5) * (z1 + u1);
+        }
+        const C F0 = F[ks];
+        C Z = C{F0.x * z0, F0.y * z0};
+        if (ncomp >= 2) {
+          const C F1 = F[NS + ks];
+          Z = Z + C{-F1.y * z1, F1.x * z1};
+        }
+        L[a + v] = Z;
+      }
+    }
+    __syncthreads();
+    Lat::inverse_lds(L, twx, twy);
+  };
+  // this workgroup's planes of the step's x-space k-noise
+  C* const kws = KN && A.knoises ? static_cast<C*>(A.kn_work) + (long long)blockIdx.x * TPB * NS : nullptr;
+
+  XSH_FOR(k, 0, PPT) f1[k] = C{R(0), R(0)}; XSH_END
+  if constexpr (!MAT && KN) {
+    if (A.krandoms && !A.a0) {
+      kfield(static_cast<const C*>(A.kr_filter), A.krandoms, TAG_KRAND_INIT, 0u, false);
+      Lat::load(L, f1);
+    }
+  }
+
   // ---- initial fields (xensemble.m:53-65)
   XSH_FOR(i, 0, CPT)
     const int pt = Lat::point(i, 0);
@@ -265,7 +333,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
         const long long g = (long long)(traj < A.E ? traj : 0) * NS + site;
         f0[2 * i + v] = C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]};
       } else {
-        f0[2 * i + v] = initial_value<R>(A, nsp, (uint32_t)traj, site, ix, iy);
+        f0[2 * i + v] = initial_value<R>(A, nsp, (uint32_t)traj, site, ix, iy, f1[2 * i + v]);
       }
       __builtin_amdgcn_sched_barrier(0);
     XSH_END
@@ -277,6 +345,12 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   // a single field resident during every transform, the peak of the register
   // budget.
   auto noise_chunk = [&](int i, uint32_t q, bool coarse, C& w0, C& w1) __attribute__((always_inline)) {
+    if (!MAT && KN && A.knoises) {  // the step's k-noise, transformed once per step
+      const int pt = Lat::point(i, 0);
+      w0 = kws[pt];
+      w1 = kws[pt + 1];
+      return;
+    }
     const int pt = Lat::point(i, 0);
     const int p = pt / NS, s = pt % NS;
     const uint32_t traj = (uint32_t)(group * TPB + p);
@@ -422,6 +496,21 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   auto method_step = [&](uint32_t q, bool coarse, int lev) __attribute__((always_inline)) {
     const bool rk4 = A.method == XSH_RK4;
     const int hl = coarse ? 2 : 1;
+    if constexpr (!MAT && KN) {
+      if (A.knoises) {
+        // the step's noise is fixed over its stages: transform it once and keep
+        // it in this workgroup's planes (the next transform synchronises
+        // before it writes LDS again)
+        kfield(static_cast<const C*>(A.kn_filter), A.knoises, TAG_KNOISE, q, coarse);
+#pragma unroll 1
+        for (int i = 0; i < CPT; ++i) {
+          const int pt = Lat::point(i, 0);
+          const int a = Lat::lds_of(pt);
+          kws[pt] = L[a];
+          kws[pt + 1] = L[a + 1];
+        }
+      }
+    }
     if (rk4) {
       // RK4.m:9-14: a1 = a + D(a) dt, into f1 (a stays in f0)
       setbound(f0, lev);
@@ -654,7 +743,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
 // work items [wstart[i], wstart[i+1]) belong to instance i (args[i]); a
 // persistent grid strides over all nwork items.  One instance: ninst = 1,
 // wstart unused.
-template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX>
+template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX, bool KN>
 __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ args, const int* __restrict__ wstart,
                                                 int ninst, int nwork) {
   using Lat = Lattice<R, NX, NY, NT, TPB, RX>;
@@ -663,7 +752,7 @@ __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ arg
 #pragma unroll 1
   for (int wk = blockIdx.x; wk < nwork; wk += gridDim.x) {
     const int inst = ninst > 1 ? find_instance(wstart, ninst, wk) : 0;
-    spde_work<R, NX, NY, NT, TPB, MAT, RX>(args[inst], wk - (ninst > 1 ? wstart[inst] : 0), L, red);
+    spde_work<R, NX, NY, NT, TPB, MAT, RX, KN>(args[inst], wk - (ninst > 1 ? wstart[inst] : 0), L, red);
     __syncthreads();
   }
 }
@@ -719,10 +808,11 @@ static hipError_t launch_sde(const PathArgs* A, int nblocks, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX>
+template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX, bool KN>
 static hipError_t launch_spde(const PathArgs* args, const int* wstart, int ninst, int nwork, int grid,
                               hipStream_t s) {
-  hipLaunchKernelGGL((spde_path<R, NX, NY, NT, TPB, MAT, RX>), dim3(grid), dim3(NT), 0, s, args, wstart, ninst, nwork);
+  hipLaunchKernelGGL((spde_path<R, NX, NY, NT, TPB, MAT, RX, KN>), dim3(grid), dim3(NT), 0, s, args, wstart, ninst,
+                     nwork);
   return hipGetLastError();
 }
 
@@ -736,11 +826,17 @@ static hipError_t launch_probe(const PathArgs* A, int nci, int nplanes, void* pl
 
 #define XSH_ENTRY(R, NX, NY, NT, TPB) XSH_VAR_ENTRY(R, NX, NY, NT, TPB, 0)
 // a variant with first x radix RX (0: default split)
-#define XSH_VAR_ENTRY(R, NX, NY, NT, TPB, RX)                                                              \
-  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, 0, &launch_spde<R, NX, NY, NT, TPB, false, RX>, \
-   &launch_probe<R, NX, NY, NT, TPB, false, RX>, 0, Split<NX, RX>::N1}
-#define XSH_MAT_ENTRY(R, NX, NY, NT, TPB)                                                             \
-  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, 1, &launch_spde<R, NX, NY, NT, TPB, true, 0>, \
-   &launch_probe<R, NX, NY, NT, TPB, true, 0>, 0, 0}
+#define XSH_VAR_ENTRY(R, NX, NY, NT, TPB, RX)                                                                    \
+  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, 0, &launch_spde<R, NX, NY, NT, TPB, false, RX, true>, \
+   &launch_probe<R, NX, NY, NT, TPB, false, RX>, 0, Split<NX, RX>::N1, 1}
+// without the k-space noise code (its register and instruction footprint
+// costs the periodic 128x128 FP32 kernel 1 %): looked up first, skipped for
+// problems with knoises or krandoms
+#define XSH_FAST_ENTRY(R, NX, NY, NT, TPB)                                                                    \
+  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, 0, &launch_spde<R, NX, NY, NT, TPB, false, 0, false>, \
+   &launch_probe<R, NX, NY, NT, TPB, false, 0>, 0, Split<NX>::N1, 0}
+#define XSH_MAT_ENTRY(R, NX, NY, NT, TPB)                                                                   \
+  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, TPB, 1, &launch_spde<R, NX, NY, NT, TPB, true, 0, false>, \
+   &launch_probe<R, NX, NY, NT, TPB, true, 0>, 0, 0, 0}
 
 }  // namespace xsh

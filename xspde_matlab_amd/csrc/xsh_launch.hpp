@@ -20,13 +20,16 @@ struct KernelEntry {
   ProbeLaunch probe;
   int stream;  // fields in a device-memory workspace (lattices too large for the chip)
   int radix;   // first radix of the x transforms (FFT kernels)
+  int kn;      // has the k-space filtered noise code (knoises, krandoms)
 };
 
 // persistent grid of the streaming kernels: at most this many workgroups,
 // each with its own workspace planes
 constexpr int kStreamMaxBlocks = 1024;
 
-const KernelEntry* find_kernel(int precision, int nx, int ny, int mat);
+// kn: the problem has k-space filtered noises or randoms (entries without that
+// code, compiled for speed, are skipped)
+const KernelEntry* find_kernel(int precision, int nx, int ny, int mat, int kn = 0);
 // all compiled variants (threads, trajectories per workgroup, radix) of a lattice kernel
 int kernel_variants(int precision, int nx, int ny, int mat, const KernelEntry** out, int max);
 const KernelEntry* find_stream_kernel(int precision, int nx, int ny);

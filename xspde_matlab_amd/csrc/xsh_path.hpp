@@ -33,6 +33,12 @@ struct PathArgs {
   int j_only, nc_only;     // > 0: one path (j, nc)
   int groups;              // trajectory groups per path
   int noises, inrandoms, urandoms;
+  int knoises, krandoms;   // k-space filtered noise / random components (0..2)
+  const void* kn_filter;   // complex R [knoises][ns]: noise filter F(k) at the k
+                           // sites in FFT order, times sqrt(nscale knoisefactor) / ns
+  const void* kr_filter;   // complex R [krandoms][ns]: random filter, times
+                           // sqrt(knoisefactor) / ns
+  void* kn_work;           // complex R [grid * TPB][ns]: x-space k-noise of the step
   double noise_scale;      // sqrt(nscale * noisefactor), xnoise.m:139
   double init_scale;       // sqrt(noisefactor), xrandom.m:189
   int deriv;
@@ -487,6 +493,34 @@ struct Lattice {
   __device__ static __forceinline__ int kx_of_pos(int pos) {
     constexpr int N1 = Split<NX, RX>::N1, N2 = Split<NX, RX>::N2;
     return pos / N2 + N1 * (pos % N2);
+  }
+
+  // frequency index held at position pos along y after the forward stages
+  __device__ static __forceinline__ int ky_of_pos(int pos) {
+    constexpr int N1 = Split<NY>::N1, N2 = Split<NY>::N2;
+    return pos / N2 + N1 * (pos % N2);
+  }
+  // FFT-order k site (kx + NX ky) whose value sits at lattice site s after the
+  // forward stages
+  __device__ static __forceinline__ int k_site(int s) {
+    const int y = s / NX, x = s - y * NX;
+    return kx_of_pos(x) + (NY > 1 ? NX * ky_of_pos(y) : 0);
+  }
+
+  // L holds a k-space field in the forward stages' frequency order (lattice
+  // position s holds k_site(s)): L -> its inverse DFT sum_k L_k exp(+2 pi i k.x / N),
+  // no 1/N.  Called with L written and synchronised; returns synchronised.
+  __device__ static __forceinline__ void inverse_lds(C* L, const C* twx, const C* twy) {
+    if constexpr (NY > 1) {
+      dita<1>(L, twy);
+      __syncthreads();
+      ditb<1>(L);
+      __syncthreads();
+    }
+    dita<0>(L, twx);
+    __syncthreads();
+    ditb<0>(L);
+    __syncthreads();
   }
 
   // a -> T^-1 op(T a): periodic transform of every plane held in f

@@ -407,6 +407,6 @@ static hipError_t launch_probe_stream(const PathArgs* A, int nci, int nplanes, v
 
 #define XSH_STREAM_ENTRY(R, NX, NY, NT)                                                                 \
   {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, 1, 0, &launch_stream<R, NX, NY, NT>,              \
-   &launch_probe_stream<R, NX, NY, NT>, 1, Split<NX>::N1}
+   &launch_probe_stream<R, NX, NY, NT>, 1, Split<NX>::N1, 0}
 
 }  // namespace xsh

@@ -55,9 +55,13 @@ def preferences(p):
     fields = int(q.get("fields", 1))
     if fields != 1:
         raise GpuFormError("the built-in forms integrate one field component")
-    q["noises"] = int(q["noises"]) if q.get("noises") is not None else fields
-    if q.get("inrandoms") is None and q.get("urandoms") is None:
-        q["inrandoms"] = q["noises"]                                 # xpreferences.m:74-76
+    q["knoises"] = int(q.get("knoises") or 0)
+    q["krandoms"] = int(q.get("krandoms") or 0)
+    if q.get("noises") is None and not q["knoises"]:
+        q["noises"] = fields                                         # xpreferences.m:65-70
+    q["noises"] = int(q.get("noises") or 0)
+    if q.get("inrandoms") is None and q.get("urandoms") is None and not q["krandoms"]:
+        q["inrandoms"] = q["noises"]                                 # xpreferences.m:71-76
     q["inrandoms"] = int(q.get("inrandoms") or 0)
     q["urandoms"] = int(q.get("urandoms") or 0)
     q["ensembles"] = [int(e) for e in _vec(q.get("ensembles"), 3, 1)]
@@ -168,6 +172,7 @@ def make_desc(q):
     tab = boundary_table(q)
     if tab is not None:
         d.set_boundary_table(tab)
+    d.set_filters(filter_table(q, "nfilter", q["knoises"]), filter_table(q, "rfilter", q["krandoms"]))
     d.seq = int(g.get("seq", 0))
     d.inst = int(g.get("inst", 0))
     d.traj_out = 1 if g.get("traj_out") else 0
@@ -258,6 +263,38 @@ def boundary_table(q):
     if np.array_equal(tab, np.broadcast_to(tab[:1], tab.shape)):
         tab = tab[:1]
     return tab
+
+
+def filter_table(q, name, count):
+    """The k-space filter p.nfilter / p.rfilter (xnoise.m:44, xrandom.m:26) as its
+    multiplier at every k site: [count][nx*ny] complex, site kx + nx*ky in FFT
+    order, with the k grids p.kx, p.ky in that order (spdpreferences.m:61-62).
+    The default filter is the identity (xpreferences.m:419-425).  A filter that
+    is not a site-wise multiplication has no such table: GpuFormError."""
+    if not count:
+        return None
+    nd = q["dimensions"]
+    space = tuple(q["points"][1:nd])
+    shape = (count,) + space + (1,)
+    f = q.get(name)
+    if f is None:
+        return np.ones((count, int(np.prod(space))), complex)
+    pp = Params(q)
+    names = ["x", "y"]
+    for k in range(nd - 1):
+        n = space[k]
+        dx = q["ranges"][k + 1] / max(1, n - 1)
+        kc = np.fft.ifftshift((np.arange(n) - n // 2) * 2 * np.pi / (n * dx))
+        sh = [1] * (nd + 1)
+        sh[k + 1] = n
+        pp["k" + names[k]] = kc.reshape(sh)
+        pp[names[k]] = (q["origins"][k + 1] + np.arange(n) * dx).reshape(sh)
+    F = np.broadcast_to(np.asarray(f(np.ones(shape), pp), dtype=complex), shape)
+    z = np.random.default_rng(12345).standard_normal(shape)
+    if not np.allclose(np.asarray(f(z, pp), dtype=complex), F * z, rtol=1e-12, atol=1e-300):
+        raise GpuFormError("%s is not a multiplication by a function of k: no GPU table" % name)
+    # [count, x, (y), 1] -> [count, site = kx + nx*ky]
+    return np.ascontiguousarray(np.transpose(F[..., 0], (0,) + tuple(range(nd - 1, 0, -1)))).reshape(count, -1)
 
 
 def lowest_modes(nx, ny, count):
