@@ -53,10 +53,10 @@ def test_unsupported_forms_rejected():
     d.deriv = 9
     rc, msg = _create(d)
     assert rc == 2 and "drift" in msg
-    d = make_desc(q)
-    d.nx = 201   # odd lattice
+    d = make_desc(preferences(configs.shrink(configs.cgl2d(n=32), E=2, e2=1, points=3)))
+    d.nx = 33    # odd 2D lattice (1D lattices of any size run padded)
     rc, msg = _create(d)
-    assert rc == 2
+    assert rc == 2 and "odd" in msg
     d = make_desc(q)
     d.struct_size = 4
     d2 = d

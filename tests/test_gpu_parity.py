@@ -417,6 +417,64 @@ def test_kspace_noise_one_component():
     _path_case(p, 1e-10, nc_list=(1, 2), j=1)
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 5e-5)])
+@pytest.mark.parametrize("nx,bnd,method", [(101, "NN", "RK4"), (101, "NN", "MP"), (100, "DD", "RK4"),
+                                           (77, "DN", "MP"), (45, "ND", "RK4"), (161, "DD", "MP"),
+                                           (101, "PP", "RK4")])
+def test_nls1d_padded_lattice(prec, tol, nx, bnd, method):
+    """1D lattices of any size (the reference's examples use 101, 161, ... points):
+    the dense-transform kernel of the next compiled size with the physical
+    matrices in its top-left corner (padding sites are neither read nor
+    written by the physical ones), Dirichlet ends at the physical last site,
+    outputs over the physical sites only; odd periodic lattices use the padded
+    DFT matrices.  Boundary values on, against the oracle on the same lattice."""
+    if prec == "fp64" and nx > 128:
+        pytest.skip("no FP64 dense kernel above 128 sites (LDS)")
+    p = shrink(nls1d(prec, nx=nx), E=20, e2=2, points=3, steps=2)
+    p["method"] = method
+    p["boundaries"] = [BOUNDS[bnd]]
+    if bnd != "PP":
+        p["boundval"] = [[0.2 - 0.1j, 0.15j]]
+    p["gpu"]["observe"] = ["abs2", "a", "abs2_ave", "a_ave"]
+    _path_case(p, tol)
+
+
+def test_solitonderivn_known_answer():
+    """Examples/SPDE_2D/SolitonDerivN.m (Batchtest #21: 2.134464e-06, tolerance
+    0.1 %) on the GPU: NLS on 101 sites with Neumann ends (DCT-I, padded to the
+    128-site kernel), RK4 with checks and order-4 extrapolation, FP64.  The
+    observables |a|^2 and Int(|da/dx|^2) are the reference's own functions of
+    the fields, evaluated on the GPU's raw fields at every stored point; the
+    xsim error vector then follows the reference's statistics."""
+    p = dict(name="SolitonDerivN", dimensions=2, points=[101, 101], ranges=[10.0, 15.0], method="RK4", order=4,
+             noises=0, inrandoms=0, boundaries=[[-1, -1]], raw=True,
+             gpu=dict(deriv="gp", g=1j, s=0.0, linear=dict(l0=-0.5j, l2=0.5j), initial="sech", amp=1.0,
+                      width=1.0, eps=0.0, observe=["abs2"], precision="fp64"))
+    prob = Problem(p)
+    prob.handle.run_ensemble(1)
+    prob.handle.synchronize()
+    raw = prob.handle.read_raw()                       # [e2, nchk, points, E, ns]
+    prob.close()
+    obs = [lambda a, p: a * np.conj(a), lambda a, p: X.Int(np.abs(X.D1(a, 2, p)) ** 2, p)]
+    q = X.xpreferences(dict(oracle_problem(p), observe=obs))
+
+    def gpu_path(q_, j, nc, npar):
+        av = [np.zeros((1, 101, 101), complex), np.zeros((1, 101, 1), complex)]
+        for t in range(101):
+            a = raw[j - 1, nc - 1, t].T.reshape(1, 101, -1)          # [field, x, E]
+            for o in range(2):
+                av[o][:, t, :] = X.xdata(a, o, q_)
+        return av
+
+    data = X.xensemble(1, q, path_fn=gpu_path)
+    q._avshape = X._probe_shapes(q)
+    err = X.xsim_stats(q, data)[0]
+    ref = X.xsim(dict(oracle_problem(p), observe=obs))[0]
+    print("SolitonDerivN on the GPU: error %.7e (oracle %.7e, Batchtest 2.134464e-06)" % (err[0], ref[0]),
+          flush=True)
+    assert 2.134464e-06 / 1.001 < err[0] < 2.134464e-06 * 1.001
+
+
 def _oracle_snapshots(q, j, nc):
     """The oracle path's fields at every stored point (xpath.m:73-78)."""
     snaps = []

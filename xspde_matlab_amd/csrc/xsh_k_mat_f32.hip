@@ -1,12 +1,10 @@
-// Fp32 dense-transform kernels (non-periodic and mixed lattices).
+// Fp32 dense-transform kernels of 2D lattices (non-periodic and mixed).
 #include "xsh_kernels.hpp"
 
 namespace xsh {
 
 static const KernelEntry kEntries[] = {
-    XSH_MAT_ENTRY(float, 64, 1, 256, 64), XSH_MAT_ENTRY(float, 128, 1, 256, 64),
-    XSH_MAT_ENTRY(float, 256, 1, 512, 64), XSH_MAT_ENTRY(float, 64, 64, 512, 1),
-    XSH_MAT_ENTRY(float, 128, 128, 512, 1),
+    XSH_MAT_ENTRY(float, 64, 64, 512, 1), XSH_MAT_ENTRY(float, 128, 128, 512, 1),
 };
 
 const KernelEntry* table_mat_f32(int* n) {

@@ -7,12 +7,16 @@ const KernelEntry* table_f32_1d(int* n);
 const KernelEntry* table_f32_2d(int* n);
 const KernelEntry* table_f64(int* n);
 const KernelEntry* table_mat_f32(int* n);
+const KernelEntry* table_mat_f32_1d(int* n);
+const KernelEntry* table_mat_f32_1d256(int* n);
+const KernelEntry* table_f32_var2d(int* n);
 const KernelEntry* table_mat_f64(int* n);
 const KernelEntry* table_f32_var(int* n);
 
 int kernel_variants(int precision, int nx, int ny, int mat, const KernelEntry** out, int max) {
-  const KernelEntry* (*tables[])(int*) = {table_f32_1d, table_f32_2d, table_f64, table_mat_f32, table_mat_f64,
-                                          table_f32_var};
+  const KernelEntry* (*tables[])(int*) = {table_f32_1d,     table_f32_2d,        table_f64,
+                                          table_mat_f32,    table_mat_f32_1d,    table_mat_f32_1d256,
+                                          table_mat_f64,    table_f32_var,       table_f32_var2d};
   int k = 0;
   for (auto tab : tables) {
     int n = 0;
@@ -24,7 +28,9 @@ int kernel_variants(int precision, int nx, int ny, int mat, const KernelEntry** 
 }
 
 const KernelEntry* find_kernel(int precision, int nx, int ny, int mat, int kn) {
-  const KernelEntry* (*tables[])(int*) = {table_f32_1d, table_f32_2d, table_f64, table_mat_f32, table_mat_f64};
+  const KernelEntry* (*tables[])(int*) = {table_f32_1d,     table_f32_2d,        table_f64,
+                                          table_mat_f32,    table_mat_f32_1d,    table_mat_f32_1d256,
+                                          table_mat_f64};
   for (auto tab : tables) {
     int n = 0;
     const KernelEntry* e = tab(&n);
@@ -33,6 +39,19 @@ const KernelEntry* find_kernel(int precision, int nx, int ny, int mat, int kn) {
         return &e[i];
   }
   return nullptr;
+}
+
+const KernelEntry* find_padded_kernel(int precision, int nx) {
+  const KernelEntry* (*tables[])(int*) = {table_mat_f32_1d, table_mat_f32_1d256, table_mat_f64};
+  const KernelEntry* best = nullptr;
+  for (auto tab : tables) {
+    int n = 0;
+    const KernelEntry* e = tab(&n);
+    for (int i = 0; i < n; ++i)
+      if (e[i].precision == precision && e[i].ny == 1 && e[i].mat == 1 && e[i].nx >= nx && (!best || e[i].nx < best->nx))
+        best = &e[i];
+  }
+  return best;
 }
 
 hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s) {

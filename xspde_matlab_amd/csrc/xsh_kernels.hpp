@@ -251,6 +251,11 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   using Lat = Lattice<R, NX, NY, NT, TPB, RX>;
   using C = cpx<R>;
   constexpr int CPT = Lat::CPT, PPT = Lat::PPT, NS = Lat::NS;
+  // physical sites per trajectory: a 1D dense-transform lattice may hold fewer
+  // than NX (sites NSP..NX-1 are padding: zero rows and columns of every
+  // transform matrix keep them out of the physical sites, and no output reads
+  // them); every other kernel has exactly NS
+  const int NSP = MAT && NY == 1 ? A.ns : NS;
 
   int pidx, nci, j, group;
   path_identity<R>(A, lb, pidx, nci, j, group);
@@ -330,8 +335,8 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     XSH_FOR(v, 0, 2)
       const int site = s + v, iy = site / NX, ix = site % NX;
       if (A.a0) {
-        const long long g = (long long)(traj < A.E ? traj : 0) * NS + site;
-        f0[2 * i + v] = C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]};
+        const long long g = (long long)(traj < A.E ? traj : 0) * NSP + (site < NSP ? site : 0);
+        f0[2 * i + v] = site < NSP ? C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]} : C{R(0), R(0)};
       } else {
         f0[2 * i + v] = initial_value<R>(A, nsp, (uint32_t)traj, site, ix, iy, f1[2 * i + v]);
       }
@@ -366,7 +371,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   // Dirichlet ends of the lattice (xsetbound.m:102-116, default boundary value 0)
   auto dir_site = [&](int i, int v) __attribute__((always_inline)) -> bool {
     const int site = Lat::point(i, v) % NS, iy = site / NX, ix = site % NX;
-    return (ix == 0 && A.dir_lo[0]) || (ix == NX - 1 && A.dir_hi[0]) ||
+    return (ix == 0 && A.dir_lo[0]) || (ix == (MAT ? A.nx : NX) - 1 && A.dir_hi[0]) ||
            (NY > 1 && ((iy == 0 && A.dir_lo[1]) || (iy == NY - 1 && A.dir_hi[1])));
   };
   // Boundary value of dimension dd, side sd at time level lev, at coordinate
@@ -381,7 +386,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     const int site = Lat::point(i, v) % NS, iy = site / NX, ix = site % NX;
     int dd = -1, sd = 0;
     if (ix == 0 && A.dir_lo[0]) dd = 0, sd = 0;
-    if (ix == NX - 1 && A.dir_hi[0]) dd = 0, sd = 1;
+    if (ix == (MAT ? A.nx : NX) - 1 && A.dir_hi[0]) dd = 0, sd = 1;
     if (NY > 1 && iy == 0 && A.dir_lo[1]) dd = 1, sd = 0;
     if (NY > 1 && iy == NY - 1 && A.dir_hi[1]) dd = 1, sd = 1;
     return dd < 0 || !A.has_patch ? C{R(0), R(0)} : bvalue(lev, dd, sd, dd == 0 ? iy : ix);
@@ -626,9 +631,11 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
         const int traj = group * TPB + p;
         if (traj < A.E) {
           XSH_FOR(v, 0, 2)
-            const long long g = (((long long)pidx * A.npts + n) * A.E + traj) * NS + s + v;
-            A.raw[2 * g] = (double)f0[2 * i + v].x;
-            A.raw[2 * g + 1] = (double)f0[2 * i + v].y;
+            if (s + v < NSP) {
+              const long long g = (((long long)pidx * A.npts + n) * A.E + traj) * NSP + s + v;
+              A.raw[2 * g] = (double)f0[2 * i + v].x;
+              A.raw[2 * g + 1] = (double)f0[2 * i + v].y;
+            }
           XSH_END
         }
       XSH_END
@@ -659,7 +666,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
                 re = (double)a.x * a.x + (double)a.y * a.y;
                 im = 0.0;
               }
-              atomic_add_c(dst + 2 * ((long long)n * NS + s + v), re * A.inv_E, im * A.inv_E);
+              if (s + v < NSP) atomic_add_c(dst + 2 * ((long long)n * NSP + s + v), re * A.inv_E, im * A.inv_E);
             XSH_END
           }
         XSH_END
@@ -674,7 +681,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
           const int p = pt / NS, s = pt % NS;
           if (group * TPB + p < A.E) {
             XSH_FOR(v, 0, 2)
-              const C a = f0[2 * i + v];
+              const C a = s + v < NSP ? f0[2 * i + v] : C{R(0), R(0)};
               const double m = (double)a.x * a.x + (double)a.y * a.y;
               if (kind == XSH_OBS_LOWK) {
                 if (s + v == 0) re += (double)f1[2 * i + v].x;
@@ -690,7 +697,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
           }
         XSH_END
         block_sum2<NT>(re, im, red);
-        const double sc0 = kind == XSH_OBS_ABS2_INT ? A.dv : (kind == XSH_OBS_LOWK ? 1.0 : 1.0 / NS);
+        const double sc0 = kind == XSH_OBS_ABS2_INT ? A.dv : (kind == XSH_OBS_LOWK ? 1.0 : 1.0 / NSP);
         const double sc = sc0 * A.inv_E;
         if (threadIdx.x == 0) {
           atomic_add_c(dst + 2 * (long long)n, re * sc, im * sc);
@@ -728,9 +735,11 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
       const int traj = group * TPB + p;
       if (traj < A.E) {
         XSH_FOR(v, 0, 2)
-          const long long g = (long long)traj * NS + s + v;
-          A.a_out[2 * g] = (double)f0[2 * i + v].x;
-          A.a_out[2 * g + 1] = (double)f0[2 * i + v].y;
+          if (s + v < NSP) {
+            const long long g = (long long)traj * NSP + s + v;
+            A.a_out[2 * g] = (double)f0[2 * i + v].x;
+            A.a_out[2 * g + 1] = (double)f0[2 * i + v].y;
+          }
         XSH_END
       }
     XSH_END
