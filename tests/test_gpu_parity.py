@@ -428,8 +428,6 @@ def test_nls1d_padded_lattice(prec, tol, nx, bnd, method):
     written by the physical ones), Dirichlet ends at the physical last site,
     outputs over the physical sites only; odd periodic lattices use the padded
     DFT matrices.  Boundary values on, against the oracle on the same lattice."""
-    if prec == "fp64" and nx > 128:
-        pytest.skip("no FP64 dense kernel above 128 sites (LDS)")
     p = shrink(nls1d(prec, nx=nx), E=20, e2=2, points=3, steps=2)
     p["method"] = method
     p["boundaries"] = [BOUNDS[bnd]]
@@ -473,6 +471,38 @@ def test_solitonderivn_known_answer():
     print("SolitonDerivN on the GPU: error %.7e (oracle %.7e, Batchtest 2.134464e-06)" % (err[0], ref[0]),
           flush=True)
     assert 2.134464e-06 / 1.001 < err[0] < 2.134464e-06 * 1.001
+
+
+def test_peregrine_known_answer():
+    """Examples/SPDE_2D/Peregrine.m itself on the GPU (Batchtest #42: 4.588574e-04,
+    tolerance 0.1 %): 161 sites (padded to the 256-site FP64 dense-transform
+    kernel), the Peregrine solution's time-dependent values at both ends for
+    DD, NN, DN and ND, MP with checks and order-2 extrapolation, FP64; the four
+    one-field runs share one xsim statistics pass (the reference's four
+    uncoupled field components)."""
+    ends = {"DD": [1, 1], "NN": [-1, -1], "DN": [1, -1], "ND": [-1, 1]}
+    datas = []
+    for name, b in ends.items():
+        f = _PEREGRINE_ENDS[name]
+        p = dict(name="Peregrine-" + name, dimensions=2, noises=1, ranges=[10.0, 10.0], origins=[-5.0, -5.0],
+                 points=[51, 161], method="MP", order=2, steps=20, checks=1, ensembles=[1, 1, 1],
+                 boundaries=[b], boundfun=lambda a, c, d, p, _f=f: _f(p.origins[1], p.t),
+                 gpu=dict(deriv="gp", g=1j, s=0.0, linear=dict(l2=0.5j), initial="const", c0=0.0,
+                          observe=["a"], precision="fp64"))
+        q = X.xpreferences(oracle_problem(p))
+        q.output = [lambda o, p: np.abs(o[0]) ** 2]
+        a0 = _peregrine(q.xc[1], -5.0)
+        prob = Problem(p)
+        gpu_av = {nc: prob.xpath(nc, npar=1, j=1, a0=a0[None, :]) for nc in (1, 2)}
+        prob.close()
+        datas.append(X.xensemble(1, q, path_fn=lambda p_, j, nc, npar: gpu_av[nc])[0])
+    q4 = X.xpreferences(dict(oracle_problem(p), observe=[lambda a, p: a] * 4))
+    q4.output = [lambda o, p, _k=k: np.abs(o[_k]) ** 2 for k in range(4)]
+    q4.compare = [lambda p: np.abs(_peregrine(p.x, p.t)) ** 2] * 4
+    q4._avshape = X._probe_shapes(q4)
+    err = X.xsim_stats(q4, datas)[0]
+    print("Peregrine on the GPU: error %.7e (Batchtest 4.588574e-04)" % err[0], flush=True)
+    assert 4.588574e-04 / 1.001 < err[0] < 4.588574e-04 * 1.001
 
 
 def _oracle_snapshots(q, j, nc):

@@ -10,13 +10,15 @@ const KernelEntry* table_mat_f32(int* n);
 const KernelEntry* table_mat_f32_1d(int* n);
 const KernelEntry* table_mat_f32_1d256(int* n);
 const KernelEntry* table_f32_var2d(int* n);
+const KernelEntry* table_mat_f64_1d256(int* n);
 const KernelEntry* table_mat_f64(int* n);
 const KernelEntry* table_f32_var(int* n);
 
 int kernel_variants(int precision, int nx, int ny, int mat, const KernelEntry** out, int max) {
   const KernelEntry* (*tables[])(int*) = {table_f32_1d,     table_f32_2d,        table_f64,
                                           table_mat_f32,    table_mat_f32_1d,    table_mat_f32_1d256,
-                                          table_mat_f64,    table_f32_var,       table_f32_var2d};
+                                          table_mat_f64,    table_mat_f64_1d256, table_f32_var,
+                                          table_f32_var2d};
   int k = 0;
   for (auto tab : tables) {
     int n = 0;
@@ -30,7 +32,7 @@ int kernel_variants(int precision, int nx, int ny, int mat, const KernelEntry** 
 const KernelEntry* find_kernel(int precision, int nx, int ny, int mat, int kn) {
   const KernelEntry* (*tables[])(int*) = {table_f32_1d,     table_f32_2d,        table_f64,
                                           table_mat_f32,    table_mat_f32_1d,    table_mat_f32_1d256,
-                                          table_mat_f64};
+                                          table_mat_f64,    table_mat_f64_1d256};
   for (auto tab : tables) {
     int n = 0;
     const KernelEntry* e = tab(&n);
@@ -42,7 +44,8 @@ const KernelEntry* find_kernel(int precision, int nx, int ny, int mat, int kn) {
 }
 
 const KernelEntry* find_padded_kernel(int precision, int nx) {
-  const KernelEntry* (*tables[])(int*) = {table_mat_f32_1d, table_mat_f32_1d256, table_mat_f64};
+  const KernelEntry* (*tables[])(int*) = {table_mat_f32_1d, table_mat_f32_1d256, table_mat_f64,
+                                          table_mat_f64_1d256};
   const KernelEntry* best = nullptr;
   for (auto tab : tables) {
     int n = 0;

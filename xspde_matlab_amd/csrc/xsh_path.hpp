@@ -333,7 +333,10 @@ struct Lattice {
   __device__ static __forceinline__ void mat(C* L, const R* __restrict__ T, const Op& op) {
     using Ln = Line<DIM>;
     constexpr int N = Ln::N, NL = Ln::NL, KB = 8;
-    static_assert(NL % 64 == 0, "matrix transforms need whole waves of lines");
+    // lanes run over lines: with whole waves of lines the block index (and the
+    // row of T) is wave-uniform; with 32-line blocks it differs between the two
+    // half-waves and T is read per lane
+    static_assert(NL % 32 == 0, "matrix transforms need half waves of lines");
     static_assert(N % KB == 0 && N % 8 == 0, "line length must be a multiple of 8");
     constexpr int ITEMS = NL * (N / KB);
     static_assert(ITEMS % NT == 0, "items must tile the threads");
@@ -344,7 +347,7 @@ struct Lattice {
       constexpr int it = decltype(I)::value;
       const int item = opaque_tid() + it * NT;
       const int l = item % NL;
-      const int kb = __builtin_amdgcn_readfirstlane(item / NL);
+      const int kb = NL % 64 == 0 ? __builtin_amdgcn_readfirstlane(item / NL) : item / NL;
       const int b = Ln::base(l);
       const R* __restrict__ Trow = T + (long long)kb * KB * N * TW;
       C acc[KB];
