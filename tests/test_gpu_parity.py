@@ -437,40 +437,68 @@ def test_nls1d_padded_lattice(prec, tol, nx, bnd, method):
     _path_case(p, tol)
 
 
-def test_solitonderivn_known_answer():
-    """Examples/SPDE_2D/SolitonDerivN.m (Batchtest #21: 2.134464e-06, tolerance
-    0.1 %) on the GPU: NLS on 101 sites with Neumann ends (DCT-I, padded to the
-    128-site kernel), RK4 with checks and order-4 extrapolation, FP64.  The
-    observables |a|^2 and Int(|da/dx|^2) are the reference's own functions of
-    the fields, evaluated on the GPU's raw fields at every stored point; the
-    xsim error vector then follows the reference's statistics."""
-    p = dict(name="SolitonDerivN", dimensions=2, points=[101, 101], ranges=[10.0, 15.0], method="RK4", order=4,
-             noises=0, inrandoms=0, boundaries=[[-1, -1]], raw=True,
-             gpu=dict(deriv="gp", g=1j, s=0.0, linear=dict(l0=-0.5j, l2=0.5j), initial="sech", amp=1.0,
-                      width=1.0, eps=0.0, observe=["abs2"], precision="fp64"))
-    prob = Problem(p)
+def _raw_known_answer(p, obs, outputs=None, compare=None):
+    """The xsim error vector of a deterministic project (one trajectory) whose
+    observables are the reference's own functions of the fields: the GPU path
+    keeps the raw fields at every stored point (xpath.m:73-78) and the
+    observables, averages and statistics follow xdata/xensemble/xsim."""
+    prob = Problem(dict(p, raw=True))
     prob.handle.run_ensemble(1)
     prob.handle.synchronize()
     raw = prob.handle.read_raw()                       # [e2, nchk, points, E, ns]
     prob.close()
-    obs = [lambda a, p: a * np.conj(a), lambda a, p: X.Int(np.abs(X.D1(a, 2, p)) ** 2, p)]
-    q = X.xpreferences(dict(oracle_problem(p), observe=obs))
+    extra = dict(observe=obs)
+    if outputs is not None:
+        extra["output"] = outputs
+    if compare is not None:
+        extra["compare"] = compare
+    q = X.xpreferences(dict(oracle_problem(p), **extra))
+    npts = q.points[0]
+    shapes = [X.xdata(np.zeros((1,) + q.space + (1,), complex), o, q).shape for o in range(len(obs))]
 
     def gpu_path(q_, j, nc, npar):
-        av = [np.zeros((1, 101, 101), complex), np.zeros((1, 101, 1), complex)]
-        for t in range(101):
-            a = raw[j - 1, nc - 1, t].T.reshape(1, 101, -1)          # [field, x, E]
-            for o in range(2):
+        av = [np.zeros((sh[0], npts, sh[1]), complex) for sh in shapes]
+        for t in range(npts):
+            a = raw[j - 1, nc - 1, t].T.reshape((1,) + q_.space + (-1,))   # [field, x, E]
+            for o in range(len(obs)):
                 av[o][:, t, :] = X.xdata(a, o, q_)
         return av
 
     data = X.xensemble(1, q, path_fn=gpu_path)
     q._avshape = X._probe_shapes(q)
-    err = X.xsim_stats(q, data)[0]
-    ref = X.xsim(dict(oracle_problem(p), observe=obs))[0]
-    print("SolitonDerivN on the GPU: error %.7e (oracle %.7e, Batchtest 2.134464e-06)" % (err[0], ref[0]),
-          flush=True)
-    assert 2.134464e-06 / 1.001 < err[0] < 2.134464e-06 * 1.001
+    return X.xsim_stats(q, data)[0]
+
+
+_NLS_GPU = dict(deriv="gp", g=1j, s=0.0, linear=dict(l0=-0.5j, l2=0.5j), initial="sech", amp=1.0, width=1.0,
+                eps=0.0, observe=["a"], precision="fp64")
+
+
+@pytest.mark.parametrize("row", ["Soliton#4", "SolitonAv#14", "SolitonDerivN#21"])
+def test_batchtest_known_answers_on_gpu(row):
+    """Deterministic Batchtest rows run on the GPU path (FP64), tolerance 0.1 %
+    (Batchtest.m:67-81): Soliton #4 (35 periodic sites: padded DFT, RK4, no
+    checks), SolitonAv #14 (lattice average and integral of Re a, RK4 with
+    checks), SolitonDerivN #21 (101 sites, Neumann DCT-I padded to 128, order-4
+    extrapolation, Int(|da/dx|^2)).  The observables are the reference's own
+    functions of the fields, evaluated on the GPU's raw fields."""
+    sech = lambda x: 1 / np.cosh(x)
+    if row == "Soliton#4":                  # Examples/SPDE_2D/Soliton.m
+        p = dict(name="Soliton", dimensions=2, checks=0, order=2, noises=0, inrandoms=0, gpu=dict(_NLS_GPU))
+        err = _raw_known_answer(p, [lambda a, p: a], compare=[lambda p: sech(p.x)])
+        expected = 5.047219e-03
+    elif row == "SolitonAv#14":             # Examples/SPDE_2D/SolitonAv.m
+        p = dict(name="SolitonAv", dimensions=2, noises=0, inrandoms=0, gpu=dict(_NLS_GPU))
+        err = _raw_known_answer(p, [lambda a, p: X.Ave(np.real(a), p), lambda a, p: X.Int(np.real(a), p)],
+                                compare=[lambda p: np.pi / 10 + 0 * p.t, lambda p: np.pi + 0 * p.t])
+        expected = 1.410144e-02
+    else:                                   # Examples/SPDE_2D/SolitonDerivN.m
+        p = dict(name="SolitonDerivN", dimensions=2, points=[101, 101], ranges=[10.0, 15.0], method="RK4",
+                 order=4, noises=0, inrandoms=0, boundaries=[[-1, -1]], gpu=dict(_NLS_GPU))
+        err = _raw_known_answer(p, [lambda a, p: a * np.conj(a),
+                                    lambda a, p: X.Int(np.abs(X.D1(a, 2, p)) ** 2, p)])
+        expected = 2.134464e-06
+    print("%s on the GPU: error %.7e (Batchtest %.6e)" % (row, err[0], expected), flush=True)
+    assert expected / 1.001 < err[0] < expected * 1.001
 
 
 def test_peregrine_known_answer():
