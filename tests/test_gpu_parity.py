@@ -437,7 +437,7 @@ def test_nls1d_padded_lattice(prec, tol, nx, bnd, method):
     _path_case(p, tol)
 
 
-def _raw_known_answer(p, obs, outputs=None, compare=None):
+def _raw_known_answer(p, obs, outputs=None, compare=None, **extra_q):
     """The xsim error vector of a deterministic project (one trajectory) whose
     observables are the reference's own functions of the fields: the GPU path
     keeps the raw fields at every stored point (xpath.m:73-78) and the
@@ -447,7 +447,7 @@ def _raw_known_answer(p, obs, outputs=None, compare=None):
     prob.handle.synchronize()
     raw = prob.handle.read_raw()                       # [e2, nchk, points, E, ns]
     prob.close()
-    extra = dict(observe=obs)
+    extra = dict(observe=obs, **extra_q)
     if outputs is not None:
         extra["output"] = outputs
     if compare is not None:
@@ -459,7 +459,9 @@ def _raw_known_answer(p, obs, outputs=None, compare=None):
     def gpu_path(q_, j, nc, npar):
         av = [np.zeros((sh[0], npts, sh[1]), complex) for sh in shapes]
         for t in range(npts):
-            a = raw[j - 1, nc - 1, t].T.reshape((1,) + q_.space + (-1,))   # [field, x, E]
+            # raw sites x + nx*y -> [field, x, (y), E]
+            a = raw[j - 1, nc - 1, t].T.reshape(tuple(reversed(q_.space)) + (-1,))
+            a = np.transpose(a, tuple(range(len(q_.space) - 1, -1, -1)) + (len(q_.space),))[None]
             for o in range(len(obs)):
                 av[o][:, t, :] = X.xdata(a, o, q_)
         return av
@@ -473,13 +475,14 @@ _NLS_GPU = dict(deriv="gp", g=1j, s=0.0, linear=dict(l0=-0.5j, l2=0.5j), initial
                 eps=0.0, observe=["a"], precision="fp64")
 
 
-@pytest.mark.parametrize("row", ["Soliton#4", "SolitonAv#14", "SolitonDerivN#21"])
+@pytest.mark.parametrize("row", ["Soliton#4", "SolitonAv#14", "GaussianAv2D#16", "SolitonDerivN#21"])
 def test_batchtest_known_answers_on_gpu(row):
     """Deterministic Batchtest rows run on the GPU path (FP64), tolerance 0.1 %
     (Batchtest.m:67-81): Soliton #4 (35 periodic sites: padded DFT, RK4, no
     checks), SolitonAv #14 (lattice average and integral of Re a, RK4 with
-    checks), SolitonDerivN #21 (101 sites, Neumann DCT-I padded to 128, order-4
-    extrapolation, Int(|da/dx|^2)).  The observables are the reference's own
+    checks), GaussianAv2D #16 (35 x 35 periodic sites padded to the 64 x 64
+    dense kernel, k-space observable, xcompress axes), SolitonDerivN #21 (101
+    sites, Neumann DCT-I padded to 128, order-4 extrapolation, Int(|da/dx|^2)).  The observables are the reference's own
     functions of the fields, evaluated on the GPU's raw fields."""
     sech = lambda x: 1 / np.cosh(x)
     if row == "Soliton#4":                  # Examples/SPDE_2D/Soliton.m
@@ -491,6 +494,19 @@ def test_batchtest_known_answers_on_gpu(row):
         err = _raw_known_answer(p, [lambda a, p: X.Ave(np.real(a), p), lambda a, p: X.Int(np.real(a), p)],
                                 compare=[lambda p: np.pi / 10 + 0 * p.t, lambda p: np.pi + 0 * p.t])
         expected = 1.410144e-02
+    elif row == "GaussianAv2D#16":          # Examples/SPDE_3D/GaussianAv2D.m
+        p = dict(name="GaussianAv2D", dimensions=3, noises=0, inrandoms=0,
+                 gpu=dict(deriv="zero", linear=dict(l2=[0.05j, 0.05j]), initial="gauss", amp=1.0, width=1.0,
+                          observe=["abs2"], precision="fp64"))
+
+        def g2(p):
+            sig = 1 + (p.t / 10) ** 2
+            return sig ** -1 * np.exp(-(p.x ** 2 + p.y ** 2) / sig)
+        ii = lambda a, p: a * np.conj(a)
+        err = _raw_known_answer(p, [ii, lambda a, p: X.Ave(ii(a, p), p), lambda a, p: X.Int(ii(a, p), p), ii],
+                                compare=[g2, None, None, lambda r: 1], transforms=[0, 0, 0, [0, 1, 1]],
+                                axes=[None, None, None, [0, -1, -1]])
+        expected = 4.813746e-07
     else:                                   # Examples/SPDE_2D/SolitonDerivN.m
         p = dict(name="SolitonDerivN", dimensions=2, points=[101, 101], ranges=[10.0, 15.0], method="RK4",
                  order=4, noises=0, inrandoms=0, boundaries=[[-1, -1]], gpu=dict(_NLS_GPU))
@@ -499,6 +515,24 @@ def test_batchtest_known_answers_on_gpu(row):
         expected = 2.134464e-06
     print("%s on the GPU: error %.7e (Batchtest %.6e)" % (row, err[0], expected), flush=True)
     assert expected / 1.001 < err[0] < expected * 1.001
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 5e-5)])
+@pytest.mark.parametrize("shape,bnd,method", [((35, 35), ("PP", "PP"), "RK4"), ((48, 40), ("NN", "DD"), "MP"),
+                                              ((33, 64), ("DN", "PP"), "RK4"), ((64, 21), ("ND", "NN"), "MP")])
+def test_cgl2d_padded_lattice(prec, tol, shape, bnd, method):
+    """2D lattices of sizes without a compiled kernel (odd, unequal) on the 64 x 64
+    dense-transform kernel, padding per dimension: the counter streams and
+    outputs keep the physical site indices x + nx*y, boundary values at the
+    physical ends; against the oracle on the same lattice."""
+    p = shrink(cgl2d(prec, n=32), E=3, e2=2, points=3, steps=2)
+    p["points"] = [p["points"][0], shape[0], shape[1]]
+    p["method"] = method
+    p["boundaries"] = [BOUNDS[b] for b in bnd]
+    if bnd != ("PP", "PP"):
+        p["boundval"] = [[0.02, -0.01 + 0.005j], [0.05 - 0.02j, 0.03]]
+    p["gpu"]["observe"] = ["abs2_ave", "abs2", "a"] + (["a_ave"] if prec == "fp64" else [])
+    _path_case(p, tol)
 
 
 def test_peregrine_known_answer():

@@ -66,7 +66,7 @@ struct xsh_handle {
   void* d_kwork;     // k-space noise: x-space noise plane per trajectory of a launch
   int work_planes;
   int btype[2];      // boundary type per space dimension (0 periodic)
-  int npad;          // x sites of the kernel's lattice (> nx: padded 1D dense-transform lattice)
+  int npad[2];       // sites per dimension of the kernel's lattice (> nx, ny: padded dense-transform lattice)
   int mat;           // dense per-dimension transforms in use
   int cplx[2];       // per dimension: complex (DFT) matrix
   double* d_av;      // [e2*nchk][avlen] complex
@@ -138,10 +138,10 @@ static int validate(const xsh_desc* d) {
       const int t = b0 + 2 * b1;  // type index of spdpreferences.m:53-54
       if (t != -3 && t != -1 && t != 0 && t != 1 && t != 3)
         return fail(XSH_EINVAL, "Invalid boundary type: [%d,%d]", b0, b1);
-      // 1D lattices of any size run on a padded dense-transform kernel; 2D ones
-      // need a compiled (even) size
-      if (n[k] % 2 && d->dimensions != 2)
-        return fail(XSH_EUNSUPPORTED, "odd lattice size %d is not on the GPU path in 2D", n[k]);
+      // sizes without a compiled kernel run padded on a dense-transform kernel
+      // (1D up to 256 sites, 2D up to 64 x 64; kernel selection reports the rest)
+      if (n[k] % 2 && d->dimensions == 3 && n[k] > 64)
+        return fail(XSH_EUNSUPPORTED, "odd lattice size %d above 64 is not on the GPU path in 2D", n[k]);
       if (n[k] < 2) return fail(XSH_EINVAL, "lattice size %d: at least 2 points", n[k]);
       if (!(d->ranges[k + 1] > 0)) return fail(XSH_EINVAL, "ranges(%d) must be > 0", k + 2);
     } else if (n[k] != 1) {
@@ -292,7 +292,8 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
       h->kern = xsh::find_kernel(d.precision, h->nx, h->ny, 1);
     }
     if (!h->kern && h->mat && d.dimensions == 2) h->kern = xsh::find_padded_kernel(d.precision, h->nx);
-    if (h->kern && h->kern->nx > h->nx) {
+    if (!h->kern && h->mat && d.dimensions == 3) h->kern = xsh::find_padded_kernel2d(d.precision, h->nx, h->ny);
+    if (h->kern && (h->kern->nx > h->nx || h->kern->ny > h->ny)) {
       // padded 1D lattice: the outputs that transform the field periodically
       // (corr, abs2k, lowk, noise modes) would see the padding
       bool fft_obs = d.nmodes > 0;
@@ -303,7 +304,7 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
         const int np = h->kern->nx;
         delete h;
         return fail(XSH_EUNSUPPORTED, "corr, abs2k, lowk and noise modes need a lattice of a compiled size "
-                                      "(%d sites run padded to %d)", npt[0], np);
+                                      "(%d x %d sites run padded to %d)", npt[0], npt[1], np);
       }
     }
     if (!h->kern) {
@@ -313,7 +314,8 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     }
     h->threads = h->kern->threads;
     h->tpb = h->kern->tpb;
-    h->npad = h->kern->nx;
+    h->npad[0] = h->kern->nx;
+    h->npad[1] = h->kern->ny;
     if ((d.knoises || d.krandoms) && (h->mat || !h->kern->kn)) {
       delete h;
       return fail(XSH_EUNSUPPORTED, "k-space noises run on periodic lattices with register-resident fields "
@@ -334,7 +336,8 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   } else {
     h->threads = 256;
     h->tpb = 256;
-    h->npad = h->nx;
+    h->npad[0] = h->nx;
+    h->npad[1] = h->ny;
   }
   h->groups = (h->E + h->tpb - 1) / h->tpb;
 
@@ -382,7 +385,7 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
         t[m] = std::exp(L * dtp);
         if (sp && !h->mat) t[m] /= (double)n;
       }
-      if (k == 0) t.resize(h->npad, cd(0.0, 0.0));  // padding sites: propagator 0
+      t.resize(h->npad[k], cd(0.0, 0.0));  // padding sites: propagator 0
       tabs[2 * nci + k] = t;
     }
   }
@@ -414,11 +417,12 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
         mats[2 + k] = trig_matrix(inv, n);
       }
     }
-    // padded 1D lattice: the n x n matrices in the top-left corner of npad x npad
+    // padded lattice: the n x n matrices in the top-left corner of npad x npad
     // zeros, so the padding sites neither feed nor receive the physical ones
-    if (h->npad > h->nx) {
-      const int n = h->nx, np = h->npad, w = cplx[0] ? 2 : 1;
-      for (int i : {0, 2}) {
+    for (int k = 0; k < d.dimensions - 1; ++k) {
+      const int n = npt[k], np = h->npad[k], w = cplx[k] ? 2 : 1;
+      if (np == n) continue;
+      for (int i : {k, 2 + k}) {
         std::vector<double> m((size_t)np * np * w, 0.0);
         for (int r = 0; r < n; ++r)
           for (int c = 0; c < n * w; ++c) m[((size_t)r * np) * w + c] = mats[i][((size_t)r * n) * w + c];
@@ -444,7 +448,7 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   // T[32 kb + lane%32][kk + lane/32] (used by kernels that run those passes on MFMA)
   std::vector<std::vector<double>> mpk(4);
   for (int i = 0; i < 4; ++i) {
-    const int k = i % 2, n = k == 0 ? h->npad : npt[k];
+    const int k = i % 2, n = h->npad[k];
     h->has_tmp[i] = h->mat && k < d.dimensions - 1 && d.precision == XSH_FP32 && !cplx[k] && n % 32 == 0 &&
                     mats[i].size() == (size_t)n * n;
     if (!h->has_tmp[i]) continue;
@@ -465,7 +469,7 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   // lp = (b_upper - b_lower) / range * da, da = -(L(k1) - L(k0)) dt range^2 / pi^2
   // from the first two k values of the dimension (xpropfactor.m:40-51), per
   // check (dt = dtr/2).
-  const int nsh = std::max(h->npad, h->ny);
+  const int nsh = std::max(h->npad[0], h->npad[1]);
   const int nbv = d.dimensions == 3 ? nsh : 1;
   const int nlev = d.boundtab ? d.boundlevels : 1;
   std::vector<double> bshape((size_t)4 * nsh, 0.0);
@@ -890,13 +894,13 @@ extern "C" int xsh_probe_propagate(xsh_handle* h, int nc, int nplanes, double* p
   if (h->nd < 2) return fail(XSH_EINVAL, "no lattice to propagate (dimensions = 1)");
   if (nc < 1 || nc > 2 || nplanes < 1) return fail(XSH_EINVAL, "bad probe arguments");
   xsh::PathArgs A = make_args(h, 1);
-  // kernel planes hold npad x ny sites (padded 1D lattices: zeros after the nx physical ones)
-  const size_t kns = (size_t)h->npad * h->ny;
+  // kernel planes hold npad[0] x npad[1] sites (padded lattices: zeros around the physical ones)
+  const size_t kns = (size_t)h->npad[0] * h->npad[1];
   const size_t n = (size_t)nplanes * kns;
   std::vector<char> buf(n * h->esize, 0);
   for (int pl = 0; pl < nplanes; ++pl)
     for (int sx = 0; sx < h->ns; ++sx) {
-      const size_t src = (size_t)pl * h->ns + sx, dst = (size_t)pl * kns + sx;
+      const size_t src = (size_t)pl * h->ns + sx, dst = (size_t)pl * kns + sx % h->nx + (size_t)h->npad[0] * (sx / h->nx);
       if (h->d.precision == XSH_FP64) {
         reinterpret_cast<double*>(buf.data())[2 * dst] = planes[2 * src];
         reinterpret_cast<double*>(buf.data())[2 * dst + 1] = planes[2 * src + 1];
@@ -920,7 +924,7 @@ extern "C" int xsh_probe_propagate(xsh_handle* h, int nc, int nplanes, double* p
   if (e != hipSuccess) return fail(XSH_EDEVICE, "probe propagate: %s", hipGetErrorString(e));
   for (int pl = 0; pl < nplanes; ++pl)
     for (int sx = 0; sx < h->ns; ++sx) {
-      const size_t dst = (size_t)pl * h->ns + sx, src = (size_t)pl * kns + sx;
+      const size_t dst = (size_t)pl * h->ns + sx, src = (size_t)pl * kns + sx % h->nx + (size_t)h->npad[0] * (sx / h->nx);
       if (h->d.precision == XSH_FP64) {
         planes[2 * dst] = reinterpret_cast<double*>(buf.data())[2 * src];
         planes[2 * dst + 1] = reinterpret_cast<double*>(buf.data())[2 * src + 1];

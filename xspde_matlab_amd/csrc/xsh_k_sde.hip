@@ -57,6 +57,20 @@ const KernelEntry* find_padded_kernel(int precision, int nx) {
   return best;
 }
 
+const KernelEntry* find_padded_kernel2d(int precision, int nx, int ny) {
+  const KernelEntry* (*tables[])(int*) = {table_mat_f32, table_mat_f64};
+  const KernelEntry* best = nullptr;
+  for (auto tab : tables) {
+    int n = 0;
+    const KernelEntry* e = tab(&n);
+    for (int i = 0; i < n; ++i)
+      if (e[i].precision == precision && e[i].mat == 1 && e[i].ny > 1 && e[i].nx == e[i].ny && e[i].nx <= 64 &&
+          e[i].nx >= nx && e[i].ny >= ny && (!best || e[i].nx < best->nx))
+        best = &e[i];
+  }
+  return best;
+}
+
 hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s) {
   return precision == XSH_FP64 ? launch_sde<double>(A, nblocks, s) : launch_sde<float>(A, nblocks, s);
 }

@@ -60,6 +60,21 @@ __device__ __forceinline__ void draw_pair(const PathArgs& A, uint32_t nsp, uint3
   }
 }
 
+// the Gaussian noise (w1 + i w2, or w1) of one site: the same counter stream as
+// draw_pair, for sites whose neighbour in the chunk is not the next site
+template <class R>
+__device__ __forceinline__ cpx<R> draw_site(const PathArgs& A, uint32_t nsp, uint32_t q, uint32_t traj, int site) {
+  if (A.noises <= 0) return cpx<R>{R(0), R(0)};
+  const R sc = (R)A.noise_scale;
+  const uint32_t L = (uint32_t)(site * A.noises);
+  R n[4];
+  normals4<R>(u4{L >> 2, q, traj, (uint32_t)A.inst}, nsp, key1(TAG_NOISE, (uint32_t)A.seq), n);
+  const uint32_t lane = L & 3u;
+  const R a = lane == 0 ? n[0] : lane == 1 ? n[1] : lane == 2 ? n[2] : n[3];
+  const R b = A.noises >= 2 ? (lane == 0 ? n[1] : n[3]) : R(0);
+  return cpx<R>{sc * a, sc * b};
+}
+
 // the ncomp (1 or 2) normals of k site ks in a k-space draw (counter layout
 // of every other stream: L = ks * ncomp + comp, lane L & 3)
 template <class R>
@@ -251,11 +266,23 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   using Lat = Lattice<R, NX, NY, NT, TPB, RX>;
   using C = cpx<R>;
   constexpr int CPT = Lat::CPT, PPT = Lat::PPT, NS = Lat::NS;
-  // physical sites per trajectory: a 1D dense-transform lattice may hold fewer
-  // than NX (sites NSP..NX-1 are padding: zero rows and columns of every
-  // transform matrix keep them out of the physical sites, and no output reads
-  // them); every other kernel has exactly NS
-  const int NSP = MAT && NY == 1 ? A.ns : NS;
+  // Dense-transform lattices may hold fewer physical sites than NX x NY (1D of
+  // any size; 2D up to 64 x 64): the other sites are padding.  Zero rows and
+  // columns of every transform matrix keep them out of the physical sites, no
+  // output reads them, and the physical sites keep their own indices
+  // x + nx * y for the counter streams and the outputs.  FFT kernels: exactly NS.
+  constexpr bool PAD2 = MAT && NY > 1 && NX <= 64 && NY <= 64;
+  const int NXP = MAT ? A.nx : NX, NYP = MAT && NY > 1 ? A.ny : NY;
+  const int NSP = MAT ? A.ns : NS;
+  // physical index of lattice site s, -1 for padding
+  auto phys = [&](int s) __attribute__((always_inline)) -> int {
+    if constexpr (MAT) {
+      const int y = s / NX, x = s - y * NX;
+      return x < NXP && y < NYP ? x + NXP * y : -1;
+    } else {
+      return s;
+    }
+  };
 
   int pidx, nci, j, group;
   path_identity<R>(A, lb, pidx, nci, j, group);
@@ -334,11 +361,12 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     const int traj = group * TPB + p;
     XSH_FOR(v, 0, 2)
       const int site = s + v, iy = site / NX, ix = site % NX;
+      const int ps = phys(site);
       if (A.a0) {
-        const long long g = (long long)(traj < A.E ? traj : 0) * NSP + (site < NSP ? site : 0);
-        f0[2 * i + v] = site < NSP ? C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]} : C{R(0), R(0)};
+        const long long g = (long long)(traj < A.E ? traj : 0) * NSP + (ps >= 0 ? ps : 0);
+        f0[2 * i + v] = ps >= 0 ? C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]} : C{R(0), R(0)};
       } else {
-        f0[2 * i + v] = initial_value<R>(A, nsp, (uint32_t)traj, site, ix, iy, f1[2 * i + v]);
+        f0[2 * i + v] = initial_value<R>(A, nsp, (uint32_t)traj, ps >= 0 ? ps : 0, ix, iy, f1[2 * i + v]);
       }
       __builtin_amdgcn_sched_barrier(0);
     XSH_END
@@ -359,6 +387,18 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     const int pt = Lat::point(i, 0);
     const int p = pt / NS, s = pt % NS;
     const uint32_t traj = (uint32_t)(group * TPB + p);
+    if constexpr (PAD2) {
+      // padded 2D lattice: each site's own physical index (odd rows of an odd
+      // width do not start on a counter group)
+      const int p0 = phys(s), p1 = phys(s + 1);
+      w0 = p0 >= 0 ? draw_site<R>(A, nsp, q, traj, p0) : C{R(0), R(0)};
+      w1 = p1 >= 0 ? draw_site<R>(A, nsp, q, traj, p1) : C{R(0), R(0)};
+      if (coarse) {
+        if (p0 >= 0) w0 = R(0.5) * (w0 + draw_site<R>(A, nsp, q + 1, traj, p0));
+        if (p1 >= 0) w1 = R(0.5) * (w1 + draw_site<R>(A, nsp, q + 1, traj, p1));
+      }
+      return;
+    }
     draw_pair<R>(A, nsp, q, traj, s, w0, w1);
     if (coarse) {
       C u0, u1;
@@ -371,8 +411,8 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   // Dirichlet ends of the lattice (xsetbound.m:102-116, default boundary value 0)
   auto dir_site = [&](int i, int v) __attribute__((always_inline)) -> bool {
     const int site = Lat::point(i, v) % NS, iy = site / NX, ix = site % NX;
-    return (ix == 0 && A.dir_lo[0]) || (ix == (MAT ? A.nx : NX) - 1 && A.dir_hi[0]) ||
-           (NY > 1 && ((iy == 0 && A.dir_lo[1]) || (iy == NY - 1 && A.dir_hi[1])));
+    return (ix == 0 && A.dir_lo[0]) || (ix == NXP - 1 && A.dir_hi[0]) ||
+           (NY > 1 && ((iy == 0 && A.dir_lo[1]) || (iy == NYP - 1 && A.dir_hi[1])));
   };
   // Boundary value of dimension dd, side sd at time level lev, at coordinate
   // index o along the other dimension (p.boundfun, xpatch.m:37, xsetbound.m:39)
@@ -386,9 +426,9 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     const int site = Lat::point(i, v) % NS, iy = site / NX, ix = site % NX;
     int dd = -1, sd = 0;
     if (ix == 0 && A.dir_lo[0]) dd = 0, sd = 0;
-    if (ix == (MAT ? A.nx : NX) - 1 && A.dir_hi[0]) dd = 0, sd = 1;
+    if (ix == NXP - 1 && A.dir_hi[0]) dd = 0, sd = 1;
     if (NY > 1 && iy == 0 && A.dir_lo[1]) dd = 1, sd = 0;
-    if (NY > 1 && iy == NY - 1 && A.dir_hi[1]) dd = 1, sd = 1;
+    if (NY > 1 && iy == NYP - 1 && A.dir_hi[1]) dd = 1, sd = 1;
     return dd < 0 || !A.has_patch ? C{R(0), R(0)} : bvalue(lev, dd, sd, dd == 0 ? iy : ix);
   };
   // (only dense-transform lattices can have Dirichlet ends)
@@ -631,8 +671,9 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
         const int traj = group * TPB + p;
         if (traj < A.E) {
           XSH_FOR(v, 0, 2)
-            if (s + v < NSP) {
-              const long long g = (((long long)pidx * A.npts + n) * A.E + traj) * NSP + s + v;
+            const int ps = phys(s + v);
+            if (ps >= 0) {
+              const long long g = (((long long)pidx * A.npts + n) * A.E + traj) * NSP + ps;
               A.raw[2 * g] = (double)f0[2 * i + v].x;
               A.raw[2 * g + 1] = (double)f0[2 * i + v].y;
             }
@@ -666,7 +707,8 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
                 re = (double)a.x * a.x + (double)a.y * a.y;
                 im = 0.0;
               }
-              if (s + v < NSP) atomic_add_c(dst + 2 * ((long long)n * NSP + s + v), re * A.inv_E, im * A.inv_E);
+              const int ps = phys(s + v);
+              if (ps >= 0) atomic_add_c(dst + 2 * ((long long)n * NSP + ps), re * A.inv_E, im * A.inv_E);
             XSH_END
           }
         XSH_END
@@ -681,7 +723,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
           const int p = pt / NS, s = pt % NS;
           if (group * TPB + p < A.E) {
             XSH_FOR(v, 0, 2)
-              const C a = s + v < NSP ? f0[2 * i + v] : C{R(0), R(0)};
+              const C a = phys(s + v) >= 0 ? f0[2 * i + v] : C{R(0), R(0)};
               const double m = (double)a.x * a.x + (double)a.y * a.y;
               if (kind == XSH_OBS_LOWK) {
                 if (s + v == 0) re += (double)f1[2 * i + v].x;
@@ -735,8 +777,9 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
       const int traj = group * TPB + p;
       if (traj < A.E) {
         XSH_FOR(v, 0, 2)
-          if (s + v < NSP) {
-            const long long g = (long long)traj * NSP + s + v;
+          const int ps = phys(s + v);
+          if (ps >= 0) {
+            const long long g = (long long)traj * NSP + ps;
             A.a_out[2 * g] = (double)f0[2 * i + v].x;
             A.a_out[2 * g + 1] = (double)f0[2 * i + v].y;
           }

@@ -36,6 +36,9 @@ const KernelEntry* find_stream_kernel(int precision, int nx, int ny);
 // 1D dense-transform kernel of the smallest compiled size >= nx (the lattice is
 // padded to that size: the padding sites take no part in the physical ones)
 const KernelEntry* find_padded_kernel(int precision, int nx);
+// 2D dense-transform kernel of the smallest square compiled size >= nx, ny among
+// those that handle padding (up to 64 x 64)
+const KernelEntry* find_padded_kernel2d(int precision, int nx, int ny);
 hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s);
 hipError_t launch_probe_normals(int precision, uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t inst,
                                 uint32_t draw, int traj0, int ntraj, int count, double* out, hipStream_t s);
