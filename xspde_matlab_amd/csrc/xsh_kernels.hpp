@@ -363,10 +363,10 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
       const int site = s + v, iy = site / NX, ix = site % NX;
       const int ps = phys(site);
       if (A.a0) {
-        const long long g = (long long)(traj < A.E ? traj : 0) * NSP + (ps >= 0 ? ps : 0);
-        f0[2 * i + v] = ps >= 0 ? C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]} : C{R(0), R(0)};
+        const long long g = (long long)(traj < A.E ? traj : 0) * NSP + (!MAT || ps >= 0 ? ps : 0);
+        f0[2 * i + v] = !MAT || ps >= 0 ? C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]} : C{R(0), R(0)};
       } else {
-        f0[2 * i + v] = initial_value<R>(A, nsp, (uint32_t)traj, ps >= 0 ? ps : 0, ix, iy, f1[2 * i + v]);
+        f0[2 * i + v] = initial_value<R>(A, nsp, (uint32_t)traj, !MAT || ps >= 0 ? ps : 0, ix, iy, f1[2 * i + v]);
       }
       __builtin_amdgcn_sched_barrier(0);
     XSH_END
@@ -672,7 +672,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
         if (traj < A.E) {
           XSH_FOR(v, 0, 2)
             const int ps = phys(s + v);
-            if (ps >= 0) {
+            if (!MAT || ps >= 0) {
               const long long g = (((long long)pidx * A.npts + n) * A.E + traj) * NSP + ps;
               A.raw[2 * g] = (double)f0[2 * i + v].x;
               A.raw[2 * g + 1] = (double)f0[2 * i + v].y;
@@ -708,7 +708,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
                 im = 0.0;
               }
               const int ps = phys(s + v);
-              if (ps >= 0) atomic_add_c(dst + 2 * ((long long)n * NSP + ps), re * A.inv_E, im * A.inv_E);
+              if (!MAT || ps >= 0) atomic_add_c(dst + 2 * ((long long)n * NSP + ps), re * A.inv_E, im * A.inv_E);
             XSH_END
           }
         XSH_END
@@ -723,7 +723,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
           const int p = pt / NS, s = pt % NS;
           if (group * TPB + p < A.E) {
             XSH_FOR(v, 0, 2)
-              const C a = phys(s + v) >= 0 ? f0[2 * i + v] : C{R(0), R(0)};
+              const C a = !MAT || phys(s + v) >= 0 ? f0[2 * i + v] : C{R(0), R(0)};
               const double m = (double)a.x * a.x + (double)a.y * a.y;
               if (kind == XSH_OBS_LOWK) {
                 if (s + v == 0) re += (double)f1[2 * i + v].x;
@@ -778,7 +778,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
       if (traj < A.E) {
         XSH_FOR(v, 0, 2)
           const int ps = phys(s + v);
-          if (ps >= 0) {
+          if (!MAT || ps >= 0) {
             const long long g = (long long)traj * NSP + ps;
             A.a_out[2 * g] = (double)f0[2 * i + v].x;
             A.a_out[2 * g + 1] = (double)f0[2 * i + v].y;
