@@ -1,0 +1,19 @@
+// Fp64 fft kernels with the k-space filtered noise code (knoises, krandoms):
+// looked up only for problems that use it.
+#include "xsh_kernels.hpp"
+
+namespace xsh {
+
+static const KernelEntry kEntries[] = {
+    XSH_ENTRY(double, 32, 1, 256, 16), XSH_ENTRY(double, 64, 1, 256, 16),
+    XSH_ENTRY(double, 128, 1, 256, 16), XSH_ENTRY(double, 256, 1, 256, 8),
+    XSH_ENTRY(double, 512, 1, 256, 4), XSH_ENTRY(double, 1024, 1, 256, 2),
+    XSH_ENTRY(double, 32, 32, 256, 1), XSH_ENTRY(double, 64, 64, 512, 1),
+};
+
+const KernelEntry* table_kn_f64(int* n) {
+  *n = (int)(sizeof(kEntries) / sizeof(kEntries[0]));
+  return kEntries;
+}
+
+}  // namespace xsh
