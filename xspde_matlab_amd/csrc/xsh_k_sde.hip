@@ -13,6 +13,7 @@ const KernelEntry* table_f32_var2d(int* n);
 const KernelEntry* table_mat_f64_1d256(int* n);
 const KernelEntry* table_kn_f32(int* n);
 const KernelEntry* table_kn_f64(int* n);
+const KernelEntry* table_kn_f32_2d(int* n);
 const KernelEntry* table_mat_f64(int* n);
 const KernelEntry* table_f32_var(int* n);
 
@@ -20,7 +21,8 @@ int kernel_variants(int precision, int nx, int ny, int mat, const KernelEntry** 
   const KernelEntry* (*tables[])(int*) = {table_f32_1d,     table_f32_2d,        table_f64,
                                           table_mat_f32,    table_mat_f32_1d,    table_mat_f32_1d256,
                                           table_mat_f64,    table_mat_f64_1d256, table_f32_var,
-                                          table_f32_var2d,  table_kn_f32,        table_kn_f64};
+                                          table_f32_var2d,  table_kn_f32,        table_kn_f64,
+                                          table_kn_f32_2d};
   int k = 0;
   for (auto tab : tables) {
     int n = 0;
@@ -35,7 +37,7 @@ const KernelEntry* find_kernel(int precision, int nx, int ny, int mat, int kn) {
   const KernelEntry* (*tables[])(int*) = {table_f32_1d,     table_f32_2d,        table_f64,
                                           table_mat_f32,    table_mat_f32_1d,    table_mat_f32_1d256,
                                           table_mat_f64,    table_mat_f64_1d256, table_kn_f32,
-                                          table_kn_f64};
+                                          table_kn_f64,     table_kn_f32_2d};
   for (auto tab : tables) {
     int n = 0;
     const KernelEntry* e = tab(&n);
