@@ -567,6 +567,32 @@ def test_peregrine_known_answer():
     assert 4.588574e-04 / 1.001 < err[0] < 4.588574e-04 * 1.001
 
 
+def test_unsupported_combinations_rejected():
+    """Combinations outside the GPU path fail at creation with XSH_EUNSUPPORTED
+    (the host then keeps the project on the CPU path), before any launch."""
+    from xspde_matlab_amd._lib import XshError
+    from xspde_matlab_amd.xsim import BatchProblem
+    cases = []
+    p = shrink(nls1d("fp64", nx=101), E=4, e2=1, points=2, steps=1)
+    p["boundaries"] = [[-1, -1]]
+    p["gpu"]["observe"] = ["corr"]                  # periodic FFT of a padded lattice
+    cases.append(p)
+    p = shrink(cgl2d("fp64", n=128), E=2, e2=1, points=2, steps=1)
+    p.update(noises=0, knoises=2, nfilter=_kfilter_n)    # k-space noise on the streaming kernel
+    cases.append(p)
+    p = shrink(cgl2d("fp32", n=64), E=2, e2=1, points=2, steps=1)
+    p["boundaries"] = [[-1, -1], [-1, -1]]
+    p.update(noises=0, knoises=2, nfilter=_kfilter_n)    # k-space noise on a dense-transform lattice
+    cases.append(p)
+    for p in cases:
+        with pytest.raises(XshError, match="XSH_EUNSUPPORTED"):
+            Problem(p)
+    p = shrink(cgl2d("fp32", n=32), E=2, e2=1, points=2, steps=1)
+    p.update(noises=0, knoises=1, nfilter=_kfilter_n)
+    with pytest.raises(XshError, match="XSH_EUNSUPPORTED"):
+        BatchProblem([p])                           # k-space noise in a batch
+
+
 def _oracle_snapshots(q, j, nc):
     """The oracle path's fields at every stored point (xpath.m:73-78)."""
     snaps = []
