@@ -567,6 +567,22 @@ def test_peregrine_known_answer():
     assert 4.588574e-04 / 1.001 < err[0] < 4.588574e-04 * 1.001
 
 
+@pytest.mark.parametrize("bnd,method", [(("NN", "NN"), "RK4"), (("NN", "PP"), "MP"), (("PP", "NN"), "RK4")])
+def test_stream_dense_fp64_128(bnd, method):
+    """FP64 128x128 with Neumann ends (config 3's Neumann run in FP64): the
+    fields stay in the streaming kernel's workspace and the dense DCT-I (and,
+    on mixed lattices, DFT) passes run on row halves and column halves in LDS;
+    against the oracle, fine and coarse paths."""
+    p = shrink(cgl2d("fp64", n=128), E=2, e2=2, points=3, steps=2)
+    p["method"] = method
+    p["boundaries"] = [BOUNDS[b] for b in bnd]
+    p["gpu"]["observe"] = ["abs2_ave", "abs2", "a_ave", "corr"]
+    prob = Problem(p)
+    assert prob.info.kernel_kind == 1
+    prob.close()
+    _path_case(p, 1e-10)
+
+
 def test_unsupported_combinations_rejected():
     """Combinations outside the GPU path fail at creation with XSH_EUNSUPPORTED
     (the host then keeps the project on the CPU path), before any launch."""
@@ -583,6 +599,9 @@ def test_unsupported_combinations_rejected():
     p = shrink(cgl2d("fp32", n=64), E=2, e2=1, points=2, steps=1)
     p["boundaries"] = [[-1, -1], [-1, -1]]
     p.update(noises=0, knoises=2, nfilter=_kfilter_n)    # k-space noise on a dense-transform lattice
+    cases.append(p)
+    p = shrink(cgl2d("fp64", n=128), E=2, e2=1, points=2, steps=1)
+    p["boundaries"] = [[-1, -1], [1, 1]]                 # Dirichlet ends on the streaming kernel
     cases.append(p)
     for p in cases:
         with pytest.raises(XshError, match="XSH_EUNSUPPORTED"):

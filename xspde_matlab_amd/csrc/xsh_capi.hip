@@ -286,10 +286,29 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
       if (h->btype[k] != 0) h->mat = 1;
     }
     h->kern = xsh::find_kernel(d.precision, h->nx, h->ny, h->mat, d.knoises || d.krandoms);
-    if (!h->kern && !h->mat) h->kern = xsh::find_stream_kernel(d.precision, h->nx, h->ny);
+    if (!h->kern && !h->mat) h->kern = xsh::find_stream_kernel(d.precision, h->nx, h->ny, 0);
     if (!h->kern && !h->mat) {  // periodic lattice without an FFT kernel: dense DFT
       h->mat = 1;
       h->kern = xsh::find_kernel(d.precision, h->nx, h->ny, 1);
+    }
+    if (!h->kern && h->mat) {
+      // fields too large for the chip: dense transforms on the streaming kernel,
+      // which has no boundary patches or Dirichlet ends (homogeneous Neumann and
+      // periodic dimensions)
+      h->kern = xsh::find_stream_kernel(d.precision, h->nx, h->ny, 1);
+      if (h->kern) {
+        bool values = d.boundtab != nullptr, dirichlet = false;
+        for (int k = 0; k < d.dimensions - 1; ++k)
+          for (int side = 0; side < 2; ++side) {
+            values = values || d.boundval[k][side][0] != 0.0 || d.boundval[k][side][1] != 0.0;
+            dirichlet = dirichlet || d.boundaries[k][side] == 1;
+          }
+        if (values || dirichlet) {
+          delete h;
+          return fail(XSH_EUNSUPPORTED, "the streaming %dx%d dense-transform kernel takes homogeneous Neumann and "
+                                        "periodic dimensions (no Dirichlet ends or boundary values)", npt[0], npt[1]);
+        }
+      }
     }
     if (!h->kern && h->mat && d.dimensions == 2) h->kern = xsh::find_padded_kernel(d.precision, h->nx);
     if (!h->kern && h->mat && d.dimensions == 3) h->kern = xsh::find_padded_kernel2d(d.precision, h->nx, h->ny);
@@ -665,7 +684,7 @@ extern "C" int xsh_query(const xsh_handle* h, xsh_info* info) {
   info->traj_steps = (long long)h->E * h->e2 * per * (h->d.checks ? 3 : 1);
   info->threads = h->threads;
   info->traj_per_block = h->tpb;
-  info->kernel_kind = h->nd == 1 ? 3 : h->mat ? 2 : (h->kern && h->kern->stream) ? 1 : 0;
+  info->kernel_kind = h->nd == 1 ? 3 : (h->kern && h->kern->stream) ? 1 : h->mat ? 2 : 0;
   return XSH_OK;
 }
 

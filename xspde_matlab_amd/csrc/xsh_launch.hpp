@@ -32,7 +32,7 @@ constexpr int kStreamMaxBlocks = 1024;
 const KernelEntry* find_kernel(int precision, int nx, int ny, int mat, int kn = 0);
 // all compiled variants (threads, trajectories per workgroup, radix) of a lattice kernel
 int kernel_variants(int precision, int nx, int ny, int mat, const KernelEntry** out, int max);
-const KernelEntry* find_stream_kernel(int precision, int nx, int ny);
+const KernelEntry* find_stream_kernel(int precision, int nx, int ny, int mat);
 // 1D dense-transform kernel of the smallest compiled size >= nx (the lattice is
 // padded to that size: the padding sites take no part in the physical ones)
 const KernelEntry* find_padded_kernel(int precision, int nx);

@@ -28,9 +28,11 @@ struct Stream {
   static constexpr int PAIRS = NS / (2 * NT); // 2-site chunks per thread (plane)
   static constexpr int HPAIRS = NSH / (2 * NT);
   static_assert(NY % 2 == 0 && NSH % (2 * NT) == 0, "half-planes must tile the threads");
-  using Lh = Lattice<R, NX, H, NT, 1, 0>;     // LDS geometry of a half-plane
+  using Lh = Lattice<R, NX, H, NT, 1, 0>;     // LDS geometry of a half-plane (rows)
+  using Lv = Lattice<R, NX / 2, NY, NT, 1, 0>; // LDS geometry of a column half (dense y passes)
   using C = cpx<R>;
   static constexpr int RS = Lh::RS;
+  static constexpr int LDS = Lh::LDS > Lv::LDS ? Lh::LDS : Lv::LDS;
 
   // first site of 2-site chunk i of a region of the plane (consecutive threads,
   // consecutive chunks: coalesced)
@@ -52,6 +54,69 @@ struct Stream {
       F[s] = L[y * RS + x];
       F[s + 1] = L[y * RS + x + 1];
     }
+  }
+
+  // columns [c NX/2, (c+1) NX/2) of the plane, every row, to and from LDS
+  __device__ static __forceinline__ void cols_to_lds(C* L, const C* F, int c) {
+#pragma unroll 4
+    for (int i = 0; i < HPAIRS; ++i) {
+      const int sl = site(i), y = sl / (NX / 2), xl = sl - y * (NX / 2);
+      const int g = y * NX + c * (NX / 2) + xl;
+      L[y * Lv::RS + xl] = F[g];
+      L[y * Lv::RS + xl + 1] = F[g + 1];
+    }
+  }
+  __device__ static __forceinline__ void cols_from_lds(const C* L, C* F, int c) {
+#pragma unroll 4
+    for (int i = 0; i < HPAIRS; ++i) {
+      const int sl = site(i), y = sl / (NX / 2), xl = sl - y * (NX / 2);
+      const int g = y * NX + c * (NX / 2) + xl;
+      F[g] = L[y * Lv::RS + xl];
+      F[g + 1] = L[y * Lv::RS + xl + 1];
+    }
+  }
+
+  // Dense-transform lattices (non-periodic boundaries, xtransform.m:208-245):
+  // F -> Tx^-1 Ty^-1 op(Ty Tx F) in place, the x passes on row halves and the y
+  // passes on column halves in LDS (each half fits), matrices Tm[0..3] =
+  // forward x, forward y, inverse x, inverse y, complex for periodic dimensions
+  // of a mixed lattice.  op(X, kx, ky) in the transforms' natural k order.
+  template <class Op>
+  __device__ static __forceinline__ void propagate_mat(C* L, C* F, const void* const* Tm, int cplx_x,
+                                                        int cplx_y, const Op& op) {
+    auto id = [](C X, int, int) __attribute__((always_inline)) { return X; };
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      __syncthreads();
+      to_lds(L, F + h * NSH);
+      __syncthreads();
+      Lh::template mat_any<0>(L, Tm[0], nullptr, cplx_x, id);
+      __syncthreads();
+      from_lds(L, F + h * NSH);
+    }
+#pragma unroll 1
+    for (int c = 0; c < 2; ++c) {
+      __syncthreads();
+      cols_to_lds(L, F, c);
+      __syncthreads();
+      Lv::template mat_any<1>(L, Tm[1], nullptr, cplx_y, [&](C X, int l, int k) __attribute__((always_inline)) {
+        return op(X, c * (NX / 2) + l, k);
+      });
+      __syncthreads();
+      Lv::template mat_any<1>(L, Tm[3], nullptr, cplx_y, id);
+      __syncthreads();
+      cols_from_lds(L, F, c);
+    }
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      __syncthreads();
+      to_lds(L, F + h * NSH);
+      __syncthreads();
+      Lh::template mat_any<0>(L, Tm[2], nullptr, cplx_x, id);
+      __syncthreads();
+      from_lds(L, F + h * NSH);
+    }
+    __syncthreads();
   }
 
   // F -> T^-1 op(T F) in place (F one plane in device memory).  twx: W_NX^m,
@@ -142,13 +207,14 @@ struct Stream {
 
 // Work items and instances as in spde_path; the workspace of instance i has
 // a region for every workgroup of the grid (A.work, A.work_planes planes each).
-template <class R, int NX, int NY, int NT>
+// MAT: dense-transform propagator (homogeneous Neumann and periodic dimensions).
+template <class R, int NX, int NY, int NT, bool MAT>
 __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ args, const int* __restrict__ wstart,
                                                   int ninst, int nwork) {
   using S = Stream<R, NX, NY, NT>;
   using C = cpx<R>;
   constexpr int NS = S::NS, PAIRS = S::PAIRS;
-  __shared__ C L[S::Lh::LDS];
+  __shared__ C L[S::LDS];
   __shared__ double red[2 * (NT / 64)];
 
 #pragma unroll 1
@@ -198,7 +264,12 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
       }
     };
     auto propagate = [&](C* F) __attribute__((always_inline)) {
-      if (A.has_linear) S::propagate(L, F, twx, twy, twh, prop);
+      if (A.has_linear) {
+        if constexpr (MAT)
+          S::propagate_mat(L, F, A.tm, A.cplx[0], A.cplx[1], prop);
+        else
+          S::propagate(L, F, twx, twy, twh, prop);
+      }
     };
 
     auto method_step = [&](uint32_t q, bool coarse) __attribute__((always_inline)) {
@@ -377,36 +448,42 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
 }
 
 // propagation probe of the streaming transform: planes [nplanes][NS] in place
-template <class R, int NX, int NY, int NT>
+template <class R, int NX, int NY, int NT, bool MAT>
 __global__ __launch_bounds__(NT) void probe_stream(const PathArgs* __restrict__ Ap, int nci, int nplanes,
                                                    cpx<R>* planes) {
   const PathArgs& A = *Ap;
   using S = Stream<R, NX, NY, NT>;
   using C = cpx<R>;
-  __shared__ C L[S::Lh::LDS];
+  __shared__ C L[S::LDS];
   if ((int)blockIdx.x >= nplanes) return;
   const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
                        static_cast<const C*>(nci ? A.py[1] : A.py[0])};
-  S::propagate(L, planes + (long long)blockIdx.x * S::NS, static_cast<const C*>(A.twx),
-               static_cast<const C*>(A.twy), static_cast<const C*>(A.twh), prop);
+  if constexpr (MAT)
+    S::propagate_mat(L, planes + (long long)blockIdx.x * S::NS, A.tm, A.cplx[0], A.cplx[1], prop);
+  else
+    S::propagate(L, planes + (long long)blockIdx.x * S::NS, static_cast<const C*>(A.twx),
+                 static_cast<const C*>(A.twy), static_cast<const C*>(A.twh), prop);
 }
 
-template <class R, int NX, int NY, int NT>
+template <class R, int NX, int NY, int NT, bool MAT>
 static hipError_t launch_stream(const PathArgs* args, const int* wstart, int ninst, int nwork, int grid,
                                 hipStream_t s) {
-  hipLaunchKernelGGL((spde_stream<R, NX, NY, NT>), dim3(grid), dim3(NT), 0, s, args, wstart, ninst, nwork);
+  hipLaunchKernelGGL((spde_stream<R, NX, NY, NT, MAT>), dim3(grid), dim3(NT), 0, s, args, wstart, ninst, nwork);
   return hipGetLastError();
 }
 
-template <class R, int NX, int NY, int NT>
+template <class R, int NX, int NY, int NT, bool MAT>
 static hipError_t launch_probe_stream(const PathArgs* A, int nci, int nplanes, void* planes, hipStream_t s) {
-  hipLaunchKernelGGL((probe_stream<R, NX, NY, NT>), dim3(nplanes), dim3(NT), 0, s, A, nci, nplanes,
+  hipLaunchKernelGGL((probe_stream<R, NX, NY, NT, MAT>), dim3(nplanes), dim3(NT), 0, s, A, nci, nplanes,
                      static_cast<cpx<R>*>(planes));
   return hipGetLastError();
 }
 
-#define XSH_STREAM_ENTRY(R, NX, NY, NT)                                                                 \
-  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, 1, 0, &launch_stream<R, NX, NY, NT>,              \
-   &launch_probe_stream<R, NX, NY, NT>, 1, Split<NX>::N1, 0}
+#define XSH_STREAM_ENTRY(R, NX, NY, NT)                                                                        \
+  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, 1, 0, &launch_stream<R, NX, NY, NT, false>,              \
+   &launch_probe_stream<R, NX, NY, NT, false>, 1, Split<NX>::N1, 0}
+#define XSH_STREAM_MAT_ENTRY(R, NX, NY, NT)                                                                   \
+  {sizeof(R) == 8 ? XSH_FP64 : XSH_FP32, NX, NY, NT, 1, 1, &launch_stream<R, NX, NY, NT, true>,              \
+   &launch_probe_stream<R, NX, NY, NT, true>, 1, 0, 0}
 
 }  // namespace xsh
