@@ -5,7 +5,7 @@
 namespace xsh {
 
 static const KernelEntry kEntries[] = {
-    XSH_STREAM_MAT_ENTRY(double, 128, 128, 512),
+    XSH_STREAM_MAT_ENTRY(double, 128, 128, 1024),
 };
 
 const KernelEntry* table_stream_mat(int* n) {
