@@ -1,5 +1,5 @@
 """FP32 against FP64 on the GPU, per observable, on the same seeded streams
-(the north-star's precision statement): 1D stochastic NLS at its full time
+(the precision statement of the FP32 path): 1D stochastic NLS at its full time
 span (1000 coarse RK4IP steps with checks) and 2D CGL at its full span
 (2000 coarse steps), fewer trajectories.  The FP32 error of each ensemble
 mean is stated relative to its FP64 value and to the sampling error bar."""
