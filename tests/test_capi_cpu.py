@@ -162,3 +162,23 @@ def test_boundary_table_lines_2d():
     assert np.allclose(tab[3, 1, 1], np.sin(x))
     t3 = q["origins"][0] + 3 * q["dt"] / 2 / 2
     assert np.allclose(tab[3, 1, 0], np.arange(32) * t3)
+
+
+def test_filter_table_site_order_and_linearity():
+    """k-space filters (p.nfilter, p.rfilter) become multiplier tables over the
+    FFT-order k sites kx + nx*ky; a filter that is not a site-wise
+    multiplication has no table and the project stays on the CPU path."""
+    from xspde_matlab_amd.xsim import filter_table
+    p = configs.shrink(configs.cgl2d(n=32), E=2, e2=1, points=2, steps=1)
+    p.update(noises=0, knoises=2, nfilter=lambda w, p: w * (p.kx + 2.0 * p.ky + 0.5j))
+    q = preferences(p)
+    F = filter_table(q, "nfilter", 2)
+    dk = 2 * np.pi / (32 * q["ranges"][1] / 31)
+    k = np.fft.ifftshift((np.arange(32) - 16) * dk)
+    kx, ky = np.meshgrid(k, k, indexing="xy")          # [ky, kx] -> site kx + 32 ky
+    assert F.shape == (2, 1024)
+    assert np.allclose(F[0], (kx + 2.0 * ky + 0.5j).reshape(-1)) and np.allclose(F[1], F[0])
+    p["nfilter"] = lambda w, p: w ** 2
+    with pytest.raises(GpuFormError):
+        filter_table(preferences(p), "nfilter", 2)
+    assert filter_table(preferences(dict(p, nfilter=None)), "nfilter", 1).shape == (1, 1024)
