@@ -9,10 +9,15 @@ Workload (one "step" = one full ensemble pass of the hot path):
   6000 trajectory-steps each.  FP32 fields, FP64 moments.  Synthetic input:
   random-phase initial fields and counter-based noise, generated on device.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fp64]
                        [--precision fp32|fp64] [--boundaries periodic|neumann]
        python bench.py --workload tw     (control variate vs plain ensemble)
-Prints one JSON line (rank 0, N=1).
+       python bench.py --workload copy   (256 MiB device copies: the reference
+                                          kernel that scales rocprofv3's
+                                          FETCH_SIZE / WRITE_SIZE to bytes)
+Prints one JSON line (rank 0, N=1).  The default FP32 line also carries the
+FP64 rate of the same workload (the other half of the metric) in "fp64",
+measured after the timed region (1 warmup + 2 launches).
 
 --workload tw measures the second half of the headline metric, the wall time
 to a fixed error bar: on-site g2 of a truncated-Wigner 2D Bose lattice
@@ -108,12 +113,15 @@ def main():
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--boundaries", default="periodic", choices=["periodic", "neumann"],
                     help="periodic (FFT) or Neumann-Neumann (DCT-I) lattice")
-    ap.add_argument("--workload", default="cgl2d", choices=["cgl2d", "nls1d", "tw", "batch"])
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion measurement")
+    ap.add_argument("--workload", default="cgl2d", choices=["cgl2d", "nls1d", "tw", "batch", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs a single MI355X (--gpus %d not supported)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copy":
+        return copy_calibration()
     if args.workload == "tw":
         return bench_tw(args)
     if args.workload == "batch":
@@ -160,6 +168,9 @@ def main():
     import numpy as np
     finite = bool(np.isfinite(av).all())
     prob.close()
+    fp64 = None
+    if args.workload == "cgl2d" and args.precision == "fp32" and not args.no_fp64:
+        fp64 = fp64_companion(args.boundaries)
     cpu = None
     if not args.no_cpu:
         log("bench: timing the CPU oracle sample")
@@ -193,7 +204,52 @@ def main():
                               "flops_per_traj_step": step_flops(ns)}},
         "cpu_baseline": cpu,
     }
+    if fp64 is not None:
+        fp64["fp32_over_fp64"] = rate / fp64["value"]
+        line["fp64"] = fp64
     print(json.dumps(line), flush=True)
+
+
+def fp64_companion(boundaries, warmup=1, launches=2):
+    """The same 128x128 workload with FP64 fields: whole-ensemble launches timed
+    with HIP events, after the FP32 timed region (the FP64 half of the metric)."""
+    from xspde_matlab_amd import Problem
+    from xspde_matlab_amd.configs import cgl2d
+    p = cgl2d("fp64")
+    if boundaries == "neumann":
+        p["boundaries"] = [[-1, -1], [-1, -1]]
+    prob = Problem(p)
+    info = prob.info
+    for w in range(warmup):
+        prob.handle.run_ensemble(1)
+        log("bench fp64: warmup %d done (%.1f ms kernel)" % (w, prob.handle.synchronize()))
+    kms = []
+    t0 = time.perf_counter()
+    for s in range(launches):
+        prob.handle.run_ensemble(1)
+        kms.append(prob.handle.synchronize())
+        log("bench fp64: launch %d kernel %.1f ms" % (s, kms[-1]))
+    wall = (time.perf_counter() - t0) / launches
+    prob.close()
+    kernel = ("spde_path", "spde_stream", "spde_path", "sde_path")[info.kernel_kind]
+    kavg = sum(kms) / len(kms)
+    return {"value": info.traj_steps / wall, "unit": "trajectory-steps/s", "dtype": "f64", "launches": launches,
+            "warmup": warmup, "ms_per_step": 1e3 * wall, "kernel": kernel, "kernel_ms": kavg,
+            "hbm_frac": info.traj_steps * 2 * info.nsites * 16 / (kavg / 1e3) / HBM_PEAK}
+
+
+def copy_calibration(n=64 * 1024 * 1024, reps=3):
+    """Three plain device copies of 256 MiB (n float32), the kernel of known
+    traffic (262,144 KiB read + 262,144 KiB written per copy) whose FETCH_SIZE and
+    WRITE_SIZE readings scale the counters of the path kernels to bytes."""
+    import torch
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n,
+                      "copies": reps}), flush=True)
 
 
 # attractive lattice whose modulational-instability band (hop k^2 < 2 |g| n0)

@@ -19,8 +19,14 @@
  * Conventions: plain pointers and sizes, no exceptions across the ABI; every
  * entry returns an xsh_status and leaves a message for xsh_last_error()
  * (thread-local), the analogue of the reference's error() aborts
- * (e.g. xtransform.m:225-226, spdpreferences.m:67-68).  Host buffers are
- * MATLAB column-major, interleaved complex double (mxGetComplexDoubles).
+ * (e.g. xtransform.m:51, spdpreferences.m:67-68).  Host buffers are
+ * MATLAB column-major, interleaved complex double (mxGetComplexDoubles):
+ *   fields    a0, a_out   [1, nx, ny, ensembles(1)]   (p.d.ca, xpreferences.m:145)
+ *   averages  per observable o, at offset obs_offset[o] of a path's [avlen]
+ *             block: [1, points, sites_o] with sites_o = nx*ny (x before y) for
+ *             per-site forms, 1 for lattice means (p.d.av{o}, xpreferences.m:340),
+ *             i.e. element (n, s) at obs_offset[o] + n + points*s: time fastest
+ *   raw       per path [1, points, nx, ny, ensembles(1)]  (p.d.raw, xpreferences.m:148)
  */
 #ifndef XSPDE_HIP_H
 #define XSPDE_HIP_H
@@ -94,7 +100,7 @@ typedef struct {
   double lin_l0[2];     /* L = l0 + sum_d l1[d] D_d + l2[d] D_d^2 (complex)  */
   double lin_l1[2][2];
   double lin_l2[2][2];
-  int has_linear;       /* 0: propagator is 1 (xpropfactor.m:144,154)        */
+  int has_linear;       /* 0: propagator is 1 (xpropfactor.m:27,37)        */
   int initial;          /* XSH_INIT_*                                        */
   double init_par[6];   /* CONST: c.re c.im; SECH: amp width eps; GAUSS: amp width;
                            PHASE: amp; COHERENT: alpha.re alpha.im            */
@@ -155,7 +161,7 @@ const char* xsh_last_error(void);
 
 /* Validate a descriptor, derive the lattice (xpreferences.m:133-175,
  * spdpreferences.m:16-72), build propagator and twiddle tables
- * (xpropfactor.m:143-175) and allocate device buffers.
+ * (xpropfactor.m:23-58) and allocate device buffers.
  * Replaces the per-(j,nc) setup of xensemble.m:41-53. */
 int xsh_create(const xsh_desc* desc, xsh_handle** out);
 
@@ -170,7 +176,8 @@ int xsh_query(const xsh_handle* h, xsh_info* info);
  * (1 fine, 2 coarse) of parallel ensemble npar (1-based), from a0
  * ([1, nx, ny, ensembles(1)] complex, or NULL to use the built-in initial
  * form, xensemble.m:53-60), and writes the final fields a_out (same shape,
- * may be NULL) and the averages av_out ([avlen] complex, xpath.m:79-83). */
+ * may be NULL) and the averages av_out ([avlen] complex, the p.d.av{o} blocks of
+ * xpath.m:79-83 one after another, each time fastest). */
 int xsh_run_path(xsh_handle* h, int npar, int j, int nc, const double* a0, double* a_out,
                  double* av_out);
 
@@ -182,8 +189,8 @@ int xsh_run_ensemble(xsh_handle* h, int npar);
 /* Wait for the handle's stream; returns the last kernel time in *ms if non-NULL. */
 int xsh_synchronize(xsh_handle* h, double* ms);
 
-/* Copy the averages of the last xsh_run_ensemble:
- * [ensembles(2)][nchk][avlen] complex (interleaved doubles). */
+/* Copy the averages of the last xsh_run_ensemble: one [avlen] block per path,
+ * path (j, nc) at block (j-1)*nchk + nc-1, each laid out as xsh_run_path's av_out. */
 int xsh_read_averages(xsh_handle* h, double* av_out);
 
 /* Per-trajectory data of the last xsh_run_ensemble (no reference counterpart;
@@ -195,8 +202,9 @@ int xsh_read_averages(xsh_handle* h, double* av_out);
 int xsh_read_trajectories(xsh_handle* h, double* obs, float* feat);
 
 /* Raw trajectories of the last xsh_run_ensemble (p.raw, xpath.m:73-78, stored by
- * the reference in raw{j}{nc,np}): [ensembles(2)][nchk][points][ensembles(1)][nx*ny]
- * complex (interleaved doubles, x fastest); the descriptor must set raw = 1. */
+ * the reference in raw{j}{nc,np}): one block per path in the order of
+ * xsh_read_averages, each p.d.raw = [1, points, nx, ny, ensembles(1)] column-major
+ * (time fastest, then x, y, trajectory); the descriptor must set raw = 1. */
 int xsh_read_raw(xsh_handle* h, double* out);
 
 /* Test hooks (no reference counterpart; used by the parity tests). */

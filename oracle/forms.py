@@ -15,7 +15,7 @@ deriv    'zero'                        D = 0
          'kubo'   c                    D = i c w0 a                       (Kubo.m:17)
          'gp'     g, u, s              D = g |a|^2 a + u a + s (w0 + i w1)   [noises >= 2]
                                        D = g |a|^2 a + u a + s w0            [noises == 1]
-linear   l0, l1[d], l2[d]              L = l0 + sum_d l1[d] D_d + l2[d] D_d^2  (xpropfactor.m:150-156)
+linear   l0, l1[d], l2[d]              L = l0 + sum_d l1[d] D_d + l2[d] D_d^2  (xpropfactor.m:37-39)
 initial  'const'  c0                   a = c0
          'sech'   amp, width, eps      a = amp sech(x/width) + eps (v0 + i v1)/sqrt2
          'gauss'  amp, width           a = amp exp(-(x^2+y^2)/(2 width^2))
@@ -111,7 +111,7 @@ def make_initial(g):
 
         def initial(*args):
             p = args[-1]
-            u = args[-2] * p.dv                                  # uniform(0,1): xrandom.m:199 scales by 1/dv
+            u = args[-2] * p.dv                                  # uniform(0,1): xrandom.m:32 scales by 1/dv
             return amp * np.exp(2j * np.pi * u[0:1])
         return initial
     if ini == "coherent":

@@ -14,11 +14,11 @@ function              reference                                     pinned by
 xpreferences          Simulator/xpreferences.m:42-219               Batchtest rows (below)
                       Applications/SPDE/spdpreferences.m:16-72
                       Applications/SPDE/xgrid.m:18-48
-xpropfactor           Applications/SPDE/xpropfactor.m:140-175       linear-only exactness,
+xpropfactor           Applications/SPDE/xpropfactor.m:23-58       linear-only exactness,
                                                                     Batchtest rows
-xprop / xtransform    xprop.m:88-117, xtransform.m:196-245          transforms brute force,
+xprop / xtransform    xprop.m:17-46, xtransform.m:21-70          transforms brute force,
                                                                     Batchtest rows
-xsetbound             xsetbound.m:92-121 (boundval constants or     Batchtest #21 (NN),
+xsetbound             xsetbound.m:30-59 (boundval constants or     Batchtest #21 (NN),
                       boundfun(a, c, d, p) at p.t)                  #42 (Peregrine), heat
                                                                     closed forms
 xpatch, da            xpatch.m:24-62, xpropfactor.m:40-51           inhomogeneous heat
@@ -27,7 +27,7 @@ xpatch, da            xpatch.m:24-62, xpropfactor.m:40-51           inhomogeneou
                                                                     (time-dependent values)
 RK4                   Applications/SDE/RK4.m:7-42                   Batchtest #4, #8, #14,
                                                                     #16, #21
-MP                    Applications/SDE/MP.m:53-70                   order test (dt vs dt/2),
+MP                    Applications/SDE/MP.m:10-27                   order test (dt vs dt/2),
                                                                     Kubo closed form
 xnoise / xrandom      Simulator/xnoise.m:26-60, xrandom.m:18-34     moments, KS test;
   (incl. knoises,     xift.m                                        filtered-noise covariance
@@ -36,7 +36,7 @@ xnoise / xrandom      Simulator/xnoise.m:26-60, xrandom.m:18-34     moments, KS 
                                                                     unpinned
 xpath                 Simulator/xpath.m:16-86                       Batchtest rows, checks
                                                                     pairing test
-xdata                 Simulator/xdata.m:129-192 (mean branch)       Batchtest rows
+xdata                 Simulator/xdata.m:21-85 (mean branch)       Batchtest rows
 xensemble             Simulator/xensemble.m:12-108                  Batchtest #21 (order
                                                                     extrapolation)
 xsim statistics       Simulator/xsim.m:64-188                       Batchtest rows,
@@ -234,7 +234,7 @@ def xpreferences(p_in):
 
 # ------------------------------------------------------------------ propagator
 def xpropfactor(p):
-    """propagator = exp(L * dtr / ipsteps) (xpropfactor.m:143-156); also the
+    """propagator = exp(L * dtr / ipsteps) (xpropfactor.m:26-39); also the
     Neumann patch correction da_d = -(L2 - L1) dt range_d^2 / pi^2 from the first
     two k values of each non-periodic dimension (xpropfactor.m:40-51), into p.da."""
     p.da = [0.0] * len(p.btypes)
@@ -313,7 +313,7 @@ def xpatch(a, p):
 
 def xsetbound(a, p):
     """Dirichlet ends set to their boundary values, dimension by dimension (later
-    dimensions win at corners) (xsetbound.m:92-121): p.boundfun at p.t, else the
+    dimensions win at corners) (xsetbound.m:30-59): p.boundfun at p.t, else the
     p.boundval constants (default 0, spdpreferences.m:26-43,75-93)."""
     if not p.setboundaries:
         return a
@@ -334,7 +334,7 @@ def xsetbound(a, p):
 
 
 def xtransform(a, p, inverse):
-    """Per space dimension spectral transform of xtransform.m:208-245."""
+    """Per space dimension spectral transform of xtransform.m:33-70."""
     for d, (t, _) in enumerate(p.btypes, start=1):
         if a.shape[d] > 1:
             a = transform(a, d, t, inverse)
@@ -383,7 +383,7 @@ def RK4(a, w, p, prop):
 
 
 def MP(a, w, p, prop):
-    """Interaction-picture midpoint with p.iterations fixed-point passes (MP.m:53-70)."""
+    """Interaction-picture midpoint with p.iterations fixed-point passes (MP.m:10-27)."""
     dt = p.dtr / 2.0
     t = p.t
     a = xprop(a, p, prop)
@@ -464,7 +464,7 @@ def xnft(a, trans, p):
 
 
 def xdata(a, n, p):
-    """Ensemble mean of observable n (xdata.m:133-192, mean branch): [lines, nsp_o]."""
+    """Ensemble mean of observable n (xdata.m:25-84, mean branch): [lines, nsp_o]."""
     if np.any(p.transforms[n][1:]):
         a = xnft(a, p.transforms[n], p)
     o = np.asarray(p.observe[n](a, p))
@@ -499,7 +499,7 @@ def xpath(a, nc, p, prop, nsp, seq=0, inst=0, noise_fn=None):
                     wn = draw(q)
                     q += 1
                     if w is not None and chk and nc == 2:
-                        wn = (wn + w) / 2                                 # xnoise.m:140-141
+                        wn = (wn + w) / 2                                 # xnoise.m:33-34
                     w = wn
                     if nc == 1 or c == 1:
                         a = method(a, w, p, prop)

@@ -1,6 +1,6 @@
 """CPU-side checks of the C ABI: the library loads, exports every declared
 entry point, and rejects invalid descriptors before touching a device
-(the analogue of the reference's error() aborts, xtransform.m:225-226)."""
+(the analogue of the reference's error() aborts, xtransform.m:51)."""
 import ctypes as C
 
 import numpy as np

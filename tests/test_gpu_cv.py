@@ -90,11 +90,16 @@ def _oracle_at_last_point(q, j):
     return np.stack([np.real(f(a, q)).reshape(-1) for f in fs], -1)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 2e-4)])
-def test_tw_trajectory_outputs_match_oracle(prec, tol):
+@pytest.mark.parametrize("prec,tol,n,lattice", [("fp64", 1e-9, 32, "repulsive"), ("fp32", 2e-4, 32, "repulsive"),
+                                               ("fp32", 2e-4, 128, "repulsive"), ("fp32", 2e-4, 128, "attractive")])
+def test_tw_trajectory_outputs_match_oracle(prec, tol, n, lattice):
     """Per-trajectory lattice means (|a|^2, |a|^4, low-mode occupation) and the
-    noise-mode features against the oracle, trajectory by trajectory."""
-    p = shrink(tw2d(prec, n=32, modes=32), E=8, e2=2, points=3, steps=5)
+    noise-mode features against the oracle, trajectory by trajectory; at 32^2 and
+    at config 5's 128^2 lattice, with the default repulsive lattice and with the
+    attractive lattice of the control-variate benchmark (g = -0.035, hop 270)."""
+    kw = dict(g=-0.035, hop=270.0) if lattice == "attractive" else {}
+    E = 8 if n == 32 else 2
+    p = shrink(tw2d(prec, n=n, modes=32, **kw), E=E, e2=2, points=3, steps=5 if n == 32 else 3)
     p["gpu"] = dict(p["gpu"], observe=["abs2_ave", "abs4_ave", "lowk"])
     prob = Problem(p)
     prob.handle.run_ensemble(1)
@@ -102,7 +107,7 @@ def test_tw_trajectory_outputs_match_oracle(prec, tol):
     obs, feat = prob.handle.read_trajectories()
     prob.close()
     q = X.xpreferences(oracle_problem(p))
-    modes = lowest_modes(32, 32, 32)
+    modes = lowest_modes(n, n, 32)
     for j in (1, 2):
         ref = _oracle_at_last_point(q, j)                              # [E, 3]
         errs = [_rel(obs[j - 1, 0, :, -1, o], ref[:, o]) for o in range(3)]

@@ -49,7 +49,7 @@ def test_streams_distinct():
 
 
 def test_noise_scale_is_one_over_dtr_dv():
-    # xnoise.m:138-139 with nscale = steps*(1+checks)/dt, noisefactor = 1/dv
+    # xnoise.m:31-32 with nscale = steps*(1+checks)/dt, noisefactor = 1/dv
     p = X.xpreferences(dict(dimensions=2, points=[11, 64], ranges=[1.0, 8.0], steps=[4, 1],
                             noises=2, ensembles=[256, 1, 1]))
     w = X.noise_draw(p, 1, 0)
@@ -60,7 +60,7 @@ def test_noise_scale_is_one_over_dtr_dv():
 
 
 def test_coarse_noise_is_pair_average():
-    # xnoise.m:140-141: the coarse step uses (w_{2m} + w_{2m+1})/2
+    # xnoise.m:33-34: the coarse step uses (w_{2m} + w_{2m+1})/2
     p = dict(dimensions=1, points=[3], ranges=[1.0], steps=[1], checks=1, noises=1,
              ensembles=[5, 1, 1], method="MP",
              deriv=lambda a, w, p: w[0:1] + 0 * a, initial=lambda v, p: 0 * v)

@@ -219,9 +219,12 @@ class Handle:
         """Raw fields [e2, nchk, points, E, nx*ny] complex of the last run_ensemble."""
         d = self.desc
         e2, nchk, E = d.ensembles[1], self.info.nchk, d.ensembles[0]
-        out = np.zeros(2 * e2 * nchk * d.points * E * self.info.nsites)
+        ns = self.info.nsites
+        out = np.zeros(2 * e2 * nchk * d.points * E * ns)
         check(load().xsh_read_raw(self._h, _ptr(out)))
-        return (out[0::2] + 1j * out[1::2]).reshape(e2, nchk, d.points, E, self.info.nsites)
+        # per path p.d.raw = [1, points, nx, ny, E] column-major: C order [E, site, points]
+        r = (out[0::2] + 1j * out[1::2]).reshape(e2, nchk, E, ns, d.points)
+        return np.ascontiguousarray(np.transpose(r, (0, 1, 4, 2, 3)))
 
     def probe_normals(self, nsp, tag, draw, traj0, ntraj, count):
         out = np.zeros(ntraj * count)

@@ -2,6 +2,8 @@
 
     python -m xspde_matlab_amd.build            # compile + link
     python -m xspde_matlab_amd.build --resource # also print VGPR/LDS/scratch use
+    python -m xspde_matlab_amd.build --changed  # recompile only units older than
+                                                # their source or any header
 
 Each translation unit is compiled by hipcc in parallel, then linked into
 ``xspde_matlab_amd/libxspde_hip.so``.  Packed FP32 instructions are
@@ -25,8 +27,19 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-munsafe-fp-atomics", "-Wno-unused-result"]
 
 
-def _compile(src, objdir, extra):
+def _stale(src, obj):
+    """The object is older than its source or than any header of the library."""
+    if not os.path.exists(obj):
+        return True
+    deps = [os.path.join(CSRC, src), os.path.join(HERE, "..", "include", "xspde_hip.h")]
+    deps += [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".inc"))]
+    return os.path.getmtime(obj) < max(os.path.getmtime(d) for d in deps)
+
+
+def _compile(src, objdir, extra, changed_only=False):
     obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+    if changed_only and not _stale(src, obj):
+        return obj, ""
     cmd = [HIPCC] + FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
@@ -36,12 +49,12 @@ def _compile(src, objdir, extra):
     return obj, out
 
 
-def build(resource=False, verbose=False):
+def build(resource=False, verbose=False, changed_only=False):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource else []
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
-        results = list(ex.map(lambda s: _compile(s, objdir, extra), SOURCES))
+        results = list(ex.map(lambda s: _compile(s, objdir, extra, changed_only and not resource), SOURCES))
     if resource or verbose:
         for _, out in results:
             if out:
@@ -56,4 +69,4 @@ def build(resource=False, verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(resource="--resource" in sys.argv, verbose="-v" in sys.argv))
+    print(build(resource="--resource" in sys.argv, verbose="-v" in sys.argv, changed_only="--changed" in sys.argv))

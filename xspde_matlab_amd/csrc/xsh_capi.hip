@@ -2,7 +2,7 @@
 //
 // Host side of the drop-in: descriptor validation, the lattice and scale
 // derivations of xpreferences.m:133-175 / spdpreferences.m:16-72, the
-// separable propagator tables of xpropfactor.m:143-156, and the launch of
+// separable propagator tables of xpropfactor.m:26-39, and the launch of
 // whole-path kernels for one path (p.path, xensemble.m:79) or all
 // ensembles(2) x checks paths of one parallel ensemble (xensemble.m:25-80).
 #include <hip/hip_runtime.h>
@@ -232,7 +232,7 @@ static std::vector<double> trig_matrix(int kind, int N) {
   return T;
 }
 // DFT on N points as an interleaved complex matrix: forward exp(-2 pi i nk/N),
-// inverse exp(+2 pi i nk/N)/N (fft / ifft of xtransform.m:220,235)
+// inverse exp(+2 pi i nk/N)/N (fft / ifft of xtransform.m:45,60)
 static std::vector<double> dft_matrix(int N, bool inverse) {
   std::vector<double> T(2 * (size_t)N * N);
   for (int k = 0; k < N; ++k)
@@ -371,7 +371,7 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   }
   h->avlen = off;
 
-  // tables: per nc the separable propagator factors (xpropfactor.m:145-156):
+  // tables: per nc the separable propagator factors (xpropfactor.m:37-39):
   //   P(kx,ky) = exp(dt l0) exp(dt (l1x i kx + l2x (i kx)^2)) exp(dt (l1y i ky + l2y (i ky)^2))
   // with dt = dtr/ipsteps, ipsteps = 2 and the k-grid of each boundary type
   // (spdpreferences.m:56-67).  FFT lattices: k in FFT order, inverse-FFT 1/N
@@ -417,7 +417,7 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   // streaming kernels: twiddles of the half-length column transforms
   std::vector<cd> twh(std::max(1, npt[1] / 2));
   for (int m = 0; m < (int)twh.size(); ++m) twh[m] = std::polar(1.0, -2.0 * M_PI * m / (npt[1] / 2 > 0 ? npt[1] / 2 : 1));
-  // dense transforms (xtransform.m:208-245): forward x, forward y, inverse x, inverse y;
+  // dense transforms (xtransform.m:33-70): forward x, forward y, inverse x, inverse y;
   // real matrices are stored with a zero imaginary part dropped (one value per entry)
   std::vector<std::vector<double>> mats(4);
   int cplx[2] = {0, 0};
@@ -489,7 +489,8 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   // from the first two k values of the dimension (xpropfactor.m:40-51), per
   // check (dt = dtr/2).
   const int nsh = std::max(h->npad[0], h->npad[1]);
-  const int nbv = d.dimensions == 3 ? nsh : 1;
+  const int nbv = d.dimensions == 3 ? nsh : 1;                      // device row width (kernel lattice)
+  const int nb_in = d.dimensions == 3 ? std::max(h->nx, h->ny) : 1;  // caller's row width (physical lattice)
   const int nlev = d.boundtab ? d.boundlevels : 1;
   std::vector<double> bshape((size_t)4 * nsh, 0.0);
   std::vector<cd> bvals((size_t)nlev * 4 * nbv, cd(0.0, 0.0));
@@ -516,8 +517,9 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     for (int lev = 0; lev < nlev; ++lev)
       for (int side = 0; side < 2; ++side)
         for (int o = 0; o < nother; ++o) {
-          const size_t at = (((size_t)lev * 2 + k) * 2 + side) * nbv + o;
-          const cd v = d.boundtab ? cd(d.boundtab[2 * at], d.boundtab[2 * at + 1])
+          const size_t line = ((size_t)lev * 2 + k) * 2 + side;
+          const size_t at = line * nbv + o, at_in = line * nb_in + o;
+          const cd v = d.boundtab ? cd(d.boundtab[2 * at_in], d.boundtab[2 * at_in + 1])
                                   : cd(d.boundval[k][side][0], d.boundval[k][side][1]);
           bvals[at] = v;
           if (v != cd(0.0, 0.0)) h->has_patch = true;
@@ -801,6 +803,34 @@ static int launch(xsh_handle* h, const xsh::PathArgs& A, int nblocks) {
   return XSH_OK;
 }
 
+// The kernels accumulate each observable as [points][sites] (sites fastest:
+// neighbouring lanes update neighbouring addresses).  The host buffers are the
+// reference's p.d.av{o} = [lines, points, space] in MATLAB column-major order
+// (xpreferences.m:340): time fastest.  One path's averages, device -> host order.
+static void av_to_host_order(const xsh_handle* h, const double* src, double* dst) {
+  const int npts = h->d.points;
+  for (int o = 0; o < h->d.nobserve; ++o) {
+    const long long off = h->obs_off[o];
+    const int ns = h->obs_ns[o];
+    for (int n = 0; n < npts; ++n)
+      for (int s = 0; s < ns; ++s) {
+        const long long a = off + (long long)n * ns + s, b = off + n + (long long)npts * s;
+        dst[2 * b] = src[2 * a];
+        dst[2 * b + 1] = src[2 * a + 1];
+      }
+  }
+}
+
+// copy n_paths consecutive path averages from the device to host order
+static int copy_averages(xsh_handle* h, const double* dev, int n_paths, double* out, hipStream_t s) {
+  const size_t per = (size_t)h->avlen * 2;
+  std::vector<double> tmp((size_t)n_paths * per);
+  HIPCHK(hipMemcpyAsync(tmp.data(), dev, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int p = 0; p < n_paths; ++p) av_to_host_order(h, tmp.data() + p * per, out + p * per);
+  return XSH_OK;
+}
+
 extern "C" int xsh_run_path(xsh_handle* h, int npar, int j, int nc, const double* a0, double* a_out,
                             double* av_out) {
   if (!h) return fail(XSH_EINVAL, "handle is NULL");
@@ -824,10 +854,8 @@ extern "C" int xsh_run_path(xsh_handle* h, int npar, int j, int nc, const double
   A.a_out = a_out ? h->d_a : nullptr;
   int rc = launch(h, A, h->groups);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(av_out, h->d_path_av, (size_t)h->avlen * 16, hipMemcpyDeviceToHost, h->stream));
   if (a_out) HIPCHK(hipMemcpyAsync(a_out, h->d_a, fbytes, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  return XSH_OK;
+  return copy_averages(h, h->d_path_av, 1, av_out, h->stream);
 }
 
 extern "C" int xsh_run_ensemble(xsh_handle* h, int npar) {
@@ -859,18 +887,28 @@ extern "C" int xsh_synchronize(xsh_handle* h, double* ms) {
 extern "C" int xsh_read_averages(xsh_handle* h, double* av_out) {
   if (!h || !av_out) return fail(XSH_EINVAL, "NULL argument");
   if (!h->ran) return fail(XSH_ESTATE, "xsh_run_ensemble has not run");
-  HIPCHK(hipMemcpyAsync(av_out, h->d_av, (size_t)h->e2 * h->nchk * h->avlen * 16, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  return XSH_OK;
+  return copy_averages(h, h->d_av, h->e2 * h->nchk, av_out, h->stream);
 }
 
 extern "C" int xsh_read_raw(xsh_handle* h, double* out) {
   if (!h || !out) return fail(XSH_EINVAL, "NULL argument");
   if (!h->ran) return fail(XSH_ESTATE, "xsh_run_ensemble has not run");
   if (!h->d_raw) return fail(XSH_ESTATE, "descriptor did not ask for raw fields");
-  const size_t rbytes = (size_t)h->e2 * h->nchk * h->d.points * (size_t)h->E * h->ns * 16;
-  HIPCHK(hipMemcpyAsync(out, h->d_raw, rbytes, hipMemcpyDeviceToHost, h->stream));
+  // device [path][points][E][sites] -> per path p.d.raw = [1, points, nx, ny, E]
+  // column-major (xpreferences.m:148): time fastest, then sites, then trajectories
+  const size_t npts = h->d.points, E = h->E, ns = h->ns, per = npts * E * ns;
+  const size_t paths = (size_t)h->e2 * h->nchk;
+  std::vector<double> tmp(2 * paths * per);
+  HIPCHK(hipMemcpyAsync(tmp.data(), h->d_raw, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
+  for (size_t p = 0; p < paths; ++p)
+    for (size_t n = 0; n < npts; ++n)
+      for (size_t e = 0; e < E; ++e)
+        for (size_t s = 0; s < ns; ++s) {
+          const size_t a = p * per + (n * E + e) * ns + s, b = p * per + n + npts * (s + ns * e);
+          out[2 * b] = tmp[2 * a];
+          out[2 * b + 1] = tmp[2 * a + 1];
+        }
   return XSH_OK;
 }
 
@@ -1276,10 +1314,9 @@ extern "C" int xsh_batch_read_averages(xsh_batch* b, int i, double* av_out) {
   if (!b || !av_out) return fail(XSH_EINVAL, "NULL argument");
   if (i < 0 || i >= (int)b->h.size()) return fail(XSH_EINVAL, "instance %d outside 0..%d", i, (int)b->h.size() - 1);
   if (!b->ran) return fail(XSH_ESTATE, "xsh_batch_run has not run");
-  const xsh_handle* h = b->h[i];
+  xsh_handle* h = b->h[i];
   HIPCHK(hipStreamSynchronize(b->s0));
-  HIPCHK(hipMemcpy(av_out, h->d_av, (size_t)h->e2 * h->nchk * h->avlen * 16, hipMemcpyDeviceToHost));
-  return XSH_OK;
+  return copy_averages(h, h->d_av, h->e2 * h->nchk, av_out, b->s0);
 }
 
 extern "C" int xsh_batch_query(const xsh_batch* b, int i, xsh_info* info) {

@@ -4,7 +4,7 @@
 //
 //  sde_path    dimensions = 1 (no space): one trajectory per lane, the whole
 //              path in registers (xpath.m:29-86 with RK4.m / MP.m on a scalar
-//              field; xprop.m:104-106 reduces to a scalar factor).
+//              field; xprop.m:33-35 reduces to a scalar factor).
 //  spde_path   dimensions = 2, 3 on periodic lattices: one workgroup per TPB
 //              trajectories, fields resident in VGPRs for the whole path.
 //  probes      normals and propagation of given planes (parity hooks).
@@ -89,7 +89,7 @@ __device__ __forceinline__ void knormals(const PathArgs& A, uint32_t nsp, uint32
   z1 = ncomp < 2 ? R(0) : (lane == 0 ? n[1] : n[3]);
 }
 
-// one normal / uniform of the initial randoms (xrandom.m:187-200)
+// one normal / uniform of the initial randoms (xrandom.m:18-33)
 template <class R>
 __device__ __forceinline__ R init_normal(const PathArgs& A, uint32_t nsp, uint32_t traj, int L) {
   R n[4];
@@ -372,7 +372,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     XSH_END
   XSH_END
 
-  // Noise of draw q (coarse: mean of draws q, q+1, xnoise.m:140-141) for
+  // Noise of draw q (coarse: mean of draws q, q+1, xnoise.m:33-34) for
   // register chunk i.  It is regenerated from the counter at each of the
   // method's drift stages instead of being held for the whole step: that keeps
   // a single field resident during every transform, the peak of the register
@@ -408,7 +408,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     }
   };
 
-  // Dirichlet ends of the lattice (xsetbound.m:102-116, default boundary value 0)
+  // Dirichlet ends of the lattice (xsetbound.m:40-53, default boundary value 0)
   auto dir_site = [&](int i, int v) __attribute__((always_inline)) -> bool {
     const int site = Lat::point(i, v) % NS, iy = site / NX, ix = site % NX;
     return (ix == 0 && A.dir_lo[0]) || (ix == NXP - 1 && A.dir_hi[0]) ||
@@ -461,7 +461,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
       XSH_END
     XSH_END
   };
-  // xprop.m:88-117: transform, propagator, inverse transform, then setbound
+  // xprop.m:17-46: transform, propagator, inverse transform, then setbound
   auto propagate = [&](C (&f)[PPT], int lev) __attribute__((always_inline)) {
     if (A.has_linear) {
       if constexpr (MAT) {
@@ -592,7 +592,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
         site(1, a1, b1, w1);
       });
     };
-    // MP.m:56-67: fixed-point passes at the midpoint, per site, on f
+    // MP.m:13-24: fixed-point passes at the midpoint, per site, on f
     auto mp_stage = [&](C (&f)[PPT]) __attribute__((always_inline)) {
       chunks1(f, [&](int i, C& x0, C& x1) __attribute__((always_inline)) {
         C w0, w1;

@@ -9,14 +9,14 @@
 //              reversed order), one LDS read+write per stage
 //   propagate: the last forward stage, the multiply by the propagator
 //              P(kx,ky) = exp(L dtr/2) and the first inverse stage are one
-//              register-resident step (xprop.m:104-106)
+//              register-resident step (xprop.m:33-35)
 //   inverse  : in-place decimation-in-time stages (digit reversed ->
 //              natural order), so no permutation pass is ever needed.
 //
 // Noise is generated in registers per step from a Philox4x32-10 counter
-// (xnoise.m:138-141 semantics), held constant over the method's stages
+// (xnoise.m:31-34 semantics), held constant over the method's stages
 // (RK4.m:13-37).  Observables are reduced in FP64 and accumulated with
-// FP64 atomics into the per-path ensemble means (xdata.m:192).
+// FP64 atomics into the per-path ensemble means (xdata.m:84).
 #pragma once
 #include <type_traits>
 #include "xsh_common.hpp"
@@ -39,8 +39,8 @@ struct PathArgs {
   const void* kr_filter;   // complex R [krandoms][ns]: random filter, times
                            // sqrt(knoisefactor) / ns
   void* kn_work;           // complex R [grid * TPB][ns]: x-space k-noise of the step
-  double noise_scale;      // sqrt(nscale * noisefactor), xnoise.m:139
-  double init_scale;       // sqrt(noisefactor), xrandom.m:189
+  double noise_scale;      // sqrt(nscale * noisefactor), xnoise.m:32
+  double init_scale;       // sqrt(noisefactor), xrandom.m:22
   int deriv;
   double dp[6];
   int initial;
@@ -60,7 +60,7 @@ struct PathArgs {
   const void* tm[4];       // forward x, forward y, inverse x, inverse y (row-major N x N)
   const void* tmp[4];      // FP32 real matrices with N % 32 == 0 in MFMA operand order
                            // Tp[kb][kk/2][lane] = T[32 kb + lane%32][kk + lane/32], else NULL
-  int dirichlet;           // any Dirichlet end (xsetbound.m:102-116)
+  int dirichlet;           // any Dirichlet end (xsetbound.m:40-53)
   double* traj;            // optional [paths][E][npts][nobs] per-trajectory lattice means
   double* raw;             // optional [paths][npts][E][ns] complex double fields
   int nmodes;              // Fourier modes of the initial noise to keep
@@ -464,7 +464,7 @@ struct Lattice {
   }
 
   // a -> T^-1 op(T a) with dense per-dimension transforms (boundary types of
-  // xtransform.m:208-245); op receives natural-order (kx, ky).
+  // xtransform.m:33-70); op receives natural-order (kx, ky).
   template <class Op>
   __device__ static __forceinline__ void spectral_mat(C* L, C (&f)[PPT], const void* const* Tm,
                                                       const void* const* Tp, int cplx_x, int cplx_y,

@@ -76,7 +76,7 @@ struct Stream {
     }
   }
 
-  // Dense-transform lattices (non-periodic boundaries, xtransform.m:208-245):
+  // Dense-transform lattices (non-periodic boundaries, xtransform.m:33-70):
   // F -> Tx^-1 Ty^-1 op(Ty Tx F) in place, the x passes on row halves and the y
   // passes on column halves in LDS (each half fits), matrices Tm[0..3] =
   // forward x, forward y, inverse x, inverse y, complex for periodic dimensions
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
           for (int v = 0; v < 2; ++v) F0[s + v] = F1[s + v] + (hdt / R(3)) * D(F0[s + v], w[v]);
         }
       } else {
-        // MP.m:56-67
+        // MP.m:13-24
         propagate(F0);
 #pragma unroll 1
         for (int i = 0; i < PAIRS; ++i) {

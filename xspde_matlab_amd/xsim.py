@@ -307,12 +307,14 @@ def lowest_modes(nx, ny, count):
 
 
 def split_averages(q, info, av):
-    """Flat per-path averages -> av[o] of shape [1, points, sites_o] (xpath.m:16,81)."""
+    """Flat per-path averages -> av[o] of shape [1, points, sites_o] (xpath.m:16,81).
+    Each observable's block is p.d.av{o} = [lines, points, space] in MATLAB
+    column-major order (xpreferences.m:340): time fastest, then x, then y."""
     out = []
     npts = q["points"][0]
     for o in range(len(q["gpu"].get("observe", ["a"]))):
         off, ns = info.obs_offset[o], info.obs_sites[o]
-        out.append(av[off:off + npts * ns].reshape(1, npts, ns))
+        out.append(np.reshape(av[off:off + npts * ns], (1, npts, ns), order="F"))
     return out
 
 
@@ -338,37 +340,43 @@ class Problem:
 
     def xensemble(self, npar):
         """All sub-ensembles and checks of one parallel ensemble (xensemble.m:25-105)."""
-        q = self.q
         self.handle.run_ensemble(npar)
         self.handle.synchronize()
-        av_all = self.handle.read_averages()
-        e2, e3 = q["ensembles"][1], q["ensembles"][2]
-        nchk = 1 + q["checks"][0]
-        ne = 1.0 / (e2 * e3) - 1
-        m = nchk
-        m1 = max(3, m + 1)
-        data = [None] * len(self.outputs)
-        for j in range(e2):
-            D = {}
-            for nc in range(nchk):
-                av = split_averages(q, self.info, av_all[j, nc])
-                for o, out in enumerate(self.outputs):
-                    D[(nc + 1, o)] = np.real(np.asarray(out(av, q))).reshape(-1)
-            for o in range(len(self.outputs)):
-                nes = 1 + ne
-                if q["checks"][0] and q["order"] > 0:                    # xensemble.m:93-97
-                    dl = (D[(1, o)] - D[(2, o)]) / (2.0 ** q["order"] - 1.0)
-                    D[(2, o)] = D[(1, o)]
-                    D[(1, o)] = D[(1, o)] + dl
-                if data[o] is None:
-                    data[o] = np.zeros((D[(1, o)].size, m1))
-                for n1 in range(1, m + 1):
-                    data[o][:, n1 - 1] += D[(n1, o)] * nes
-                data[o][:, m1 - 1] += D[(1, o)] ** 2 * nes
-        return data
+        return ensemble_moments(self.q, self.info, self.outputs, self.handle.read_averages())
 
     def close(self):
         self.handle.close()
+
+
+def ensemble_moments(q, info, outputs, av_all):
+    """D and D^2 sums of one parallel ensemble from the averages of its paths
+    (xensemble.m:84-105): av_all[j, nc] is the flat average of path (j, nc);
+    each output's D is extrapolated by (D_fine - D_coarse) / (2^order - 1) when
+    checks and order > 0; the sums carry the weight 1 / (ensembles(2) ensembles(3)).
+    Returns data[o] of shape [ndata_o, max(3, nchk + 1)]: sum D per check, then
+    sum D_fine^2."""
+    e2, e3 = q["ensembles"][1], q["ensembles"][2]
+    nchk = 1 + q["checks"][0]
+    w = 1.0 / (e2 * e3)
+    m1 = max(3, nchk + 1)
+    data = [None] * len(outputs)
+    for j in range(e2):
+        D = {}
+        for nc in range(nchk):
+            av = split_averages(q, info, av_all[j, nc])
+            for o, out in enumerate(outputs):
+                D[(nc + 1, o)] = np.real(np.asarray(out(av, q))).reshape(-1)
+        for o in range(len(outputs)):
+            if q["checks"][0] and q["order"] > 0:                        # xensemble.m:93-97
+                dl = (D[(1, o)] - D[(2, o)]) / (2.0 ** q["order"] - 1.0)
+                D[(2, o)] = D[(1, o)]
+                D[(1, o)] = D[(1, o)] + dl
+            if data[o] is None:
+                data[o] = np.zeros((D[(1, o)].size, m1))
+            for n1 in range(1, nchk + 1):
+                data[o][:, n1 - 1] += D[(n1, o)] * w
+            data[o][:, m1 - 1] += D[(1, o)] ** 2 * w
+    return data
 
 
 def statistics(q, data):
