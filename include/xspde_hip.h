@@ -151,6 +151,9 @@ typedef struct {
   int threads, traj_per_block; /* launch shape of the path kernel              */
   int kernel_kind;          /* 0 register-resident FFT, 1 streaming (fields in
                                device memory), 2 dense transforms, 3 SDE       */
+  int ensembles[3];         /* the descriptor's ensembles (sizes of a0, a_out
+                               and of the averages of xsh_read_averages)       */
+  int points;               /* stored time points                              */
 } xsh_info;
 
 /* Library version (XSH_ABI_VERSION). */

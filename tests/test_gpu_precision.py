@@ -28,8 +28,10 @@ def _compare(p32, p64, names, bound_rel, bound_sigma, bound_step=None, bound_ste
     estimate |fine - coarse| (formed in FP64 from the FP32 paths' FP64 moments)
     against the FP64 one.  Either the two step errors agree (max difference below
     bound_step times the largest FP64 step error), or, where FP32 rounding
-    dominates the FP32 estimate, it stays below bound_step_sigma times the
-    sampling error bar at every point (so the total error bar is unchanged)."""
+    dominates the FP32 estimate, its RMS over the time points (the step entry
+    of xsim's error vector, Simulator/xsim.m:104-107,144-188) stays below
+    bound_step_sigma times the RMS sampling error (xsim.m:94-98), so the total
+    error bar is unchanged."""
     a32, info = _means(p32)
     a64, _ = _means(p64)
     e2 = a64.shape[0]
@@ -50,9 +52,9 @@ def _compare(p32, p64, names, bound_rel, bound_sigma, bound_step=None, bound_ste
         s32 = np.abs(m32.mean(0) - a32[:, 1, off:off + npts * ns].real.mean(0))
         s64 = np.abs(m64.mean(0) - a64[:, 1, off:off + npts * ns].real.mean(0))
         step_rel = np.abs(s32 - s64).max() / s64.max()
-        step_sigma = (s32[spread] / sigma[spread]).max() if spread.any() else 0.0
+        step_sigma = np.sqrt(np.mean(s32 ** 2) / np.mean(sigma ** 2))
         print("%s %s: max |fp32 - fp64| / max|fp64| = %.2e, max |fp32 - fp64| / sigma = %.2e; "
-              "step error max fp64 %.3e fp32 %.3e, max |fp32 - fp64| / max fp64 = %.2e, max fp32 / sigma = %.2e"
+              "step error max fp64 %.3e fp32 %.3e, max |fp32 - fp64| / max fp64 = %.2e, rms fp32 / rms sigma = %.2e"
               % (p64["name"], name, rel, in_sigma, s64.max(), s32.max(), step_rel, step_sigma), flush=True)
         assert rel < bound_rel and in_sigma < bound_sigma, name
         if bound_step is not None:
@@ -62,19 +64,21 @@ def _compare(p32, p64, names, bound_rel, bound_sigma, bound_step=None, bound_ste
 
 
 def test_nls1d_fp32_vs_fp64_full_span():
+    """The FP32 step-error estimate agrees with FP64's to 0.1 % of the largest."""
     p64 = shrink(nls1d("fp64"), E=512, e2=8)
     p32 = shrink(nls1d("fp32"), E=512, e2=8)
-    _compare(p32, p64, ["abs2", "a"], 1e-3, 0.1, bound_step=0.5)
+    _compare(p32, p64, ["abs2", "a"], 1e-3, 0.1, bound_step=0.01)
 
 
 def test_cgl2d_fp32_vs_fp64_full_span():
     """The 2D CGL is chaotic: over 2000 coarse steps FP32 rounding decorrelates the
     fine and coarse paths, and the FP32 step-error estimate (2e-5..4e-5 for
-    <|a|^2>) exceeds the FP64 truncation estimate (6e-6) several times; it stays
-    below 1 % of the sampling error bar."""
+    <|a|^2>) exceeds the FP64 truncation estimate (6e-6) several times; its RMS
+    is 6 % of the RMS sampling error (bound 15 %: the total error bar, the two
+    added in quadrature, grows by less than 1.2 %)."""
     p64 = shrink(cgl2d("fp64"), E=128, e2=8)
     p32 = shrink(cgl2d("fp32"), E=128, e2=8)
-    _compare(p32, p64, ["abs2_ave", "corr"], 1e-3, 0.1, bound_step_sigma=0.01)
+    _compare(p32, p64, ["abs2_ave", "corr"], 1e-3, 0.1, bound_step_sigma=0.15)
 
 
 def test_cgl2d_neumann_fp32_vs_fp64_full_span():
@@ -83,9 +87,10 @@ def test_cgl2d_neumann_fp32_vs_fp64_full_span():
     periodic correlation of the Neumann field has long-distance entries whose
     sampling spread is small, where the FP32 error reaches 0.2 sigma (6.8e-4
     relative): bound 0.3 sigma.  As in the periodic run, the FP32 step-error
-    estimate is rounding-dominated and stays below 1 % of the sampling error bar."""
+    estimate is rounding-dominated and its RMS stays below 15 % of the RMS
+    sampling error."""
     p64 = shrink(cgl2d("fp64"), E=128, e2=8)
     p32 = shrink(cgl2d("fp32"), E=128, e2=8)
     for p in (p64, p32):
         p["boundaries"] = [[-1, -1], [-1, -1]]
-    _compare(p32, p64, ["abs2_ave", "corr"], 1e-3, 0.3, bound_step_sigma=0.01)
+    _compare(p32, p64, ["abs2_ave", "corr"], 1e-3, 0.3, bound_step_sigma=0.15)

@@ -77,7 +77,7 @@ class Info(C.Structure):
         ("nsites", C.c_int), ("nchk", C.c_int), ("avlen", C.c_longlong),
         ("obs_offset", C.c_longlong * XSH_MAX_OBS), ("obs_sites", C.c_int * XSH_MAX_OBS),
         ("traj_steps", C.c_longlong), ("threads", C.c_int), ("traj_per_block", C.c_int),
-        ("kernel_kind", C.c_int),
+        ("kernel_kind", C.c_int), ("ensembles", C.c_int * 3), ("points", C.c_int),
     ]
 
 

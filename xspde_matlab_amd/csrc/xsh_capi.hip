@@ -41,6 +41,12 @@ int fail(int code, const char* fmt, ...) {
   } while (0)
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// clears device doubles (a kernel, not a memset node, so captured graphs replay it)
+__global__ void zero_doubles(double* p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
 }  // namespace
 
 struct xsh_handle {
@@ -83,6 +89,7 @@ struct xsh_handle {
   hipEvent_t ev_args;      // recorded after the last argument upload
   bool args_pending;
   bool ran;
+  int device;              // device current at xsh_create
 };
 
 extern "C" int xsh_version(void) { return XSH_ABI_VERSION; }
@@ -255,6 +262,10 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(XSH_EDEVICE, "no HIP device");
   auto* h = new xsh_handle();
   std::memset(h, 0, sizeof(*h));
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return fail(XSH_EDEVICE, "hipGetDevice");
+  }
   h->d = *desc;
   const xsh_desc& d = h->d;
   h->nd = d.dimensions;
@@ -687,6 +698,8 @@ extern "C" int xsh_query(const xsh_handle* h, xsh_info* info) {
   info->threads = h->threads;
   info->traj_per_block = h->tpb;
   info->kernel_kind = h->nd == 1 ? 3 : (h->kern && h->kern->stream) ? 1 : h->mat ? 2 : 0;
+  for (int i = 0; i < 3; ++i) info->ensembles[i] = h->d.ensembles[i];
+  info->points = h->d.points;
   return XSH_OK;
 }
 
@@ -1001,11 +1014,6 @@ extern "C" int xsh_probe_propagate(xsh_handle* h, int nc, int nplanes, double* p
 // the groups run concurrently on forked streams inside one captured HIP
 // graph, after fill kernels that clear the averages.
 namespace {
-
-__global__ void zero_doubles(double* p, long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
-}
 
 struct BatchGroup {
   int precision, nx, ny, mat;

@@ -10,6 +10,7 @@
 //  probes      normals and propagation of given planes (parity hooks).
 #pragma once
 #include "xsh_path.hpp"
+#include "xsh_plane.hpp"
 #include "xsh_launch.hpp"
 
 namespace xsh {
@@ -260,6 +261,31 @@ __device__ __forceinline__ int find_instance(const int* __restrict__ wstart, int
   return lo;
 }
 
+// Lattices whose periodic transform runs on Plane (wave-owned rows, two
+// barriers per transform): one 128 x 128 trajectory per 1024-thread workgroup.
+template <int NX, int NY, int NT, int TPB, bool MAT, int RX, bool KN>
+constexpr bool kPlaneFFT = !MAT && !KN && RX == 0 && NX == 128 && NY == 128 && NT == 1024 && TPB == 1;
+
+// LDS elements of the transform workspace
+template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX, bool KN>
+constexpr int lds_elems() {
+  if constexpr (kPlaneFFT<NX, NY, NT, TPB, MAT, RX, KN>)
+    return Plane<R, NX, NT>::LDS > Lattice<R, NX, NY, NT, TPB, RX>::LDS ? Plane<R, NX, NT>::LDS
+                                                                          : Lattice<R, NX, NY, NT, TPB, RX>::LDS;
+  else
+    return Lattice<R, NX, NY, NT, TPB, RX>::LDS;
+}
+
+// a -> T^-1 op(T a) of the planes held in f, periodic lattices
+template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX, bool KN, class Op>
+__device__ __forceinline__ void periodic_spectral(cpx<R>* L, cpx<R> (&f)[Lattice<R, NX, NY, NT, TPB, RX>::PPT],
+                                                  const cpx<R>* twx, const cpx<R>* twy, const Op& op) {
+  if constexpr (kPlaneFFT<NX, NY, NT, TPB, MAT, RX, KN>)
+    Plane<R, NX, NT>::spectral(L, f, twx, twy, op);
+  else
+    Lattice<R, NX, NY, NT, TPB, RX>::spectral(L, f, twx, twy, op);
+}
+
 // One work item (a group of TPB trajectories of one path) of one instance.
 template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX, bool KN>
 __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __restrict__ L, double* red) {
@@ -290,7 +316,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   const R hdt = (R)(nci ? A.hdt[1] : A.hdt[0]);
   Drift<R> D;
   D.init(A);
-  const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
+  const std::conditional_t<(NY > 1), PropOp2<R>, PropOp<R>> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
                        NY > 1 ? static_cast<const C*>(nci ? A.py[1] : A.py[0]) : nullptr};
   const C* twx = static_cast<const C*>(A.twx);
   const C* twy = static_cast<const C*>(A.twy);
@@ -376,7 +402,9 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   // register chunk i.  It is regenerated from the counter at each of the
   // method's drift stages instead of being held for the whole step: that keeps
   // a single field resident during every transform, the peak of the register
-  // budget.
+  // budget (holding the step's noise of a 128x128 plane adds 32 VGPRs per
+  // thread to the two fields' 64, and the compiler then spills the fields
+  // around every transform).
   auto noise_chunk = [&](int i, uint32_t q, bool coarse, C& w0, C& w1) __attribute__((always_inline)) {
     if (!MAT && KN && A.knoises) {  // the step's k-noise, transformed once per step
       const int pt = Lat::point(i, 0);
@@ -469,7 +497,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
         Lat::spectral_mat(L, f, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop);
         patch(f, R(1), true, lev);
       } else {
-        Lat::spectral(L, f, twx, twy, prop);
+        periodic_spectral<R, NX, NY, NT, TPB, MAT, RX, KN>(L, f, twx, twy, prop);
       }
     }
     setbound(f, lev);
@@ -495,6 +523,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   // 3.23 s, UR = 1 3.03 s, UR = 2 2.94 s, UR = 4 3.04 s).  1D lattices, with
   // few chunks, unroll.
   constexpr int UR = CPT % 2 == 0 ? 2 : 1;
+  constexpr bool ROTATE = NY > 1;
   auto rotate = [&](C (&f)[PPT]) __attribute__((always_inline)) {
     C t[2 * UR];
     XSH_FOR(k, 0, 2 * UR) t[k] = f[k]; XSH_END
@@ -503,7 +532,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   };
   // body(i, a0, a1, b0, b1): chunk i of fields a and b (distinct arrays), in place
   auto chunks2 = [&](C (&a)[PPT], C (&b)[PPT], auto&& body) __attribute__((always_inline)) {
-    if constexpr (NY > 1) {
+    if constexpr (ROTATE) {
 #pragma unroll 1
       for (int i = 0; i < CPT; i += UR) {
         XSH_FOR(u, 0, UR)
@@ -520,7 +549,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     }
   };
   auto chunks1 = [&](C (&a)[PPT], auto&& body) __attribute__((always_inline)) {
-    if constexpr (NY > 1) {
+    if constexpr (ROTATE) {
 #pragma unroll 1
       for (int i = 0; i < CPT; i += UR) {
         XSH_FOR(u, 0, UR)
@@ -653,7 +682,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     ObsOp<R> op{mode, mode == 0 ? R(1.0 / ((double)NS * (double)NS)) : (mode == 2 ? fs * fs : fs), NX, A.fmap,
                 A.feat + ((long long)pidx * A.E + group * TPB) * 2 * A.nmodes, acc, A.kscale * A.inv_E,
                 group * TPB, A.E};
-    Lat::spectral(L, f1, twx, twy, op);
+    periodic_spectral<R, NX, NY, NT, TPB, MAT, RX, KN>(L, f1, twx, twy, op);
   };
 
   // Fourier modes of the initial noise (coherent initial form: a = alpha + n/2)
@@ -750,21 +779,22 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     }
   };
 
-  // ---- the time loop of xpath.m:29-86
+  // ---- the time loop of xpath.m:29-86.  One method_step code site for the
+  // fine and the coarse runs (coarse: one step per two fine draws, xnoise.m:33-34):
+  // the step is the kernel's largest code, and one copy keeps it in the
+  // instruction cache.
+  const bool coarse = A.checks && nci == 1;
+  const int nsub = coarse ? 1 : 1 + A.checks;
   uint32_t q = 0;
   int lev = 0;
   for (int n = 0; n < A.npts; ++n) {
     for (int st = 0; st < A.steps; ++st) {
       if (n > 0 || st > 0 || A.npts == 1) {
-        if (A.checks && nci == 1) {
-          method_step(q, true, lev);
-          q += 2;
-          lev += 4;
-        } else {
-          for (int c = 0; c <= A.checks; ++c) {
-            method_step(q++, false, lev);
-            lev += 2;
-          }
+#pragma unroll 1
+        for (int c = 0; c < nsub; ++c) {
+          method_step(q, coarse, lev);
+          q += coarse ? 2 : 1;
+          lev += coarse ? 4 : 2;
         }
       }
       if (st == 0) observe(n);
@@ -798,8 +828,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
 template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX, bool KN>
 __global__ __launch_bounds__(NT) void spde_path(const PathArgs* __restrict__ args, const int* __restrict__ wstart,
                                                 int ninst, int nwork) {
-  using Lat = Lattice<R, NX, NY, NT, TPB, RX>;
-  __shared__ cpx<R> L[Lat::LDS];
+  __shared__ cpx<R> L[lds_elems<R, NX, NY, NT, TPB, MAT, RX, KN>()];
   __shared__ double red[2 * (NT / 64)];
 #pragma unroll 1
   for (int wk = blockIdx.x; wk < nwork; wk += gridDim.x) {
@@ -815,7 +844,7 @@ __global__ __launch_bounds__(NT) void probe_propagate(const PathArgs* __restrict
   const PathArgs& A = *Ap;
   using Lat = Lattice<R, NX, NY, NT, TPB, RX>;
   using C = cpx<R>;
-  __shared__ C L[Lat::LDS];
+  __shared__ C L[lds_elems<R, NX, NY, NT, TPB, MAT, RX, false>()];
   C f[Lat::PPT];
   const long long base = (long long)blockIdx.x * Lat::NPTS;
 #pragma unroll
@@ -825,12 +854,13 @@ __global__ __launch_bounds__(NT) void probe_propagate(const PathArgs* __restrict
       const long long g = base + Lat::point(i, v);
       f[2 * i + v] = g < (long long)nplanes * Lat::NS ? planes[g] : C{R(0), R(0)};
     }
-  const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
+  const std::conditional_t<(NY > 1), PropOp2<R>, PropOp<R>> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
                        NY > 1 ? static_cast<const C*>(nci ? A.py[1] : A.py[0]) : nullptr};
   if constexpr (MAT)
     Lat::spectral_mat(L, f, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop);
   else
-    Lat::spectral(L, f, static_cast<const C*>(A.twx), static_cast<const C*>(A.twy), prop);
+    periodic_spectral<R, NX, NY, NT, TPB, MAT, RX, false>(L, f, static_cast<const C*>(A.twx),
+                                                          static_cast<const C*>(A.twy), prop);
 #pragma unroll
   for (int i = 0; i < Lat::CPT; ++i)
 #pragma unroll

@@ -88,31 +88,26 @@ struct PathArgs {
 };
 
 // ------------------------------------------------------------ drift forms
+// Every built-in drift is D(a, xi) = (g |a|^2 + u + i c xi_1) a + s xi with the
+// coefficients of the form (the others zero): Kubo.m:17 (i c w a), and the
+// GP/NLS/CGL family (g |a|^2 a + u a + s (w1 + i w2)).  One branch-free
+// expression: the stages run it for every site without control flow.
 template <class R>
 struct Drift {
-  int kind;
   cpx<R> g, u;
   R s, c;
   __device__ __forceinline__ void init(const PathArgs& A) {
-    kind = A.deriv;
-    g = cpx<R>{(R)A.dp[0], (R)A.dp[1]};
-    u = cpx<R>{(R)A.dp[2], (R)A.dp[3]};
-    s = (R)A.dp[4];
-    c = (R)A.dp[0];
+    const bool gp = A.deriv == XSH_DERIV_GP, kubo = A.deriv == XSH_DERIV_KUBO;
+    g = gp ? cpx<R>{(R)A.dp[0], (R)A.dp[1]} : cpx<R>{R(0), R(0)};
+    u = gp ? cpx<R>{(R)A.dp[2], (R)A.dp[3]} : cpx<R>{R(0), R(0)};
+    s = gp ? (R)A.dp[4] : R(0);
+    c = kubo ? (R)A.dp[0] : R(0);
   }
-  // D(a, xi): Kubo.m:17 (i c w a) or the GP/NLS/CGL family
   __device__ __forceinline__ cpx<R> operator()(cpx<R> a, cpx<R> xi) const {
-    if (kind == XSH_DERIV_KUBO) {
-      const R k = c * xi.x;
-      return cpx<R>{-k * a.y, k * a.x};
-    }
-    if (kind == XSH_DERIV_GP) {
-      const R n = norm2(a);
-      cpx<R> ga = g * a;
-      cpx<R> r = cpx<R>{n * ga.x, n * ga.y} + u * a;
-      return cpx<R>{r.x + s * xi.x, r.y + s * xi.y};
-    }
-    return cpx<R>{R(0), R(0)};
+    const R n = norm2(a);
+    const cpx<R> k = {g.x * n + u.x, g.y * n + u.y + c * xi.x};
+    const cpx<R> r = k * a;
+    return cpx<R>{r.x + s * xi.x, r.y + s * xi.y};
   }
 };
 
@@ -561,7 +556,7 @@ struct Lattice {
   }
 };
 
-// propagator op: X * Px[kx] * Py[ky]
+// propagator op: X * Px[kx] (1D) or X * Px[kx] * Py[ky] (py set: 2D)
 template <class R>
 struct PropOp {
   const cpx<R>* px;
@@ -570,6 +565,15 @@ struct PropOp {
   __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int kx, int ky, int p = 0) const {
     cpx<R> r = X * px[kx];
     return py ? r * py[ky] : r;
+  }
+};
+// the same on a 2D lattice: no test of py per element
+template <class R>
+struct PropOp2 {
+  const cpx<R>* px;
+  const cpx<R>* py;
+  __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int kx, int ky, int p = 0) const {
+    return (X * px[kx]) * py[ky];
   }
 };
 // observable ops, sharing one transform instance:

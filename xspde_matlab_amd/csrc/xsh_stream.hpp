@@ -233,7 +233,7 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
     path_identity<R>(A, wk - (ninst > 1 ? wstart[inst] : 0), pidx, nci, j, traj);
     const uint32_t nsp = (uint32_t)(A.seed + A.npar + (j - 1) * A.e3);
     const R hdt = (R)(nci ? A.hdt[1] : A.hdt[0]);
-    const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
+    const PropOp2<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
                          static_cast<const C*>(nci ? A.py[1] : A.py[0])};
     double* av = A.av + 2 * (long long)pidx * A.avlen;
 
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(NT) void probe_stream(const PathArgs* __restrict__ 
   using C = cpx<R>;
   __shared__ C L[S::LDS];
   if ((int)blockIdx.x >= nplanes) return;
-  const PropOp<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
+  const PropOp2<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
                        static_cast<const C*>(nci ? A.py[1] : A.py[0])};
   if constexpr (MAT)
     S::propagate_mat(L, planes + (long long)blockIdx.x * S::NS, A.tm, A.cplx[0], A.cplx[1], prop);
