@@ -184,6 +184,23 @@ int xsh_query(const xsh_handle* h, xsh_info* info);
 int xsh_run_path(xsh_handle* h, int npar, int j, int nc, const double* a0, double* a_out,
                  double* av_out);
 
+/* No counterpart in the reference: xsh_run_path for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, hosts
+ * that forbid hipMalloc).  Same path and kernels, but inputs and outputs are
+ * the caller's device buffers - a0_dev (NULL: the built-in initial form),
+ * a_out_dev (may be NULL), av_dev laid out as xsh_run_path's av_out - and the
+ * scratch (kernel arguments, the averages as the kernels accumulate them, the
+ * streaming kernel's field planes, k-space noise planes) is a static device
+ * array of XSH_SCRATCH_BYTES per device, held until the process ends.  Paths
+ * needing more scratch fail with XSH_EINVAL; the streaming kernel's persistent
+ * grid shrinks to the workgroups whose planes fit.  Asynchronous on stream (a
+ * hipStream_t, NULL: the handle's stream); nothing is allocated or freed.
+ * Every call on a device shares that device's array: calls that overlap on one
+ * device race. */
+#define XSH_SCRATCH_BYTES (64 * 1024 * 1024)
+int xsh_run_path_noalloc(xsh_handle* h, int npar, int j, int nc, const double* a0_dev, double* a_out_dev,
+                         double* av_dev, void* stream);
+
 /* All ensembles(2) x (1+checks) paths of parallel ensemble npar in one
  * launch (xensemble.m:25-80), asynchronously on the handle's stream.  The
  * averages stay in device memory until xsh_read_averages. */

@@ -23,6 +23,7 @@ STATUS = {0: "XSH_OK", 1: "XSH_EINVAL", 2: "XSH_EUNSUPPORTED", 3: "XSH_ENOMEM", 
           5: "XSH_ESTATE"}
 
 EXPORTS = ["xsh_version", "xsh_last_error", "xsh_create", "xsh_destroy", "xsh_query", "xsh_run_path",
+           "xsh_run_path_noalloc",
            "xsh_run_ensemble", "xsh_synchronize", "xsh_read_averages", "xsh_read_trajectories", "xsh_read_raw",
            "xsh_probe_normals", "xsh_probe_propagate", "xsh_mlp_create", "xsh_mlp_destroy",
            "xsh_mlp_last_error", "xsh_mlp_get_params", "xsh_mlp_forward", "xsh_mlp_loss_grad",
@@ -106,6 +107,8 @@ def load():
     lib.xsh_destroy.restype = None
     lib.xsh_query.argtypes = [C.c_void_p, C.POINTER(Info)]
     lib.xsh_run_path.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, dp]
+    lib.xsh_run_path_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
     lib.xsh_run_ensemble.argtypes = [C.c_void_p, C.c_int]
     lib.xsh_synchronize.argtypes = [C.c_void_p, dp]
     lib.xsh_read_averages.argtypes = [C.c_void_p, dp]

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -869,6 +870,133 @@ extern "C" int xsh_run_path(xsh_handle* h, int npar, int j, int nc, const double
   if (rc) return rc;
   if (a_out) HIPCHK(hipMemcpyAsync(a_out, h->d_a, fbytes, hipMemcpyDeviceToHost, h->stream));
   return copy_averages(h, h->d_path_av, 1, av_out, h->stream);
+}
+
+// ---------------------------------------------------------------------------
+// xsh_run_path_noalloc: xsh_run_path for callers that cannot allocate device
+// memory.  Its scratch - the kernel arguments, the averages as the kernels
+// accumulate them, the streaming kernel's field planes and the k-space noise
+// planes - lives in one file-scope device array of XSH_SCRATCH_BYTES, which
+// every device the library runs on holds until the process ends.  Host code
+// reaches it through the address hipGetSymbolAddress returns for the current
+// device (kept per device), never through the array's name.
+__device__ __align__(256) unsigned char scratch[XSH_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices];
+
+// address of this device's scratch array (looked up once per device)
+int scratch_address(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(XSH_EDEVICE, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(XSH_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch[dev];
+  return XSH_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+// observable blocks of one path: offsets and site counts
+struct AvLayout {
+  long long off[XSH_MAX_OBS];
+  int ns[XSH_MAX_OBS];
+  int nobs, npts;
+};
+
+// one path's averages, kernel order [points][sites] -> p.d.av order (time fastest)
+__global__ void av_reorder(const double* __restrict__ src, double* __restrict__ dst, AvLayout lay, long long avlen) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= avlen) return;
+  int o = 0;
+  while (o + 1 < lay.nobs && i >= lay.off[o + 1]) ++o;
+  const long long r = i - lay.off[o];
+  const long long n = r / lay.ns[o], st = r - n * lay.ns[o];
+  const long long j = lay.off[o] + n + (long long)lay.npts * st;
+  dst[2 * j] = src[2 * i];
+  dst[2 * j + 1] = src[2 * i + 1];
+}
+}  // namespace
+
+extern "C" int xsh_run_path_noalloc(xsh_handle* h, int npar, int j, int nc, const double* a0_dev, double* a_out_dev,
+                                    double* av_dev, void* stream) {
+  if (!h) return fail(XSH_EINVAL, "handle is NULL");
+  if (npar < 1 || npar > h->e3) return fail(XSH_EINVAL, "npar %d outside 1..%d", npar, h->e3);
+  if (j < 1 || j > h->e2) return fail(XSH_EINVAL, "j %d outside 1..%d", j, h->e2);
+  if (nc < 1 || nc > h->nchk) return fail(XSH_EINVAL, "nc %d outside 1..%d", nc, h->nchk);
+  if (!av_dev) return fail(XSH_EINVAL, "av_dev is NULL");
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != h->device) return fail(XSH_ESTATE, "handle belongs to device %d, current device is %d", h->device, dev);
+  // scratch layout: kernel arguments | averages (kernel order) | streaming planes | k-noise planes
+  const size_t args_bytes = align256(sizeof(xsh::PathArgs));
+  const size_t av_bytes = align256((size_t)h->avlen * 16);
+  const int items = h->groups;
+  int grid = items;
+  size_t work_bytes = 0;
+  if (h->kern && h->kern->stream) {
+    const size_t per_block = (size_t)h->work_planes * h->ns * h->esize;
+    const size_t room = XSH_SCRATCH_BYTES > args_bytes + av_bytes ? XSH_SCRATCH_BYTES - args_bytes - av_bytes : 0;
+    grid = (int)std::min<size_t>({(size_t)items, (size_t)xsh::kStreamMaxBlocks, room / per_block});
+    if (grid < 1)
+      return fail(XSH_EINVAL, "lattice too large for the %d-byte scratch (one workgroup needs %zu bytes)",
+                  (int)XSH_SCRATCH_BYTES, args_bytes + av_bytes + per_block);
+    work_bytes = align256((size_t)grid * per_block);
+  }
+  const size_t kn_bytes = h->d.knoises ? align256((size_t)items * h->tpb * h->ns * h->esize) : 0;
+  const size_t need = args_bytes + av_bytes + work_bytes + kn_bytes;
+  if (need > (size_t)XSH_SCRATCH_BYTES)
+    return fail(XSH_EINVAL, "path needs %zu bytes of scratch, more than the %d available (averages %zu, "
+                            "k-space noise planes %zu)", need, (int)XSH_SCRATCH_BYTES, av_bytes, kn_bytes);
+  unsigned char* base = nullptr;
+  int rc = scratch_address(dev, &base);
+  if (rc) return rc;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
+  xsh::PathArgs* d_args = reinterpret_cast<xsh::PathArgs*>(base);
+  double* d_av = reinterpret_cast<double*>(base + args_bytes);
+  xsh::PathArgs A = make_args(h, npar);
+  A.j_only = j;
+  A.nc_only = nc;
+  A.av = d_av;
+  A.traj = nullptr;  // per-trajectory outputs belong to xsh_run_ensemble
+  A.raw = nullptr;
+  A.nmodes = 0;
+  A.a0 = a0_dev;
+  A.a_out = a_out_dev;
+  A.work = work_bytes ? base + args_bytes + av_bytes : nullptr;
+  A.kn_work = kn_bytes ? base + args_bytes + av_bytes + work_bytes : nullptr;
+  // arguments through the handle's pinned staging copy (reused once its last upload is done)
+  if (h->args_pending) HIPCHK(hipEventSynchronize(h->ev_args));
+  *h->h_args = A;
+  HIPCHK(hipMemcpyAsync(d_args, h->h_args, sizeof(xsh::PathArgs), hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(h->ev_args, s));
+  h->args_pending = true;
+  // the averages accumulate with atomics: clear them (a fill kernel, no memset node)
+  const long long nd = h->avlen * 2;
+  hipLaunchKernelGGL(zero_doubles, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, d_av, nd);
+  HIPCHK(hipGetLastError());
+  hipError_t e;
+  if (h->nd == 1)
+    e = xsh::launch_sde_path(h->d.precision, d_args, items, s);
+  else
+    e = h->kern->launch(d_args, nullptr, 1, items, grid, s);
+  if (e != hipSuccess) return fail(XSH_EDEVICE, "kernel launch: %s", hipGetErrorString(e));
+  AvLayout lay{};
+  lay.nobs = h->d.nobserve;
+  lay.npts = h->d.points;
+  for (int o = 0; o < h->d.nobserve; ++o) {
+    lay.off[o] = h->obs_off[o];
+    lay.ns[o] = h->obs_ns[o];
+  }
+  hipLaunchKernelGGL(av_reorder, dim3((unsigned)((h->avlen + 255) / 256)), dim3(256), 0, s, d_av, av_dev, lay, h->avlen);
+  HIPCHK(hipGetLastError());
+  return XSH_OK;
 }
 
 extern "C" int xsh_run_ensemble(xsh_handle* h, int npar) {
