@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--boundaries", default="periodic", choices=["periodic", "neumann"],
                     help="periodic (FFT) or Neumann-Neumann (DCT-I) lattice")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion measurement")
+    ap.add_argument("--observe", default=None,
+                    help="comma-separated observe forms replacing the workload's (e.g. abs2_ave): "
+                         "traffic attribution runs")
     ap.add_argument("--workload", default="cgl2d", choices=["cgl2d", "nls1d", "tw", "batch", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
@@ -138,6 +141,8 @@ def main():
         p = cgl2d(args.precision)
         if args.boundaries == "neumann":
             p["boundaries"] = [[-1, -1], [-1, -1]]
+    if args.observe:
+        p["gpu"] = dict(p["gpu"], observe=args.observe.split(","))
     log("bench: creating problem %s %s %s" % (p["name"], p["points"], p["gpu"]["precision"]))
     prob = Problem(p)
     info = prob.info
@@ -163,6 +168,8 @@ def main():
     flops = traj_steps * step_flops(ns) / (kavg / 1e3)
     kernel = ("spde_path", "spde_stream", "spde_path", "sde_path")[info.kernel_kind]
     wl = "%s-%s-%s" % ("nls1d256" if args.workload == "nls1d" else "cgl2d128", args.precision, args.boundaries)
+    if args.observe:
+        wl += "-" + args.observe.replace(",", "+")
     # a sanity read of the moments so the run is checked, outside the timed region
     av = prob.handle.read_averages()
     import numpy as np
