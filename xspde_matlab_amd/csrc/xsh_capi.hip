@@ -474,24 +474,26 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   }
   h->tab_off[10] = total;
   total += ((twh.size() * h->esize + 255) / 256) * 256;
-  // FP32 real matrices with N % 32 == 0: the same entries in the
-  // operand order of the 32x32x2 matrix-core instruction, Tp[kb][kk/2][lane] =
-  // T[32 kb + lane%32][kk + lane/32] (used by kernels that run those passes on MFMA)
+  // Real matrices in the operand order of the matrix-core instruction of the
+  // precision (used by kernels that run those passes on MFMA):
+  //   FP32, N % 32 == 0, 32x32x2: Tp[kb][kk/2][lane] = T[32 kb + lane%32][kk + lane/32]
+  //   FP64, N % 16 == 0, 16x16x4: Tp[kb][kk/4][lane] = T[16 kb + lane%16][kk + lane/16]
   std::vector<std::vector<double>> mpk(4);
+  const int tile = d.precision == XSH_FP64 ? 16 : 32, kstep = 64 / tile;
   for (int i = 0; i < 4; ++i) {
     const int k = i % 2, n = h->npad[k];
-    h->has_tmp[i] = h->mat && k < d.dimensions - 1 && d.precision == XSH_FP32 && !cplx[k] && n % 32 == 0 &&
-                    mats[i].size() == (size_t)n * n;
+    h->has_tmp[i] = h->mat && k < d.dimensions - 1 && !cplx[k] && n % tile == 0 && mats[i].size() == (size_t)n * n;
     if (!h->has_tmp[i]) continue;
     mpk[i].resize((size_t)n * n);
-    for (int kb = 0; kb < n / 32; ++kb)
-      for (int kk = 0; kk < n; kk += 2)
+    for (int kb = 0; kb < n / tile; ++kb)
+      for (int kk = 0; kk < n; kk += kstep)
         for (int lane = 0; lane < 64; ++lane)
-          mpk[i][((size_t)kb * (n / 2) + kk / 2) * 64 + lane] = mats[i][(size_t)(32 * kb + lane % 32) * n + kk + lane / 32];
+          mpk[i][((size_t)kb * (n / kstep) + kk / kstep) * 64 + lane] =
+              mats[i][(size_t)(tile * kb + lane % tile) * n + kk + lane / tile];
   }
   for (int i = 0; i < 4; ++i) {
     h->tab_off[11 + i] = total;
-    total += ((mpk[i].size() * sizeof(float) + 255) / 256) * 256;
+    total += ((mpk[i].size() * (h->esize / 2) + 255) / 256) * 256;
   }
   // Boundary values (the boundval constants, or the caller's boundfun table per
   // time level) and the patch of xpatch.m:24-62: pa(x, y) = sum over the
@@ -588,8 +590,13 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     pack<double>(host, h->tab_off[10], twh);
   else
     pack<float>(host, h->tab_off[10], twh);
-  for (int i = 0; i < 4; ++i)
-    if (h->has_tmp[i]) pack_real<float>(host, h->tab_off[11 + i], mpk[i]);
+  for (int i = 0; i < 4; ++i) {
+    if (!h->has_tmp[i]) continue;
+    if (d.precision == XSH_FP64)
+      pack_real<double>(host, h->tab_off[11 + i], mpk[i]);
+    else
+      pack_real<float>(host, h->tab_off[11 + i], mpk[i]);
+  }
   if (d.precision == XSH_FP64) {
     pack_real<double>(host, h->tab_off[15], bshape);
     pack<double>(host, h->tab_off[16], bvals);

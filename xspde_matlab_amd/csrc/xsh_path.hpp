@@ -58,8 +58,9 @@ struct PathArgs {
   int mat;                 // 1: dense per-dimension transforms (non-periodic lattice)
   int cplx[2];             // per dimension: 1 if its matrix is complex (periodic DFT)
   const void* tm[4];       // forward x, forward y, inverse x, inverse y (row-major N x N)
-  const void* tmp[4];      // FP32 real matrices with N % 32 == 0 in MFMA operand order
-                           // Tp[kb][kk/2][lane] = T[32 kb + lane%32][kk + lane/32], else NULL
+  const void* tmp[4];      // real matrices in MFMA operand order, else NULL: FP32 (N % 32 == 0)
+                           // Tp[kb][kk/2][lane] = T[32 kb + lane%32][kk + lane/32],
+                           // FP64 (N % 16 == 0) Tp[kb][kk/4][lane] = T[16 kb + lane%16][kk + lane/16]
   int dirichlet;           // any Dirichlet end (xsetbound.m:40-53)
   double* traj;            // optional [paths][E][npts][nobs] per-trajectory lattice means
   double* raw;             // optional [paths][npts][E][ns] complex double fields
@@ -444,11 +445,69 @@ struct Lattice {
     });
   }
 
+  // FP64 real transforms on the matrix cores: the same GEMM on
+  // v_mfma_f64_16x16x4f64 (lane l supplies A[l%16][l/16] and B[l/16][l%16] and
+  // receives D[4 r + l/16][l%16] in register r), T in the packed operand
+  // order Tp[kb][kk/4][lane] = T[16 kb + lane%16][kk + lane/16] (one coalesced
+  // 512 B load per K step), each wave owning PW pairs of 16x16 tiles (real and
+  // imaginary columns of the same 16 lines, one 16-row block of k).
+  template <int DIM>
+  struct MfmaPlan64 {
+    static constexpr int N = Line<DIM>::N, NL = Line<DIM>::NL, NW = NT / 64;
+    static constexpr int KB = N / 16, LB = NL / 16, P = KB * LB;
+    static constexpr int PW = P >= NW && P % NW == 0 ? P / NW : 0;
+    static constexpr bool ok = std::is_same<R, double>::value && (NY > 1 ? TPB == 1 : DIM == 0) &&
+                               N % 16 == 0 && NL % 16 == 0 && PW > 0 && LB % PW == 0;
+  };
+  template <int DIM, class Op>
+  __device__ static __forceinline__ void mat_mfma64(C* L, const double* __restrict__ Tp, const Op& op) {
+    using Pl = MfmaPlan64<DIM>;
+    constexpr int N = Pl::N, PW = Pl::PW, LB = Pl::LB;
+    typedef double v4d __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63;
+    const int p0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * PW;
+    const int kb = p0 / LB;
+    v4d acc[2 * PW];
+    static_for<0, 2 * PW>([&](auto T) __attribute__((always_inline)) {
+      acc[decltype(T)::value] = v4d{0.0, 0.0, 0.0, 0.0};
+    });
+    const double* __restrict__ Ta = Tp + (size_t)kb * (N / 4) * 64 + lane;
+#pragma unroll 4
+    for (int kk = 0; kk < N; kk += 4) {
+      const double a = Ta[(kk / 4) * 64];
+      const int n = kk + (lane >> 4);
+      static_for<0, PW>([&](auto T) __attribute__((always_inline)) {
+        constexpr int t = decltype(T)::value;
+        const int ln = ((p0 + t) % LB) * 16 + (lane & 15);
+        const C x = L[DIM == 0 ? ln * RS + n : n * RS + ln];
+        acc[2 * t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, x.x, acc[2 * t], 0, 0, 0);
+        acc[2 * t + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, x.y, acc[2 * t + 1], 0, 0, 0);
+      });
+    }
+    __syncthreads();
+    static_for<0, PW>([&](auto T) __attribute__((always_inline)) {
+      constexpr int t = decltype(T)::value;
+      const int ln = ((p0 + t) % LB) * 16 + (lane & 15);
+      static_for<0, 4>([&](auto Rg) __attribute__((always_inline)) {
+        constexpr int r = decltype(Rg)::value;
+        const int k = kb * 16 + 4 * r + (lane >> 4);  // D[4 r + lane / 16][lane % 16]
+        const C X = op(C{acc[2 * t][r], acc[2 * t + 1][r]}, ln, k);
+        L[DIM == 0 ? ln * RS + k : k * RS + ln] = X;
+      });
+    });
+  }
+
   template <int DIM, class Op>
   __device__ static __forceinline__ void mat_any(C* L, const void* T, const void* Tp, bool cplx, const Op& op) {
     if constexpr (MfmaPlan<DIM>::ok) {
       if (!cplx && Tp) {
         mat_mfma<DIM>(L, static_cast<const float*>(Tp), op);
+        return;
+      }
+    }
+    if constexpr (MfmaPlan64<DIM>::ok) {
+      if (!cplx && Tp) {
+        mat_mfma64<DIM>(L, static_cast<const double*>(Tp), op);
         return;
       }
     }
