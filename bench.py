@@ -36,6 +36,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
 VALU_PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}   # FLOP/s, MI355X vector
+MFMA_PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}   # FLOP/s, MI355X dense matrix cores
 
 
 def step_flops(ns, transforms=8, drift_evals=4, c_drift=20):
@@ -46,16 +47,26 @@ def step_flops(ns, transforms=8, drift_evals=4, c_drift=20):
     return transforms * 5 * ns * math.log2(ns) + 4 * 6 * ns + drift_evals * c_drift * ns + 10 * ns
 
 
+def dense_flops(lattice, transforms=8):
+    """Flops of the dense real transform passes of one RK4IP trajectory-step on
+    a non-periodic lattice: per transform and dimension d, a real N_d x N_d
+    matrix times the complex plane, 4 N_d ns flops."""
+    ns = 1
+    for n in lattice:
+        ns *= n
+    return transforms * sum(4 * n * ns for n in lattice)
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if it was
     taken for this kernel and workload (else None)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            rec = json.load(f).get(kernel)
+            rec = json.load(f).get(workload)
     except (OSError, ValueError):
         return None
-    if not rec or rec.get("workload") != workload:
+    if not rec or kernel not in rec.get("kernel", ""):
         return None
     return rec.get("bytes_per_launch")
 
@@ -211,6 +222,13 @@ def main():
                               "flops_per_traj_step": step_flops(ns)}},
         "cpu_baseline": cpu,
     }
+    if args.boundaries == "neumann":
+        lat = config["lattice"]
+        dfl = traj_steps * dense_flops(lat) / (kavg / 1e3)
+        line["roofline"]["mfma"] = {"achieved": dfl / 1e12, "peak": MFMA_PEAK[args.precision] / 1e12,
+                                    "unit": "TFLOP/s", "frac": dfl / MFMA_PEAK[args.precision],
+                                    "flops_per_traj_step": dense_flops(lat),
+                                    "what": "dense DCT-I passes on the matrix cores"}
     if fp64 is not None:
         fp64["fp32_over_fp64"] = rate / fp64["value"]
         line["fp64"] = fp64
@@ -242,7 +260,10 @@ def fp64_companion(boundaries, warmup=1, launches=2):
     kavg = sum(kms) / len(kms)
     return {"value": info.traj_steps / wall, "unit": "trajectory-steps/s", "dtype": "f64", "launches": launches,
             "warmup": warmup, "ms_per_step": 1e3 * wall, "kernel": kernel, "kernel_ms": kavg,
-            "hbm_frac": info.traj_steps * 2 * info.nsites * 16 / (kavg / 1e3) / HBM_PEAK}
+            "hbm_frac": info.traj_steps * 2 * info.nsites * 16 / (kavg / 1e3) / HBM_PEAK,
+            "traffic": measured_traffic(kernel, "cgl2d128-fp64-%s" % boundaries),
+            "mfma_frac": (info.traj_steps * dense_flops([128, 128]) / (kavg / 1e3) / MFMA_PEAK["fp64"]
+                          if boundaries == "neumann" else None)}
 
 
 def copy_calibration(n=64 * 1024 * 1024, reps=3):
