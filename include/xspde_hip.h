@@ -115,7 +115,7 @@ typedef struct {
                                (p.boundval, spdpreferences.m:26-43): the field at
                                Dirichlet ends, its derivative at Neumann ends     */
   int raw;              /* 1: keep every trajectory's field at every stored point
-                           (p.raw, xpath.m:73-78), read with xsh_read_raw      */
+                           (p.rawdata, xpath.m:73-78), read with xsh_read_raw  */
   /* Boundary values that change in time or along the boundary (p.boundfun,
      called by xpatch.m:37 and xsetbound.m:39 at the method's p.t), tabulated by
      the caller: boundtab[level][dim][lower,upper][b][re,im] with
@@ -221,8 +221,8 @@ int xsh_read_averages(xsh_handle* h, double* av_out);
  *        Fourier modes of the initial noise, scaled to unit variance (NULL to skip) */
 int xsh_read_trajectories(xsh_handle* h, double* obs, float* feat);
 
-/* Raw trajectories of the last xsh_run_ensemble (p.raw, xpath.m:73-78, stored by
- * the reference in raw{j}{nc,np}): one block per path in the order of
+/* Raw trajectories of the last xsh_run_ensemble (p.rawdata, xpath.m:73-78, stored
+ * by the reference in raw{s,nc,j}, xensemble.m:79): one block per path in the order of
  * xsh_read_averages, each p.d.raw = [1, points, nx, ny, ensembles(1)] column-major
  * (time fastest, then x, y, trajectory); the descriptor must set raw = 1. */
 int xsh_read_raw(xsh_handle* h, double* out);

@@ -111,7 +111,7 @@ def test_kubo_mean_within_error_bars():
     """<Re a>(t) = exp(-t/2) (Kubo.m:19) within 4 sampling error bars, 4096 trajectories."""
     from xspde_matlab_amd import xsim
     p = kubo("fp64")
-    err, data, q = xsim(p)
+    err, data, q, _ = xsim(p)
     t = np.linspace(0, 2, q["points"][0])
     mean, step, sig = data[0][:, 0], data[0][:, 1], data[0][:, 2]
     dev = np.abs(mean - np.exp(-t / 2))
@@ -130,7 +130,7 @@ def test_ensemble_matches_oracle_fp64(order, e2, e3):
     p = shrink(nls1d("fp64", nx=64), E=8, e2=e2, points=4, steps=2)
     p["ensembles"][2] = e3
     p["order"] = order
-    err_g, data_g, q = xsim(p)
+    err_g, data_g, q, _ = xsim(p)
     err_r, outd, qo = X.xsim(oracle_problem(p))
     sx = max(3, 1 + qo.nchk)
     for o in range(len(outd)):

@@ -32,6 +32,14 @@ info = xsh_mex('query', h);
 xsh_mex('run_ensemble', h, npar);
 avall = xsh_mex('read_averages', h);             % [avlen, ensembles(2)*nchk]
 raw = cell(1,p.nchk,p.ensembles(2));
+if p.rawdata                                     % raw{s,nc,j} of xensemble.m:79
+  rawall = xsh_mex('read_raw', h);               % [prod(p.d.raw{1}), ensembles(2)*nchk]
+  for j = 1:p.ensembles(2)
+    for nc = 1:p.nchk
+      raw{1,nc,j} = {reshape(rawall(:,(j-1)*p.nchk+nc), p.d.raw{1})};
+    end
+  end
+end
 ne = 1./(p.ensembles(2)*p.ensembles(3)) - 1;     % ensemble scaling factor
 m  = p.fbsw+p.nchk;
 m1 = max(3,m+1);

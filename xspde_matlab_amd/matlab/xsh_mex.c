@@ -249,5 +249,14 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     if (rc) fail_status(rc, "xsh_read_averages");
     return;
   }
+  if (!strcmp(cmd, "read_raw")) {
+    /* one column per path as read_averages; each column reshapes to
+       p.d.raw{1} = [1, points, nx, ny, ensembles(1)] */
+    const mwSize len = (mwSize)info.points * info.nsites * (mwSize)info.ensembles[0];
+    plhs[0] = mxCreateDoubleMatrix(len, (mwSize)info.ensembles[1] * info.nchk, mxCOMPLEX);
+    rc = xsh_read_raw(h, (double*)mxGetComplexDoubles(plhs[0]));
+    if (rc) fail_status(rc, "xsh_read_raw");
+    return;
+  }
   mexErrMsgIdAndTxt("xSPDE:gpu:invalid", "unknown command '%s'", cmd);
 }

@@ -10,7 +10,7 @@ names the built-in forms the GPU integrates::
              gpu=dict(deriv='gp', g=-(1 + 0.5j), u=0, s=0.05 / 2 ** 0.5,
                       linear=dict(l0=1, l2=1 + 1j), initial='phase', amp=0.1,
                       observe=['abs2_ave', 'corr'], precision='fp32'))
-    error, data, p = xsim(p)
+    error, data, p, raw = xsim(p)
 
 Names follow the reference: ``xpath`` is the p.path contract
 (Simulator/xpath.m:1), ``xensemble`` the sub-ensemble/check loops with
@@ -91,6 +91,9 @@ def preferences(p):
     q["order"] = int(q.get("order", 0))
     q["seed"] = int(q.get("seed", 0))
     q["dt"] = q["ranges"][0] / max(1, pts[0] - 1)                   # :174
+    q["rawdata"] = int(bool(q.get("rawdata", q.get("raw", 0))))      # :85
+    q["file"] = q.get("file") or " "                                 # :90
+    q["version"] = q.get("version") or "xSPDE4.21"                   # :19
     return q
 
 
@@ -176,7 +179,7 @@ def make_desc(q):
     d.seq = int(g.get("seq", 0))
     d.inst = int(g.get("inst", 0))
     d.traj_out = 1 if g.get("traj_out") else 0
-    d.raw = 1 if q.get("raw") else 0
+    d.raw = 1 if q.get("rawdata", q.get("raw")) else 0
     modes = g.get("modes")
     if modes:
         if isinstance(modes, int):
@@ -318,6 +321,50 @@ def split_averages(q, info, av):
     return out
 
 
+def space_dims(q):
+    """Lattice sizes of the space dimensions, x first (p.points(2:end))."""
+    return [int(n) for n in q["points"][1:q["dimensions"]]]
+
+
+def _unflatten_sites(x, axis, dims):
+    """Split axis (sites, x fastest) of x into the space dimensions."""
+    if len(dims) < 2:
+        return x
+    k = len(dims)
+    x = x.reshape(x.shape[:axis] + tuple(reversed(dims)) + x.shape[axis + 1:])   # C order: x last
+    perm = list(range(axis)) + list(reversed(range(axis, axis + k))) + list(range(axis + k, x.ndim))
+    return np.transpose(x, perm)
+
+
+def matlab_data(q, info, outputs, outd):
+    """Per output, the [ndata, 3] statistics as the array xsim.m:150-157 stores:
+    the output's own shape with space dimensions unflattened when it is a
+    per-site observable ([lines, points, space..., 3]), the mean, step error and
+    sampling error along the last axis."""
+    zero = split_averages(q, info, np.zeros(info.avlen, dtype=complex))
+    dims = space_dims(q)
+    ns = int(np.prod(dims)) if dims else 1
+    res = []
+    for o, out in enumerate(outputs):
+        shape = np.asarray(out(zero, q)).shape
+        arr = np.asarray(outd[o]).reshape(tuple(shape) + (3,))
+        if len(shape) == 3 and shape[2] == ns and ns > 1:
+            arr = _unflatten_sites(arr, 2, dims)
+        res.append(arr)
+    return res
+
+
+def matlab_raw(q, r):
+    """One path's raw fields from read_raw ([points, E, sites]) as p.d.raw:
+    [1, points, space..., E] (xpreferences.m:148, xpath.m:73-78)."""
+    dims = space_dims(q)
+    x = np.transpose(r, (0, 2, 1))                                   # [points, sites, E]
+    x = _unflatten_sites(x, 1, dims) if len(dims) > 1 else x
+    if not dims:
+        x = x[:, 0, :]
+    return x[None]
+
+
 class Problem:
     """A GPU handle plus the host-side bookkeeping of one xSPDE project."""
 
@@ -338,11 +385,18 @@ class Problem:
             return split_averages(self.q, self.info, r[0]), r[1]
         return split_averages(self.q, self.info, r)
 
-    def xensemble(self, npar):
-        """All sub-ensembles and checks of one parallel ensemble (xensemble.m:25-105)."""
+    def xensemble(self, npar, want_raw=False):
+        """All sub-ensembles and checks of one parallel ensemble (xensemble.m:25-105).
+        With want_raw (needs p.rawdata) also returns raw[nc][j], each path's
+        fields as p.d.raw."""
         self.handle.run_ensemble(npar)
         self.handle.synchronize()
-        return ensemble_moments(self.q, self.info, self.outputs, self.handle.read_averages())
+        data = ensemble_moments(self.q, self.info, self.outputs, self.handle.read_averages())
+        if not want_raw:
+            return data
+        r = self.handle.read_raw()
+        raw = [[matlab_raw(self.q, r[j, nc]) for j in range(r.shape[0])] for nc in range(r.shape[1])]
+        return data, raw
 
     def close(self):
         self.handle.close()
@@ -419,19 +473,33 @@ def statistics(q, data):
 
 
 def xsim(p):
-    """[error, data, p] = xsim(p) on the GPU path (xsim.m:28-188, no comparisons)."""
+    """[error, data, input, raw] = xsim(p) on the GPU path (xsim.m:28-188, no
+    comparisons).  data[o] is [ndata, 3] (mean, step error, sampling error);
+    raw is raw[nc][j] with j over all ensembles(2) x ensembles(3) paths
+    (xsim.m:42,56-57), or None without p.rawdata.  A non-blank p.file saves
+    error, data, input and raw to that '.mat' file (xsim.m:207-210)."""
     t0 = time.time()
     prob = Problem(p)
+    q = prob.q
+    raw = [[] for _ in range(1 + q["checks"][0])] if q["rawdata"] else None
     try:
         data = None
-        for npar in range(1, prob.q["ensembles"][2] + 1):          # parfor == serial (xsim.rst:275)
-            dp = prob.xensemble(npar)
+        for npar in range(1, q["ensembles"][2] + 1):               # parfor == serial (xsim.rst:275)
+            if raw is not None:
+                dp, rp = prob.xensemble(npar, want_raw=True)
+                for nc in range(len(raw)):
+                    raw[nc].extend(rp[nc])                          # j + e2 (npar - 1)
+            else:
+                dp = prob.xensemble(npar)
             data = dp if data is None else [a + b for a, b in zip(data, dp)]
+        error, outd = statistics(q, data)
+        error[-1] = time.time() - t0
+        if q["file"].strip():
+            from .xfile import xsave
+            xsave(q["file"], error, matlab_data(q, prob.info, prob.outputs, outd), q, raw)
     finally:
         prob.close()
-    error, outd = statistics(prob.q, data)
-    error[-1] = time.time() - t0
-    return error, outd, prob.q
+    return error, outd, q, raw
 
 
 class BatchProblem:
