@@ -102,8 +102,10 @@ typedef struct {
   double lin_l2[2][2];
   int has_linear;       /* 0: propagator is 1 (xpropfactor.m:27,37)        */
   int initial;          /* XSH_INIT_*                                        */
-  double init_par[6];   /* CONST: c.re c.im; SECH: amp width eps; GAUSS: amp width;
-                           PHASE: amp; COHERENT: alpha.re alpha.im            */
+  double init_par[6];   /* CONST: c.re c.im; SECH: amp width eps center
+                           (amp sech((x - center)/width)); GAUSS: amp width
+                           width_y (amp exp(-x^2/2w^2 - y^2/2w_y^2), width_y 0:
+                           = width); PHASE: amp; COHERENT: alpha.re alpha.im  */
   int nobserve;
   int observe[XSH_MAX_OBS];
   int seq;              /* sequence index (random stream key)                */

@@ -17,8 +17,10 @@ deriv    'zero'                        D = 0
                                        D = g |a|^2 a + u a + s w0            [noises == 1]
 linear   l0, l1[d], l2[d]              L = l0 + sum_d l1[d] D_d + l2[d] D_d^2  (xpropfactor.m:37-39)
 initial  'const'  c0                   a = c0
-         'sech'   amp, width, eps      a = amp sech(x/width) + eps (v0 + i v1)/sqrt2
-         'gauss'  amp, width           a = amp exp(-(x^2+y^2)/(2 width^2))
+         'sech'   amp, width, eps, center
+                                       a = amp sech((x-center)/width) + eps (v0 + i v1)/sqrt2
+         'gauss'  amp, width, width_y  a = amp exp(-x^2/(2 width^2) - y^2/(2 width_y^2))
+                                       (width_y defaults to width)
          'phase'  amp                  a = amp exp(2 pi i u0),  u0 = uniform(0,1)
          'coherent' alpha              a = alpha + (v0 + i v1)/2  (Wigner vacuum)
 observe  'a', 'abs2', 'a_ave', 'abs2_ave', 'abs2_int', 'abs4_ave', 'corr'
@@ -89,9 +91,10 @@ def make_initial(g):
         amp = float(g.get("amp", 1.0))
         wd = float(g.get("width", 1.0))
         eps = float(g.get("eps", 0.0))
+        x0 = float(g.get("center", 0.0))
 
         def initial(v, p):
-            a = amp / np.cosh(p.x / wd) + 0j
+            a = amp / np.cosh((p.x - x0) / wd) + 0j
             if eps != 0.0:
                 a = a + eps * (v[0:1] + 1j * v[1:2]) / np.sqrt(2.0)
             return a
@@ -99,12 +102,13 @@ def make_initial(g):
     if ini == "gauss":
         amp = float(g.get("amp", 1.0))
         wd = float(g.get("width", 1.0))
+        wy = float(g.get("width_y", 0.0)) or wd
 
         def initial(v, p):
-            r2 = p.x ** 2
+            r2 = p.x ** 2 / wd ** 2
             if p.dimensions > 2:
-                r2 = r2 + p.y ** 2
-            return amp * np.exp(-0.5 * r2 / wd ** 2) + 0j
+                r2 = r2 + p.y ** 2 / wy ** 2
+            return amp * np.exp(-0.5 * r2) + 0j
         return initial
     if ini == "phase":
         amp = float(g.get("amp", 1.0))

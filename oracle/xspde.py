@@ -737,11 +737,16 @@ def Ave(o, p):
     return np.mean(o, axis=tuple(range(1, p.dimensions)), keepdims=False)
 
 
-def Int(o, p):
-    """Lattice integral sum * dx over space axes (Int.m, no limits)."""
+def Int(o, *args):
+    """Lattice integral over the space axes (Applications/SPDE/Int.m:23-61, no
+    limits): Int(o, p) uses the steps p.dx, Int(o, dx, p) the given steps (p.dk
+    for transformed fields); a space dimension with step 0 is not integrated."""
+    p = args[-1]
+    dx = p.dx if len(args) == 1 else np.asarray(args[0], float)
     r = o
-    for d in range(p.dimensions - 1, 0, -1):
-        r = np.sum(r, axis=d) * p.dx[d]
+    for d in range(p.dimensions - 1, 0, -1):       # last axis first: lower indices stay valid
+        if dx[d] > 0:
+            r = np.sum(r, axis=d) * dx[d]
     return r
 
 

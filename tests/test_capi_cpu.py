@@ -54,7 +54,7 @@ def test_unsupported_forms_rejected():
     rc, msg = _create(d)
     assert rc == 2 and "drift" in msg
     d = make_desc(preferences(configs.shrink(configs.cgl2d(n=32), E=2, e2=1, points=3)))
-    d.nx = 65    # odd 2D lattice above the padded 64 x 64 kernels
+    d.nx = 129   # odd 2D lattice above the padded 128 x 64 kernels
     rc, msg = _create(d)
     assert rc == 2 and "odd" in msg
     d = make_desc(q)

@@ -491,17 +491,38 @@ _NLS_GPU = dict(deriv="gp", g=1j, s=0.0, linear=dict(l0=-0.5j, l2=0.5j), initial
                 eps=0.0, observe=["a"], precision="fp64")
 
 
-@pytest.mark.parametrize("row", ["Soliton#4", "SolitonAv#14", "GaussianAv2D#16", "SolitonDerivN#21"])
+@pytest.mark.parametrize("row", ["Soliton#4", "Characteristic#8", "SolitonAv#14", "GaussianAv2D#16",
+                                 "GaussianKinetic#19", "SolitonDerivN#21"])
 def test_batchtest_known_answers_on_gpu(row):
     """Deterministic Batchtest rows run on the GPU path (FP64), tolerance 0.1 %
     (Batchtest.m:67-81): Soliton #4 (35 periodic sites: padded DFT, RK4, no
-    checks), SolitonAv #14 (lattice average and integral of Re a, RK4 with
-    checks), GaussianAv2D #16 (35 x 35 periodic sites padded to the 64 x 64
-    dense kernel, k-space observable, xcompress axes), SolitonDerivN #21 (101
-    sites, Neumann DCT-I padded to 128, order-4 extrapolation, Int(|da/dx|^2)).  The observables are the reference's own
-    functions of the fields, evaluated on the GPU's raw fields."""
+    checks), Characteristic #8 (first-derivative propagator, shifted sech, a
+    compare at one site), SolitonAv #14 (lattice average and integral of Re a,
+    RK4 with checks), GaussianAv2D #16 (35 x 35 periodic sites padded to the
+    64 x 64 dense kernel, k-space observable, xcompress axes), GaussianKinetic
+    #19 (70 x 34 periodic sites padded to the 128 x 64 dense kernel, anisotropic
+    Gaussian, k-space integrals with dk), SolitonDerivN #21 (101 sites, Neumann
+    DCT-I padded to 128, order-4 extrapolation, Int(|da/dx|^2)).  The
+    observables are the reference's own functions of the fields, evaluated on
+    the GPU's raw fields."""
     sech = lambda x: 1 / np.cosh(x)
-    if row == "Soliton#4":                  # Examples/SPDE_2D/Soliton.m
+    if row == "Characteristic#8":           # Examples/SPDE_2D/Characteristic.m:12-22
+        p = dict(name="Characteristic", dimensions=2, points=[51, 35], noises=0, inrandoms=0,
+                 gpu=dict(deriv="zero", linear=dict(l1=-1.0), initial="sech", amp=1.0, width=0.5, center=-2.5,
+                          observe=["a"], precision="fp64"))
+        err = _raw_known_answer(p, [lambda a, p: a, lambda a, p: a ** 2],
+                                compare=[None, lambda p: sech(2 * (p.t - 2.5)) ** 2], axes=[None, [0, 18]])
+        expected = 9.654854e-05
+    elif row == "GaussianKinetic#19":       # Examples/SPDE_3D/GaussianKinetic.m:17-42
+        from test_oracle_batchtest import X0, Y0, gaussiankinetic_observe
+        p = dict(name="GaussianKinetic", dimensions=3, ranges=[5, 200, 6], points=[49, 70, 34], noises=0,
+                 inrandoms=0, gpu=dict(deriv="zero", linear=dict(l2=[1j, 0]), initial="gauss",
+                                       amp=1 / np.sqrt(np.pi * X0 * Y0), width=X0, width_y=Y0, observe=["abs2"],
+                                       precision="fp64"))
+        ob = gaussiankinetic_observe()
+        err = _raw_known_answer(p, ob["observe"], compare=ob["compare"], transforms=ob["transforms"])
+        expected = 4.049843e-06
+    elif row == "Soliton#4":                # Examples/SPDE_2D/Soliton.m
         p = dict(name="Soliton", dimensions=2, checks=0, order=2, noises=0, inrandoms=0, gpu=dict(_NLS_GPU))
         err = _raw_known_answer(p, [lambda a, p: a], compare=[lambda p: sech(p.x)])
         expected = 5.047219e-03
@@ -535,10 +556,13 @@ def test_batchtest_known_answers_on_gpu(row):
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 5e-5)])
 @pytest.mark.parametrize("shape,bnd,method", [((35, 35), ("PP", "PP"), "RK4"), ((48, 40), ("NN", "DD"), "MP"),
-                                              ((33, 64), ("DN", "PP"), "RK4"), ((64, 21), ("ND", "NN"), "MP")])
+                                              ((33, 64), ("DN", "PP"), "RK4"), ((64, 21), ("ND", "NN"), "MP"),
+                                              ((70, 34), ("PP", "PP"), "RK4"), ((34, 100), ("NN", "DD"), "MP"),
+                                              ((128, 41), ("DN", "PP"), "RK4"), ((64, 128), ("ND", "NN"), "MP")])
 def test_cgl2d_padded_lattice(prec, tol, shape, bnd, method):
-    """2D lattices of sizes without a compiled kernel (odd, unequal) on the 64 x 64
-    dense-transform kernel, padding per dimension: the counter streams and
+    """2D lattices of sizes without a compiled kernel (odd, unequal) on the 64 x 64,
+    128 x 64 and 64 x 128 dense-transform kernels, padding per dimension (and the
+    two rectangular kernels at their own sizes): the counter streams and
     outputs keep the physical site indices x + nx*y, boundary values at the
     physical ends; against the oracle on the same lattice."""
     p = shrink(cgl2d(prec, n=32), E=3, e2=2, points=3, steps=2)

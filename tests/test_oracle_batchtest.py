@@ -55,8 +55,36 @@ def solitonderivn():                # Examples/SPDE_2D/SolitonDerivN.m:8-23, Bat
                 boundaries=[[-1, -1]], **NLS)
 
 
+def _ratio(num, den):
+    with np.errstate(invalid="ignore", divide="ignore"):   # shape probes run on zero fields
+        return num / den
+
+
+Y0, X0 = 0.235, 3.3
+
+
+def gaussiankinetic_observe():
+    """Examples/SPDE_3D/GaussianKinetic.m:31-42: |a|^2 in x and k space, the
+    kinetic energy Int(|a_k|^2 (kx^2 + ky^2), dk) and three widths."""
+    ii = lambda a, p: np.abs(a) ** 2
+    return dict(
+        observe=[ii, ii, lambda a, p: X.Int(ii(a, p) * (p.kx ** 2 + p.ky ** 2), p.dk, p),
+                 lambda a, p: np.sqrt(_ratio(X.Int(p.y ** 2 * ii(a, p), p.dx, p), X.Int(ii(a, p), p.dx, p))),
+                 lambda a, p: np.sqrt(_ratio(X.Int(p.x ** 2 * ii(a, p), p.dx, p), X.Int(ii(a, p), p.dx, p))),
+                 lambda a, p: np.sqrt(_ratio(X.Int(p.ky ** 2 * ii(a, p), p.dk, p), X.Int(ii(a, p), p.dk, p)))],
+        transforms=[0, [0, 1, 1], [0, 1, 1], 0, 0, [0, 1, 1]],
+        compare=[None, None, lambda p: 1 / (2 * X0 ** 2) + 1 / (2 * Y0 ** 2), lambda p: Y0 / np.sqrt(2),
+                 None, lambda p: 1 / (np.sqrt(2) * Y0)])
+
+
+def gaussiankinetic():              # Examples/SPDE_3D/GaussianKinetic.m:17-42, Batchtest #19
+    return dict(dimensions=3, ranges=[5, 200, 6], points=[49, 70, 34],
+                initial=lambda v, p: np.sqrt(np.exp(-p.x ** 2 / X0 ** 2 - p.y ** 2 / Y0 ** 2) / (np.pi * X0 * Y0)),
+                linear=lambda p: 1j * p.Dx ** 2, **gaussiankinetic_observe())
+
+
 ROWS = [(soliton, 5.047219e-03), (characteristic, 9.654854e-05), (solitonav, 1.410144e-02),
-        (gaussianav2d, 4.813746e-07), (solitonderivn, 2.134464e-06)]
+        (gaussianav2d, 4.813746e-07), (gaussiankinetic, 4.049843e-06), (solitonderivn, 2.134464e-06)]
 
 
 @pytest.mark.parametrize("make,expected", ROWS, ids=[r[0].__name__ for r in ROWS])

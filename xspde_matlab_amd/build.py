@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libxspde_hip.so")
 # longest translation units first (they run in parallel)
-SOURCES = ["xsh_k_stream_mat.hip", "xsh_k_kn64.hip", "xsh_k_f64.hip", "xsh_k_kn.hip", "xsh_k_mat_f64.hip", "xsh_k_kn2d.hip", "xsh_k_f32_1d.hip", "xsh_k_mat_f32_1d.hip", "xsh_k_f32_var.hip",
+SOURCES = ["xsh_k_stream_mat.hip", "xsh_k_mat_pad.hip", "xsh_k_kn64.hip", "xsh_k_f64.hip", "xsh_k_kn.hip", "xsh_k_mat_f64.hip", "xsh_k_kn2d.hip", "xsh_k_f32_1d.hip", "xsh_k_mat_f32_1d.hip", "xsh_k_f32_var.hip",
            "xsh_k_mat_f32_1d256.hip", "xsh_k_mat_f32.hip", "xsh_k_f32_var2d.hip", "xsh_k_f32_2d.hip",
            "xsh_k_mat_f64_1d256.hip", "xsh_k_stream.hip", "xsh_capi.hip", "xsh_mlp.hip", "xsh_k_sde.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

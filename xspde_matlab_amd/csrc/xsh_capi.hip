@@ -130,7 +130,8 @@ static int validate(const xsh_desc* d) {
     return fail(XSH_EINVAL, "initial form needs inrandoms >= 2 (or krandoms = 2)");
   if (d->initial == XSH_INIT_PHASE && d->urandoms < 1) return fail(XSH_EINVAL, "phase initial form needs urandoms >= 1");
   if (d->initial == XSH_INIT_SECH && !(d->init_par[1] > 0)) return fail(XSH_EINVAL, "sech width must be > 0");
-  if (d->initial == XSH_INIT_GAUSS && !(d->init_par[1] > 0)) return fail(XSH_EINVAL, "gauss width must be > 0");
+  if (d->initial == XSH_INIT_GAUSS && (!(d->init_par[1] > 0) || d->init_par[2] < 0))
+    return fail(XSH_EINVAL, "gauss widths must be > 0");
   if (d->nobserve < 1 || d->nobserve > XSH_MAX_OBS) return fail(XSH_EINVAL, "nobserve must be 1..%d", XSH_MAX_OBS);
   for (int o = 0; o < d->nobserve; ++o) {
     if (d->observe[o] == XSH_OBS_LOWK && d->nmodes < 1) return fail(XSH_EINVAL, "observe form lowk needs noise modes");
@@ -147,9 +148,10 @@ static int validate(const xsh_desc* d) {
       if (t != -3 && t != -1 && t != 0 && t != 1 && t != 3)
         return fail(XSH_EINVAL, "Invalid boundary type: [%d,%d]", b0, b1);
       // sizes without a compiled kernel run padded on a dense-transform kernel
-      // (1D up to 256 sites, 2D up to 64 x 64; kernel selection reports the rest)
-      if (n[k] % 2 && d->dimensions == 3 && n[k] > 64)
-        return fail(XSH_EUNSUPPORTED, "odd lattice size %d above 64 is not on the GPU path in 2D", n[k]);
+      // (1D up to 256 sites, 2D up to 64 x 64, 128 x 64 and 64 x 128; kernel
+      // selection reports the rest)
+      if (n[k] % 2 && d->dimensions == 3 && n[k] > 128)
+        return fail(XSH_EUNSUPPORTED, "odd lattice size %d above 128 is not on the GPU path in 2D", n[k]);
       if (n[k] < 2) return fail(XSH_EINVAL, "lattice size %d: at least 2 points", n[k]);
       if (!(d->ranges[k + 1] > 0)) return fail(XSH_EINVAL, "ranges(%d) must be > 0", k + 2);
     } else if (n[k] != 1) {

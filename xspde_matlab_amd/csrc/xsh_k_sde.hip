@@ -16,13 +16,14 @@ const KernelEntry* table_kn_f64(int* n);
 const KernelEntry* table_kn_f32_2d(int* n);
 const KernelEntry* table_mat_f64(int* n);
 const KernelEntry* table_f32_var(int* n);
+const KernelEntry* table_mat_pad(int* n);
 
 int kernel_variants(int precision, int nx, int ny, int mat, const KernelEntry** out, int max) {
   const KernelEntry* (*tables[])(int*) = {table_f32_1d,     table_f32_2d,        table_f64,
                                           table_mat_f32,    table_mat_f32_1d,    table_mat_f32_1d256,
                                           table_mat_f64,    table_mat_f64_1d256, table_f32_var,
                                           table_f32_var2d,  table_kn_f32,        table_kn_f64,
-                                          table_kn_f32_2d};
+                                          table_kn_f32_2d,  table_mat_pad};
   int k = 0;
   for (auto tab : tables) {
     int n = 0;
@@ -37,7 +38,7 @@ const KernelEntry* find_kernel(int precision, int nx, int ny, int mat, int kn) {
   const KernelEntry* (*tables[])(int*) = {table_f32_1d,     table_f32_2d,        table_f64,
                                           table_mat_f32,    table_mat_f32_1d,    table_mat_f32_1d256,
                                           table_mat_f64,    table_mat_f64_1d256, table_kn_f32,
-                                          table_kn_f64,     table_kn_f32_2d};
+                                          table_kn_f64,     table_kn_f32_2d,     table_mat_pad};
   for (auto tab : tables) {
     int n = 0;
     const KernelEntry* e = tab(&n);
@@ -63,14 +64,14 @@ const KernelEntry* find_padded_kernel(int precision, int nx) {
 }
 
 const KernelEntry* find_padded_kernel2d(int precision, int nx, int ny) {
-  const KernelEntry* (*tables[])(int*) = {table_mat_f32, table_mat_f64};
+  const KernelEntry* (*tables[])(int*) = {table_mat_f32, table_mat_f64, table_mat_pad};
   const KernelEntry* best = nullptr;
   for (auto tab : tables) {
     int n = 0;
     const KernelEntry* e = tab(&n);
     for (int i = 0; i < n; ++i)
-      if (e[i].precision == precision && e[i].mat == 1 && e[i].ny > 1 && e[i].nx == e[i].ny && e[i].nx <= 64 &&
-          e[i].nx >= nx && e[i].ny >= ny && (!best || e[i].nx < best->nx))
+      if (e[i].precision == precision && e[i].mat == 1 && e[i].ny > 1 && e[i].nx * e[i].ny < kPadMaxSites &&
+          e[i].nx >= nx && e[i].ny >= ny && (!best || e[i].nx * e[i].ny < best->nx * best->ny))
         best = &e[i];
   }
   return best;

@@ -116,7 +116,7 @@ __device__ __forceinline__ cpx<R> initial_value(const PathArgs& A, uint32_t nsp,
   const R isc = (R)A.init_scale;
   switch (A.initial) {
     case XSH_INIT_SECH: {
-      cpx<R> a = {(R)(A.ip[0] / cosh(x / A.ip[1])), R(0)};
+      cpx<R> a = {(R)(A.ip[0] / cosh((x - A.ip[3]) / A.ip[1])), R(0)};
       if (A.ip[2] != 0.0) {
         const R e = (R)(A.ip[2] * 0.70710678118654752440);
         if (A.krandoms) return cpx<R>{a.x + e * kv.x, a.y + e * kv.y};
@@ -128,8 +128,9 @@ __device__ __forceinline__ cpx<R> initial_value(const PathArgs& A, uint32_t nsp,
       return a;
     }
     case XSH_INIT_GAUSS: {
-      double r2 = x * x + (A.ny > 1 ? y * y : 0.0);
-      return cpx<R>{(R)(A.ip[0] * exp(-0.5 * r2 / (A.ip[1] * A.ip[1]))), R(0)};
+      const double wy = A.ip[2] > 0.0 ? A.ip[2] : A.ip[1];
+      const double r2 = x * x / (A.ip[1] * A.ip[1]) + (A.ny > 1 ? y * y / (wy * wy) : 0.0);
+      return cpx<R>{(R)(A.ip[0] * exp(-0.5 * r2)), R(0)};
     }
     case XSH_INIT_PHASE: {
       const R u = init_uniform<R>(A, nsp, traj, site * A.urandoms);
@@ -293,11 +294,12 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   using C = cpx<R>;
   constexpr int CPT = Lat::CPT, PPT = Lat::PPT, NS = Lat::NS;
   // Dense-transform lattices may hold fewer physical sites than NX x NY (1D of
-  // any size; 2D up to 64 x 64): the other sites are padding.  Zero rows and
-  // columns of every transform matrix keep them out of the physical sites, no
-  // output reads them, and the physical sites keep their own indices
-  // x + nx * y for the counter streams and the outputs.  FFT kernels: exactly NS.
-  constexpr bool PAD2 = MAT && NY > 1 && NX <= 64 && NY <= 64;
+  // any size; 2D kernels of fewer than kPadMaxSites sites): the other sites are
+  // padding.  Zero rows and columns of every transform matrix keep them out of
+  // the physical sites, no output reads them, and the physical sites keep their
+  // own indices x + nx * y for the counter streams and the outputs.  FFT
+  // kernels and the 128 x 128 dense kernel: exactly NS.
+  constexpr bool PAD2 = MAT && NY > 1 && NX * NY < kPadMaxSites;
   const int NXP = MAT ? A.nx : NX, NYP = MAT && NY > 1 ? A.ny : NY;
   const int NSP = MAT ? A.ns : NS;
   // physical index of lattice site s, -1 for padding

@@ -26,6 +26,9 @@ struct KernelEntry {
 // persistent grid of the streaming kernels: at most this many workgroups,
 // each with its own workspace planes
 constexpr int kStreamMaxBlocks = 1024;
+// 2D dense-transform kernels of fewer sites than this handle padded lattices
+// (the 128 x 128 kernels keep their compile-time size)
+constexpr int kPadMaxSites = 128 * 128;
 
 // kn: the problem has k-space filtered noises or randoms (entries without that
 // code, compiled for speed, are skipped)
@@ -36,8 +39,8 @@ const KernelEntry* find_stream_kernel(int precision, int nx, int ny, int mat);
 // 1D dense-transform kernel of the smallest compiled size >= nx (the lattice is
 // padded to that size: the padding sites take no part in the physical ones)
 const KernelEntry* find_padded_kernel(int precision, int nx);
-// 2D dense-transform kernel of the smallest square compiled size >= nx, ny among
-// those that handle padding (up to 64 x 64)
+// 2D dense-transform kernel of the fewest sites with both sizes >= nx, ny among
+// those that handle padding (fewer than kPadMaxSites: 64 x 64, 128 x 64, 64 x 128)
 const KernelEntry* find_padded_kernel2d(int precision, int nx, int ny);
 hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s);
 hipError_t launch_probe_normals(int precision, uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t inst,

@@ -7,7 +7,8 @@ function d = xgpudesc(p)
 %     precision 'fp32' | 'fp64'
 %     deriv     'zero' | 'kubo' (c) | 'gp' (g, u, s)
 %     linear    struct with l0, l1 (1 or 2 values), l2 (1 or 2 values)
-%     initial   'const' (c0) | 'sech' (amp, width, eps) | 'gauss' (amp, width)
+%     initial   'const' (c0) | 'sech' (amp, width, eps, center)
+%               | 'gauss' (amp, width, width_y)
 %               | 'phase' (amp) | 'coherent' (alpha)
 %     observe   cell of 'a','abs2','a_ave','abs2_ave','abs2_int','abs4_ave',
 %               'corr','lowk','abs2k'
@@ -61,8 +62,8 @@ d.initial = find(strcmp(inames, g.initial)) - 1;
 ip = zeros(1,6);
 switch g.initial
   case 'const',    c = getdef(g,'c0',1); ip(1:2) = [real(c), imag(c)];
-  case 'sech',     ip(1:3) = [getdef(g,'amp',1), getdef(g,'width',1), getdef(g,'eps',0)];
-  case 'gauss',    ip(1:2) = [getdef(g,'amp',1), getdef(g,'width',1)];
+  case 'sech',     ip(1:4) = [getdef(g,'amp',1), getdef(g,'width',1), getdef(g,'eps',0), getdef(g,'center',0)];
+  case 'gauss',    ip(1:3) = [getdef(g,'amp',1), getdef(g,'width',1), getdef(g,'width_y',0)];
   case 'phase',    ip(1) = getdef(g,'amp',1);
   case 'coherent', a = getdef(g,'alpha',1); ip(1:2) = [real(a), imag(a)];
 end

@@ -150,8 +150,10 @@ def make_desc(q):
         d.init_par[0], d.init_par[1] = c.real, c.imag
     elif ini == "sech":
         d.init_par[0], d.init_par[1], d.init_par[2] = g.get("amp", 1.0), g.get("width", 1.0), g.get("eps", 0.0)
+        d.init_par[3] = g.get("center", 0.0)
     elif ini == "gauss":
         d.init_par[0], d.init_par[1] = g.get("amp", 1.0), g.get("width", 1.0)
+        d.init_par[2] = g.get("width_y", 0.0)
     elif ini == "phase":
         d.init_par[0] = g.get("amp", 1.0)
     elif ini == "coherent":
