@@ -142,15 +142,18 @@ __device__ __forceinline__ uint32_t key1(uint32_t tag, uint32_t seq) {
   return (tag & 0xFFu) | (seq << 8);
 }
 
-// Box-Muller on (x0,x1): u = (x + 1/2) 2^-32 in (0,1)
+// Box-Muller on (x0,x1): u = (x + 1/2) 2^-32 in (0,1).  FP32 on the hardware
+// transcendentals directly: v_log_f32 is log2 (so -2 ln u = -2 ln2 log2 u),
+// v_sin_f32 / v_cos_f32 take the angle in revolutions (sin(2 pi u1) =
+// v_sin(u1), u1 < 1 is inside their +-256 input range), v_sqrt_f32 (1 ulp);
+// u0 >= 2^-33 and -2 ln u0 <= 46 keep every operand normal, so none of the
+// library's denormal scaling is needed.
 __device__ __forceinline__ void box_muller(uint32_t x0, uint32_t x1, float& n0, float& n1) {
   const float u0 = (float)x0 * 2.3283064365386963e-10f + 1.1641532182693481e-10f;
   const float u1 = (float)x1 * 2.3283064365386963e-10f + 1.1641532182693481e-10f;
-  const float r = __fsqrt_rn(-2.0f * __logf(u0));
-  float s, c;
-  __sincosf(6.2831853071795865f * u1, &s, &c);
-  n0 = r * c;
-  n1 = r * s;
+  const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u0));
+  n0 = r * __builtin_amdgcn_cosf(u1);
+  n1 = r * __builtin_amdgcn_sinf(u1);
 }
 __device__ __forceinline__ void box_muller(uint32_t x0, uint32_t x1, double& n0, double& n1) {
   const double u0 = ((double)x0 + 0.5) * 2.3283064365386963e-10;
