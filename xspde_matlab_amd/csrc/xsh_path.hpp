@@ -149,6 +149,24 @@ __device__ __forceinline__ void atomic_add_c(double* p, double re, double im) {
 // Thread index through an empty asm: address arithmetic derived from it is
 // recomputed where it is used instead of being hoisted out of the time loop
 // and held in VGPRs for the whole path (that hoisting caused register spills).
+// Element i of a read-only table in device memory (twiddles, propagator
+// factors), reached through a pointer of the argument block that the compiler
+// sees as generic: reading it through the global address space makes the load
+// a global load (vector-memory counter only, 32-bit offsets) instead of a flat
+// load, which also counts against the LDS counter and so delays the LDS waits
+// of the transforms around it.
+// (The host pass of the compiler, which also analyses device templates, keeps
+// the plain read.)
+template <class R>
+__device__ __forceinline__ cpx<R> gld(const cpx<R>* p, int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __attribute__((address_space(1))) R* q = (const __attribute__((address_space(1))) R*)p;
+  return cpx<R>{q[2 * i], q[2 * i + 1]};
+#else
+  return p[i];
+#endif
+}
+
 __device__ __forceinline__ int opaque_tid() {
   int t = threadIdx.x;
   asm volatile("" : "+v"(t));
@@ -238,7 +256,7 @@ struct Lattice {
       for (int n1 = 0; n1 < N1; ++n1) v[n1] = L[b + (n1 * N2 + n2) * Ln::ES];
       dft<R, N1, -1>(v);
 #pragma unroll
-      for (int k1 = 1; k1 < N1; ++k1) v[k1] = v[k1] * tw[n2 * k1];
+      for (int k1 = 1; k1 < N1; ++k1) v[k1] = v[k1] * gld(tw, n2 * k1);
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) L[b + (k1 * N2 + n2) * Ln::ES] = v[k1];
     }
@@ -274,7 +292,7 @@ struct Lattice {
       for (int n = 0; n < N2; ++n) v[n] = L[b + n * Ln::ES];
       dft<R, N2, 1>(v);
 #pragma unroll
-      for (int n2 = 1; n2 < N2; ++n2) v[n2] = v[n2] * conjc(tw[n2 * k1]);
+      for (int n2 = 1; n2 < N2; ++n2) v[n2] = v[n2] * conjc(gld(tw, n2 * k1));
 #pragma unroll
       for (int n = 0; n < N2; ++n) L[b + n * Ln::ES] = v[n];
     }
@@ -296,7 +314,7 @@ struct Lattice {
       for (int k2 = 0; k2 < N2; ++k2) v[k2] = op(v[k2], l, k1 + N1 * k2);
       dft<R, N2, 1>(v);
 #pragma unroll
-      for (int n2 = 1; n2 < N2; ++n2) v[n2] = v[n2] * conjc(tw[n2 * k1]);
+      for (int n2 = 1; n2 < N2; ++n2) v[n2] = v[n2] * conjc(gld(tw, n2 * k1));
 #pragma unroll
       for (int n = 0; n < N2; ++n) L[b + n * Ln::ES] = v[n];
     }
@@ -622,8 +640,8 @@ struct PropOp {
   const cpx<R>* py;
   // (kx, ky): frequency indices; p: trajectory within the workgroup
   __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int kx, int ky, int p = 0) const {
-    cpx<R> r = X * px[kx];
-    return py ? r * py[ky] : r;
+    cpx<R> r = X * gld(px, kx);
+    return py ? r * gld(py, ky) : r;
   }
 };
 // the same on a 2D lattice: no test of py per element
@@ -632,7 +650,7 @@ struct PropOp2 {
   const cpx<R>* px;
   const cpx<R>* py;
   __device__ __forceinline__ cpx<R> operator()(cpx<R> X, int kx, int ky, int p = 0) const {
-    return (X * px[kx]) * py[ky];
+    return (X * gld(px, kx)) * gld(py, ky);
   }
 };
 // observable ops, sharing one transform instance:

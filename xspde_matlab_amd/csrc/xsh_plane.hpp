@@ -63,12 +63,12 @@ struct Plane {
         dft<R, CPT, -1>(a);
         static_for<1, CPT>([&](auto K) __attribute__((always_inline)) {
           constexpr int k = decltype(K)::value;
-          a[k] = a[k] * twy[r * k];
+          a[k] = a[k] * gld(twy, r * k);
         });
       } else {
         static_for<1, CPT>([&](auto K) __attribute__((always_inline)) {
           constexpr int k = decltype(K)::value;
-          a[k] = a[k] * conjc(twy[r * k]);
+          a[k] = a[k] * conjc(gld(twy, r * k));
         });
         dft<R, CPT, 1>(a);
       }
@@ -130,7 +130,7 @@ struct Plane {
 #pragma unroll
       for (int n = 0; n < XN2; ++n) v[n] = blk[n];
 #pragma unroll
-      for (int n2 = 1; n2 < XN2; ++n2) v[n2] = v[n2] * twx[n2 * k1];
+      for (int n2 = 1; n2 < XN2; ++n2) v[n2] = v[n2] * gld(twx, n2 * k1);
       dft<R, XN2, -1>(v);
 #pragma unroll
       for (int n = 0; n < XN2; ++n) blk[n] = v[n];
@@ -149,7 +149,7 @@ struct Plane {
       for (int n = 0; n < XN2; ++n) v[n] = blk[n];
       dft<R, XN2, 1>(v);
 #pragma unroll
-      for (int n2 = 1; n2 < XN2; ++n2) v[n2] = v[n2] * conjc(twx[n2 * k1]);
+      for (int n2 = 1; n2 < XN2; ++n2) v[n2] = v[n2] * conjc(gld(twx, n2 * k1));
 #pragma unroll
       for (int n = 0; n < XN2; ++n) blk[n] = v[n];
     }
