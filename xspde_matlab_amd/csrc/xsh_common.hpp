@@ -59,10 +59,35 @@ __host__ __device__ constexpr int bit_reverse(int i, int bits) {
   return r;
 }
 
+__device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// (a + w b, a - w b) for the compile-time twiddle w = cos(2 pi m / 64) +
+// SIGN i sin(2 pi m / 64) in six fused multiply-adds instead of a complex
+// multiply and two complex additions (eight): w b = c (b + i (s/c) b) when
+// |c| >= |s|, else s ((c/s) b + i b), and the outer factor goes into the
+// multiply-adds with a (the ratio never exceeds 1 in magnitude).
+template <class R, int SIGN, int M>
+__device__ __forceinline__ void twiddle_butterfly(cpx<R> a, cpx<R> b, cpx<R>& out0, cpx<R>& out1) {
+  constexpr double c = kCos64[M], s = SIGN * kSin64[M];
+  constexpr bool cos_major = (c < 0 ? -c : c) >= (s < 0 ? -s : s);
+  constexpr R f = R(cos_major ? c : s);
+  constexpr R r = R(cos_major ? s / c : c / s);
+  cpx<R> t;
+  if constexpr (cos_major) {
+    t = cpx<R>{fmad(-r, b.y, b.x), fmad(r, b.x, b.y)};   // w b = c t
+  } else {
+    t = cpx<R>{fmad(r, b.x, -b.y), fmad(r, b.y, b.x)};   // w b = s t
+  }
+  out0 = cpx<R>{fmad(f, t.x, a.x), fmad(f, t.y, a.y)};
+  out1 = cpx<R>{fmad(-f, t.x, a.x), fmad(-f, t.y, a.y)};
+}
+
 // In-register DFT of length N (power of two, N <= 64), natural order in and
 // out.  SIGN = -1 forward (exp(-2 pi i nk/N)), +1 inverse (no 1/N).
 // Radix-2 decimation in time over the bit-reversed order; the twiddles are
-// compile-time constants (k = 0 and k = len/4 need no multiply).
+// compile-time constants (k = 0 and k = len/4 need no multiply, the others
+// run as twiddle_butterfly).
 template <class R, int N, int SIGN>
 __device__ __forceinline__ void dft(cpx<R> (&v)[N]) {
   if constexpr (N == 1) {
@@ -98,16 +123,13 @@ __device__ __forceinline__ void dft(cpx<R> (&v)[N]) {
           constexpr int k = decltype(K)::value;
           const cpx<R> a = v[i + k];
           cpx<R> b = v[i + k + half];
-          if constexpr (k == 0) {
-          } else if constexpr (4 * k == len) {
-            b = mul_i<SIGN>(b);
+          if constexpr (k == 0 || 4 * k == len) {
+            if constexpr (k != 0) b = mul_i<SIGN>(b);
+            v[i + k] = a + b;
+            v[i + k + half] = a - b;
           } else {
-            constexpr int m = k * (64 / len);
-            const cpx<R> w = {R(kCos64[m]), R(SIGN) * R(kSin64[m])};
-            b = b * w;
+            twiddle_butterfly<R, SIGN, k * (64 / len)>(a, b, v[i + k], v[i + k + half]);
           }
-          v[i + k] = a + b;
-          v[i + k + half] = a - b;
         });
       });
     });

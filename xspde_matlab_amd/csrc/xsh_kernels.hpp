@@ -277,6 +277,20 @@ constexpr int lds_elems() {
     return Lattice<R, NX, NY, NT, TPB, RX>::LDS;
 }
 
+// The propagator multiply of a lattice kernel (per-dimension factors).  A
+// product table P[kx + nx ky] (one complex multiply instead of two) measured
+// slower on the 128 x 128 kernel: its 16 per-lane loads per column item sit
+// between the transform's barriers (1784 -> 2231 ms per bench launch), where
+// Py[ky] is a wave-uniform scalar load.
+template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX, bool KN>
+__device__ __forceinline__ auto make_prop(const PathArgs& A, int nci) {
+  using C = cpx<R>;
+  if constexpr (NY > 1)
+    return PropOp2<R>{static_cast<const C*>(A.px[nci]), static_cast<const C*>(A.py[nci])};
+  else
+    return PropOp<R>{static_cast<const C*>(A.px[nci]), nullptr};
+}
+
 // a -> T^-1 op(T a) of the planes held in f, periodic lattices
 template <class R, int NX, int NY, int NT, int TPB, bool MAT, int RX, bool KN, class Op>
 __device__ __forceinline__ void periodic_spectral(cpx<R>* L, cpx<R> (&f)[Lattice<R, NX, NY, NT, TPB, RX>::PPT],
@@ -318,8 +332,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   const R hdt = (R)(nci ? A.hdt[1] : A.hdt[0]);
   Drift<R> D;
   D.init(A);
-  const std::conditional_t<(NY > 1), PropOp2<R>, PropOp<R>> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
-                       NY > 1 ? static_cast<const C*>(nci ? A.py[1] : A.py[0]) : nullptr};
+  const auto prop = make_prop<R, NX, NY, NT, TPB, MAT, RX, KN>(A, nci);
   const C* twx = static_cast<const C*>(A.twx);
   const C* twy = static_cast<const C*>(A.twy);
   double* av = A.av + 2 * (long long)pidx * A.avlen;
@@ -518,14 +531,17 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   };
 
   // Pointwise stages visit the register chunks (2 sites each) of one or two
-  // fields.  2D lattices run a runtime loop that works on the first UR chunks
-  // and then rotates the arrays by UR chunks (register moves, static indices):
-  // UR copies of the noise generator and drift per stage instead of CPT keep
-  // the time step inside the instruction cache (128x128 per launch: unrolled
-  // 3.23 s, UR = 1 3.03 s, UR = 2 2.94 s, UR = 4 3.04 s).  1D lattices, with
-  // few chunks, unroll.
+  // fields.  Other 2D lattices run a runtime loop that works on the first UR
+  // chunks and then rotates the arrays by UR chunks (register moves, static
+  // indices): UR copies of the noise generator and drift per stage instead of
+  // CPT keep their larger time steps inside the instruction cache (the
+  // generic-transform 128x128 kernel per launch: unrolled 3.23 s, UR = 1
+  // 3.03 s, UR = 2 2.94 s, UR = 4 3.04 s).  The 128 x 128 FFT kernel's
+  // transform (Plane) is small enough that its fully unrolled stages fit: no
+  // rotation moves (1871 -> 1784 ms per bench launch; UR = 4 rotating
+  // 1866 ms).  1D lattices, with few chunks, unroll.
   constexpr int UR = CPT % 2 == 0 ? 2 : 1;
-  constexpr bool ROTATE = NY > 1;
+  constexpr bool ROTATE = NY > 1 && !kPlaneFFT<NX, NY, NT, TPB, MAT, RX, KN>;
   auto rotate = [&](C (&f)[PPT]) __attribute__((always_inline)) {
     C t[2 * UR];
     XSH_FOR(k, 0, 2 * UR) t[k] = f[k]; XSH_END
@@ -856,8 +872,7 @@ __global__ __launch_bounds__(NT) void probe_propagate(const PathArgs* __restrict
       const long long g = base + Lat::point(i, v);
       f[2 * i + v] = g < (long long)nplanes * Lat::NS ? planes[g] : C{R(0), R(0)};
     }
-  const std::conditional_t<(NY > 1), PropOp2<R>, PropOp<R>> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
-                       NY > 1 ? static_cast<const C*>(nci ? A.py[1] : A.py[0]) : nullptr};
+  const auto prop = make_prop<R, NX, NY, NT, TPB, MAT, RX, false>(A, nci);
   if constexpr (MAT)
     Lat::spectral_mat(L, f, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop);
   else
