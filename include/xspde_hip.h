@@ -139,6 +139,16 @@ typedef struct {
   int knoises, krandoms;
   const double* nfilter;
   const double* rfilter;
+  /* Time-domain spectra (p.transforms{o}(1), xpath.m:45-68,90-106): tspec[o] = 1
+     takes observable o of the field's time transform instead of the field:
+     a(w_m) = fsc/M sum_n a_n exp(+2 pi i j_m n / M) over the M = points * steps
+     step fields a_n of each trajectory (a_0 the initial field; a fine run with
+     checks stores the mean of each step's two half-step results), at the
+     points frequencies j_m = m - floor(points/2), w_m = j_m dw,
+     dw = 2 pi / (ranges[0] + dt), fsc = sqrt(2 pi) / dw (xpreferences.m:199-203).
+     Its averages fill the observable's [points] block with frequency in place of
+     time.  SDEs (dimensions = 1); not in batches or xsh_run_path_noalloc. */
+  int tspec[XSH_MAX_OBS];
 } xsh_desc;
 
 typedef struct xsh_handle xsh_handle;

@@ -94,7 +94,8 @@ def make_initial(g):
         x0 = float(g.get("center", 0.0))
 
         def initial(v, p):
-            a = amp / np.cosh((p.x - x0) / wd) + 0j
+            x = p.x if p.dimensions > 1 else 0.0         # an SDE has no space grid: x = 0
+            a = amp / np.cosh((x - x0) / wd) + 0j
             if eps != 0.0:
                 a = a + eps * (v[0:1] + 1j * v[1:2]) / np.sqrt(2.0)
             return a
@@ -105,7 +106,7 @@ def make_initial(g):
         wy = float(g.get("width_y", 0.0)) or wd
 
         def initial(v, p):
-            r2 = p.x ** 2 / wd ** 2
+            r2 = (p.x ** 2 if p.dimensions > 1 else 0.0) / wd ** 2
             if p.dimensions > 2:
                 r2 = r2 + p.y ** 2 / wy ** 2
             return amp * np.exp(-0.5 * r2) + 0j
@@ -191,5 +192,7 @@ def apply_forms(p_in):
     p["observe"] = [make_observe(o, g) for o in names]
     if "abs2k" in names:
         nd = p["dimensions"]
-        p["transforms"] = [[0] + [1 if o == "abs2k" else 0] * (nd - 1) for o in names]
+        given = p.get("transforms") or [0] * len(names)
+        tflag = [int(np.ravel(t)[0]) if np.size(t) else 0 for t in given] + [0] * len(names)
+        p["transforms"] = [[tflag[i]] + [1 if o == "abs2k" else 0] * (nd - 1) for i, o in enumerate(names)]
     return p

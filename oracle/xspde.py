@@ -34,8 +34,10 @@ xnoise / xrandom      Simulator/xnoise.m:26-60, xrandom.m:18-34     moments, KS 
   krandoms)                                                         closed form; MATLAB
                                                                     stream itself: parity
                                                                     unpinned
-xpath                 Simulator/xpath.m:16-86                       Batchtest rows, checks
-                                                                    pairing test
+xpath                 Simulator/xpath.m:16-107                      Batchtest rows, checks
+                      (incl. time-domain spectra, xpath.m:45-68,    pairing test; spectra:
+                      90-106, xpreferences.m:199-203)               geometric-series closed
+                                                                    form, OU spectrum
 xdata                 Simulator/xdata.m:21-85 (mean branch)       Batchtest rows
 xensemble             Simulator/xensemble.m:12-108                  Batchtest #21 (order
                                                                     extrapolation)
@@ -227,6 +229,13 @@ def xpreferences(p_in):
     ax = p.get("axes", None)
     p.axes = _listify(ax, len(p.output), None) if ax is not None else [None] * len(p.output)
     p.cutoff = float(p.get("cutoff", -1e20))
+    # time-frequency grid of the spectra (xpreferences.m:199-203) and the
+    # spectrum switch (xpreferences.m:99,246)
+    p.dw = 2 * np.pi / (p.ranges[0] + p.dt)
+    p.fsc = math.sqrt(2 * np.pi) / p.dw
+    p.plotpts = np.arange(pts[0]) - pts[0] // 2
+    p.w = p.plotpts * p.dw
+    p.spectrum = int(bool(p.get("spectrum", 0)) or any(t[0] for t in p.transforms))
     p.t = float(p.origins[0])
     p.dtr = 0.0
     return p
@@ -491,8 +500,14 @@ def xpath(a, nc, p, prop, nsp, seq=0, inst=0, noise_fn=None):
     av = [None] * len(p.observe)
     q = 0
     draw = noise_fn or (lambda qq: noise_draw(p, nsp, qq, seq, inst))
+    # the field at every step for the time-domain spectra (xpath.m:45-53,64-68):
+    # aft[:, n1] after step n1 (n1 = 0: the initial field); a fine run with
+    # checks stores the mean of its two half steps' results
+    aft = np.zeros((npts * nsteps,) + a.shape, dtype=complex) if p.spectrum else None
+    n1 = -1
     for n in range(npts):
         for step in range(nsteps):
+            n1 += 1
             if n > 0 or step > 0 or npts == 1:
                 w = None
                 for c in range(chk + 1):
@@ -503,14 +518,36 @@ def xpath(a, nc, p, prop, nsp, seq=0, inst=0, noise_fn=None):
                     w = wn
                     if nc == 1 or c == 1:
                         a = method(a, w, p, prop)
+                        if aft is not None:
+                            aft[n1] = (a + aft[n1]) / 2 if (c == 1 and nc == 1) else a
                         w = None
                         p.t = p.t + p.dtr
+            elif aft is not None:
+                aft[n1] = a
             if step == 0:
                 for o in range(len(p.observe)):
+                    if p.transforms[o][0]:
+                        continue                                          # xpath.m:80
                     d = xdata(a, o, p)
                     if av[o] is None:
                         av[o] = np.zeros((d.shape[0], npts, d.shape[1]), dtype=complex)
                     av[o][:, n, :] = d
+    if aft is not None:
+        # xpath.m:90-105: ifft over time times fsc, fftshift, keep the points(1)
+        # frequencies around zero (specpts), observables of the transformed field
+        M = npts * nsteps
+        ft = np.fft.ifft(aft, axis=0) * p.fsc
+        ft = np.fft.fftshift(ft, axes=0)
+        specpts = p.plotpts + M // 2
+        ft = ft[specpts]
+        for o in range(len(p.observe)):
+            if not p.transforms[o][0]:
+                continue
+            for n in range(npts):
+                d = xdata(ft[n], o, p)
+                if av[o] is None:
+                    av[o] = np.zeros((d.shape[0], npts, d.shape[1]), dtype=complex)
+                av[o][:, n, :] = d
     return a, av
 
 

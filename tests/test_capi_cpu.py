@@ -182,3 +182,19 @@ def test_filter_table_site_order_and_linearity():
     with pytest.raises(GpuFormError):
         filter_table(preferences(p), "nfilter", 2)
     assert filter_table(preferences(dict(p, nfilter=None)), "nfilter", 1).shape == (1, 1024)
+
+
+def test_time_spectra_flags():
+    """p.transforms{o}(1) reaches the descriptor as tspec[o]; spectra run on SDEs
+    only, and a flag other than 0/1 is invalid (rejected before any device call)."""
+    p = configs.shrink(configs.nls1d(), E=16, e2=1, points=3)
+    p["transforms"] = [0, [1, 0]]
+    d = make_desc(preferences(p))
+    assert list(d.tspec)[:2] == [0, 1]
+    rc, msg = _create(d)
+    assert rc == 2 and "SDEs" in msg
+    q = preferences(configs.shrink(configs.kubo(), E=16, e2=1, points=3))
+    d = make_desc(q)
+    d.tspec[0] = 2
+    rc, msg = _create(d)
+    assert rc == 1 and "tspec" in msg

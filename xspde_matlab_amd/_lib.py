@@ -51,6 +51,7 @@ class Desc(C.Structure):
         ("boundtab", C.POINTER(C.c_double)), ("boundlevels", C.c_int),
         ("knoises", C.c_int), ("krandoms", C.c_int),
         ("nfilter", C.POINTER(C.c_double)), ("rfilter", C.POINTER(C.c_double)),
+        ("tspec", C.c_int * XSH_MAX_OBS),
     ]
 
     def set_filters(self, nfilter=None, rfilter=None):

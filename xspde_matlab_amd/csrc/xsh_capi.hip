@@ -83,6 +83,10 @@ struct xsh_handle {
   double* d_raw;     // [e2*nchk][points][E][ns] complex raw fields
   float* d_feat;     // [e2*nchk][E][2*nmodes] noise modes
   int* d_fmap;       // [nx*ny] mode index per FFT position
+  double* d_ft;      // SDE spectra: [e2*nchk][points*steps][E] complex step fields
+  double* d_twt;     // [points*steps] complex exp(2 pi i k / M)
+  int tspec_mask;    // bit o: observable o of the time transform
+  double fsc;        // sqrt(2 pi) / dw (xpreferences.m:199-200)
   hipStream_t stream;
   hipEvent_t ev0, ev1;
   xsh::PathArgs* d_args;   // kernel arguments in device memory
@@ -171,6 +175,13 @@ static int validate(const xsh_desc* d) {
     if ((long long)d->boundlevels * 4 * nb * 16 > (1LL << 31))
       return fail(XSH_EUNSUPPORTED, "boundary-value table beyond 2 GiB");
   }
+  int tspec = 0;
+  for (int o = 0; o < d->nobserve; ++o) {
+    if (d->tspec[o] != 0 && d->tspec[o] != 1) return fail(XSH_EINVAL, "tspec[%d] must be 0 or 1", o);
+    tspec |= d->tspec[o];
+  }
+  if (tspec && d->dimensions != 1)
+    return fail(XSH_EUNSUPPORTED, "time-domain spectra (p.transforms{o}(1)) run on SDEs (dimensions = 1)");
   if (d->nmodes < 0 || d->nmodes > XSH_MAX_MODES) return fail(XSH_EINVAL, "nmodes must be 0..%d", XSH_MAX_MODES);
   if (d->nmodes > 0) {
     if (d->initial != XSH_INIT_COHERENT) return fail(XSH_EUNSUPPORTED, "noise modes need the coherent initial form");
@@ -642,6 +653,28 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     if (hipMemcpy(h->d_fmap, fmap.data(), fmap.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(fail(XSH_EDEVICE, "copy mode map"));
   }
+  for (int o = 0; o < d.nobserve; ++o)
+    if (d.tspec[o]) h->tspec_mask |= 1 << o;
+  if (h->tspec_mask) {
+    // xpath.m:18 aft = [fields, points * steps, ens] per path; the frequency
+    // grid of xpreferences.m:199-200
+    const long long M = (long long)d.points * d.steps;
+    const size_t fbytes = paths * (size_t)M * h->E * 16;
+    if (hipMalloc(&h->d_ft, fbytes) != hipSuccess)
+      return cleanup(fail(XSH_ENOMEM, "hipMalloc spectrum step fields (%zu bytes)", fbytes));
+    std::vector<double> tw(2 * (size_t)M);
+    for (long long k = 0; k < M; ++k) {
+      const cd w = std::polar(1.0, 2.0 * M_PI * (double)k / (double)M);
+      tw[2 * k] = w.real();
+      tw[2 * k + 1] = w.imag();
+    }
+    if (hipMalloc(&h->d_twt, tw.size() * sizeof(double)) != hipSuccess)
+      return cleanup(fail(XSH_ENOMEM, "hipMalloc spectrum twiddles"));
+    if (hipMemcpy(h->d_twt, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(XSH_EDEVICE, "copy spectrum twiddles"));
+    const double dw = 2.0 * M_PI / (d.ranges[0] + h->dt);
+    h->fsc = std::sqrt(2.0 * M_PI) / dw;
+  }
   if (h->kern && h->kern->stream) {
     // one workspace per resident workgroup of the persistent grid
     bool corr = false;
@@ -682,6 +715,8 @@ extern "C" void xsh_destroy(xsh_handle* h) {
   if (h->d_kwork) (void)hipFree(h->d_kwork);
   if (h->d_feat) (void)hipFree(h->d_feat);
   if (h->d_fmap) (void)hipFree(h->d_fmap);
+  if (h->d_ft) (void)hipFree(h->d_ft);
+  if (h->d_twt) (void)hipFree(h->d_twt);
   if (h->d_work) (void)hipFree(h->d_work);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -761,6 +796,9 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
   }
   A.avlen = h->avlen;
   A.inv_E = 1.0 / h->E;
+  A.ft = h->d_ft;
+  A.ft_len = d.points * d.steps;
+  A.tspec_mask = h->tspec_mask;
   A.has_linear = d.has_linear;
   char* t = static_cast<char*>(h->d_tables);
   A.px[0] = t + h->tab_off[0];
@@ -818,6 +856,9 @@ static int launch(xsh_handle* h, const xsh::PathArgs& A, int nblocks) {
   hipError_t e;
   if (h->nd == 1) {
     e = xsh::launch_sde_path(h->d.precision, h->d_args, nblocks, h->stream);
+    if (e == hipSuccess && h->tspec_mask)
+      e = xsh::launch_sde_spectrum(h->d_args, h->d_twt, h->fsc, h->E, h->d.points, A.j_only > 0 ? 1 : h->e2 * h->nchk,
+                                   h->stream);
   } else {
     const int grid = h->kern->stream ? std::min(nblocks, xsh::kStreamMaxBlocks) : nblocks;
     e = h->kern->launch(h->d_args, nullptr, 1, nblocks, grid, h->stream);
@@ -940,6 +981,8 @@ extern "C" int xsh_run_path_noalloc(xsh_handle* h, int npar, int j, int nc, cons
   if (j < 1 || j > h->e2) return fail(XSH_EINVAL, "j %d outside 1..%d", j, h->e2);
   if (nc < 1 || nc > h->nchk) return fail(XSH_EINVAL, "nc %d outside 1..%d", nc, h->nchk);
   if (!av_dev) return fail(XSH_EINVAL, "av_dev is NULL");
+  if (h->tspec_mask)
+    return fail(XSH_EUNSUPPORTED, "time-domain spectra keep every step's fields: not in the scratch of this entry");
   int dev = -1;
   HIPCHK(hipGetDevice(&dev));
   if (dev != h->device) return fail(XSH_ESTATE, "handle belongs to device %d, current device is %d", h->device, dev);

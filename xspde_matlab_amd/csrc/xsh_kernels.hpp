@@ -206,6 +206,7 @@ __global__ __launch_bounds__(256) void sde_path(const PathArgs* __restrict__ Ap)
 #pragma unroll
     for (int o = 0; o < XSH_MAX_OBS; ++o) {
       if (o >= A.nobs) break;
+      if ((A.tspec_mask >> o) & 1) continue;  // taken of the time transform (sde_spectrum)
       double re = 0.0, im = 0.0;
       if (active) {
         switch (A.obs[o]) {
@@ -228,17 +229,37 @@ __global__ __launch_bounds__(256) void sde_path(const PathArgs* __restrict__ Ap)
     }
   };
 
+  // the field at every step for the time-domain spectra (xpath.m:45-68):
+  // sample n1 after step n1 (n1 = 0: the initial field); a fine run with checks
+  // stores the mean of the step's two half-step results
+  double* ft = A.ft && active ? A.ft + 2 * (long long)pidx * A.ft_len * A.E : nullptr;
+  auto sample = [&](int n1, double re, double im) __attribute__((always_inline)) {
+    if (ft) {
+      ft[2 * ((long long)n1 * A.E + traj)] = re;
+      ft[2 * ((long long)n1 * A.E + traj) + 1] = im;
+    }
+  };
   uint32_t q = 0;
+  int n1 = 0;
   for (int n = 0; n < A.npts; ++n) {
-    for (int st = 0; st < A.steps; ++st) {
+    for (int st = 0; st < A.steps; ++st, ++n1) {
       if (n > 0 || st > 0 || A.npts == 1) {
         if (A.checks && nci == 1) {
           const C w0 = draw(q), w1 = draw(q + 1);
           q += 2;
           step(R(0.5) * (w0 + w1));
+          sample(n1, (double)a.x, (double)a.y);
+        } else if (A.checks) {
+          step(draw(q++));
+          const double hr = (double)a.x, hi = (double)a.y;
+          step(draw(q++));
+          sample(n1, 0.5 * ((double)a.x + hr), 0.5 * ((double)a.y + hi));
         } else {
-          for (int c = 0; c <= A.checks; ++c) step(draw(q++));
+          step(draw(q++));
+          sample(n1, (double)a.x, (double)a.y);
         }
+      } else {
+        sample(n1, (double)a.x, (double)a.y);
       }
       if (st == 0) observe(n);
     }

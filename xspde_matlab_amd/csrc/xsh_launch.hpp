@@ -43,6 +43,9 @@ const KernelEntry* find_padded_kernel(int precision, int nx);
 // those that handle padding (fewer than kPadMaxSites: 64 x 64, 128 x 64, 64 x 128)
 const KernelEntry* find_padded_kernel2d(int precision, int nx, int ny);
 hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s);
+// time-domain spectra of the step fields sde_path stored (tw: exp(2 pi i k / M), k < M)
+hipError_t launch_sde_spectrum(const PathArgs* A, const double* tw, double fsc, int E, int npts, int npaths,
+                               hipStream_t s);
 hipError_t launch_probe_normals(int precision, uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t inst,
                                 uint32_t draw, int traj0, int ntraj, int count, double* out, hipStream_t s);
 }  // namespace xsh

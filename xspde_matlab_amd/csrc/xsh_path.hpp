@@ -83,6 +83,10 @@ struct PathArgs {
   void* work;              // streaming kernels: per-workgroup field planes in device memory
   int work_planes;         // planes per workgroup (2, or 3 with the corr observable)
   const void* twh;         // streaming kernels: W_{ny/2}^m, m < ny/2
+  double* ft;              // SDE time-domain spectra: [paths][ft_len][E] complex double
+                           // step fields (time-major, trajectories fastest)
+  int ft_len;              // points * steps
+  int tspec_mask;          // bit o: observable o is taken of the time transform
   double* av;              // [paths][avlen] complex double
   const double* a0;        // optional [E][ns] complex double initial fields
   double* a_out;           // optional [E][ns] complex double final fields

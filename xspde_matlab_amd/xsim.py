@@ -167,6 +167,13 @@ def make_desc(q):
         if name not in _lib.OBSERVE:
             raise GpuFormError("unknown observe form %r" % (name,))
         d.observe[o] = _lib.OBSERVE[name]
+    # p.transforms{o}(1): observable o of the time-domain spectrum (xpath.m:90-106)
+    tr = q.get("transforms")
+    if tr is not None:
+        tr = list(tr) if isinstance(tr, (list, tuple)) else [tr]
+        for o in range(len(obs)):
+            t = tr[o] if o < len(tr) else 0
+            d.tspec[o] = 1 if (np.ravel(t)[0] if np.size(t) else 0) else 0
     bv = q.get("boundval") or []
     for k in range(min(len(bv), 2)):
         if bv[k] is None:
