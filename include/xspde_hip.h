@@ -225,6 +225,13 @@ int xsh_synchronize(xsh_handle* h, double* ms);
  * path (j, nc) at block (j-1)*nchk + nc-1, each laid out as xsh_run_path's av_out. */
 int xsh_read_averages(xsh_handle* h, double* av_out);
 
+/* Divergence check (no reference counterpart: MATLAB carries NaN into the
+ * averages without notice): waits for the handle's stream and sets *flag to 1
+ * if a trajectory's field held a NaN or infinite value at a stored time point
+ * of the last xsh_run_path or xsh_run_ensemble, else 0.  The kernels test
+ * every site at every stored point (xsh_run_path_noalloc does not check). */
+int xsh_nonfinite(xsh_handle* h, int* flag);
+
 /* Per-trajectory data of the last xsh_run_ensemble (no reference counterpart;
  * feeds the control-variate estimator):
  *   obs  [ensembles(2)][nchk][ensembles(1)][points][nobserve] real parts of the
@@ -271,6 +278,8 @@ int xsh_batch_synchronize(xsh_batch* b, double* ms);
 /* Averages of instance i, laid out as xsh_read_averages. */
 int xsh_batch_read_averages(xsh_batch* b, int i, double* av_out);
 int xsh_batch_query(const xsh_batch* b, int i, xsh_info* info);
+/* xsh_nonfinite of instance i for the last xsh_batch_run. */
+int xsh_batch_nonfinite(xsh_batch* b, int i, int* flag);
 /* Kernel plan: *ngroups families; plan[8*g ..] = precision, nx, ny, instances,
  * threads, trajectories per workgroup, first x radix, persistent grid. */
 int xsh_batch_plan(const xsh_batch* b, int* ngroups, int* plan);

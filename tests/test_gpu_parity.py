@@ -719,3 +719,29 @@ def test_raw_fields(case, tol):
                 err = np.abs(got - ref).max() / np.abs(ref).max()
                 assert err < tol, (j, nc, t, err)
     print("%s raw fields: %d paths x %d points match" % (case, 4, p["points"][0]))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("shape", [(), (64,), (128, 128)])
+def test_divergence_flag(prec, shape):
+    """The kernels' non-finite check: a drift g |a|^2 a with g = 50 blows up
+    within the path (the flag is set and xsim warns); g = -1 stays finite."""
+    import warnings as W
+    from xspde_matlab_amd.xsim import DivergenceWarning
+    nd = len(shape) + 1
+    p = dict(name="Blowup", dimensions=nd, points=[5] + list(shape), ranges=[2.0] + [10.0] * len(shape),
+             steps=[2] + [1] * len(shape), checks=0, noises=0, method="RK4", ensembles=[2, 1, 1],
+             gpu=dict(deriv="gp", g=50.0, u=0.0, s=0.0, initial="const", c0=1.0, observe=["abs2_ave"],
+                      precision=prec))
+    prob = Problem(p)
+    with W.catch_warnings(record=True) as rec:
+        W.simplefilter("always")
+        prob.xensemble(1)
+    assert prob.handle.nonfinite()
+    assert any(issubclass(r.category, DivergenceWarning) for r in rec)
+    prob.close()
+    p["gpu"]["g"] = -1.0
+    prob = Problem(p)
+    prob.xensemble(1)
+    assert not prob.handle.nonfinite()
+    prob.close()

@@ -25,6 +25,7 @@ STATUS = {0: "XSH_OK", 1: "XSH_EINVAL", 2: "XSH_EUNSUPPORTED", 3: "XSH_ENOMEM", 
 EXPORTS = ["xsh_version", "xsh_last_error", "xsh_create", "xsh_destroy", "xsh_query", "xsh_run_path",
            "xsh_run_path_noalloc",
            "xsh_run_ensemble", "xsh_synchronize", "xsh_read_averages", "xsh_read_trajectories", "xsh_read_raw",
+           "xsh_nonfinite", "xsh_batch_nonfinite",
            "xsh_probe_normals", "xsh_probe_propagate", "xsh_mlp_create", "xsh_mlp_destroy",
            "xsh_mlp_last_error", "xsh_mlp_get_params", "xsh_mlp_forward", "xsh_mlp_loss_grad",
            "xsh_mlp_train", "xsh_mlp_set_translations", "xsh_mlp_gaussian_moments",
@@ -118,6 +119,8 @@ def load():
     fp = C.POINTER(C.c_float)
     lib.xsh_read_trajectories.argtypes = [C.c_void_p, dp, fp]
     lib.xsh_read_raw.argtypes = [C.c_void_p, dp]
+    lib.xsh_nonfinite.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    lib.xsh_batch_nonfinite.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
     lib.xsh_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, fp, C.POINTER(C.c_void_p)]
     lib.xsh_mlp_destroy.argtypes = [C.c_void_p]
     lib.xsh_mlp_destroy.restype = None
@@ -198,6 +201,12 @@ class Handle:
         ms = C.c_double()
         check(load().xsh_synchronize(self._h, C.byref(ms)))
         return ms.value
+
+    def nonfinite(self):
+        """True if a field value at a stored point of the last run was NaN or infinite."""
+        f = C.c_int()
+        check(load().xsh_nonfinite(self._h, C.byref(f)))
+        return bool(f.value)
 
     def read_averages(self):
         n = self.desc.ensembles[1] * self.info.nchk * self.info.avlen
@@ -353,6 +362,12 @@ class Batch:
         ms = C.c_double()
         check(load().xsh_batch_synchronize(self._b, C.byref(ms)))
         return ms.value
+
+    def nonfinite(self, i):
+        """Instance i's divergence flag for the last run."""
+        f = C.c_int()
+        check(load().xsh_batch_nonfinite(self._b, i, C.byref(f)))
+        return bool(f.value)
 
     def read_averages(self, i):
         d, info = self.descs[i], self.infos[i]

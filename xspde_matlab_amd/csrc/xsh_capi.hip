@@ -85,6 +85,7 @@ struct xsh_handle {
   int* d_fmap;       // [nx*ny] mode index per FFT position
   double* d_ft;      // SDE spectra: [e2*nchk][points*steps][E] complex step fields
   double* d_twt;     // [points*steps] complex exp(2 pi i k / M)
+  int* d_nonfinite;  // [2] ints: [0] set by the kernels when a stored field value is not finite
   int tspec_mask;    // bit o: observable o of the time transform
   double fsc;        // sqrt(2 pi) / dw (xpreferences.m:199-200)
   hipStream_t stream;
@@ -675,6 +676,8 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     const double dw = 2.0 * M_PI / (d.ranges[0] + h->dt);
     h->fsc = std::sqrt(2.0 * M_PI) / dw;
   }
+  if (hipMalloc(&h->d_nonfinite, 2 * sizeof(int)) != hipSuccess) return cleanup(fail(XSH_ENOMEM, "hipMalloc flags"));
+  if (hipMemset(h->d_nonfinite, 0, 2 * sizeof(int)) != hipSuccess) return cleanup(fail(XSH_EDEVICE, "clear flags"));
   if (h->kern && h->kern->stream) {
     // one workspace per resident workgroup of the persistent grid
     bool corr = false;
@@ -716,6 +719,7 @@ extern "C" void xsh_destroy(xsh_handle* h) {
   if (h->d_feat) (void)hipFree(h->d_feat);
   if (h->d_fmap) (void)hipFree(h->d_fmap);
   if (h->d_ft) (void)hipFree(h->d_ft);
+  if (h->d_nonfinite) (void)hipFree(h->d_nonfinite);
   if (h->d_twt) (void)hipFree(h->d_twt);
   if (h->d_work) (void)hipFree(h->d_work);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -797,6 +801,7 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
   A.avlen = h->avlen;
   A.inv_E = 1.0 / h->E;
   A.ft = h->d_ft;
+  A.nonfinite = h->d_nonfinite;
   A.ft_len = d.points * d.steps;
   A.tspec_mask = h->tspec_mask;
   A.has_linear = d.has_linear;
@@ -911,6 +916,7 @@ extern "C" int xsh_run_path(xsh_handle* h, int npar, int j, int nc, const double
   A.nmodes = 0;
   const size_t fbytes = (size_t)h->E * h->ns * 16;
   HIPCHK(hipMemsetAsync(h->d_path_av, 0, (size_t)h->avlen * 16, h->stream));
+  HIPCHK(hipMemsetAsync(h->d_nonfinite, 0, 2 * sizeof(int), h->stream));
   if (a0) {
     HIPCHK(hipMemcpyAsync(h->d_a, a0, fbytes, hipMemcpyHostToDevice, h->stream));
     A.a0 = h->d_a;
@@ -1021,6 +1027,7 @@ extern "C" int xsh_run_path_noalloc(xsh_handle* h, int npar, int j, int nc, cons
   A.nmodes = 0;
   A.a0 = a0_dev;
   A.a_out = a_out_dev;
+  A.nonfinite = nullptr;  // the divergence flag is device memory of the handle's own runs
   A.work = work_bytes ? base + args_bytes + av_bytes : nullptr;
   A.kn_work = kn_bytes ? base + args_bytes + av_bytes + work_bytes : nullptr;
   // arguments through the handle's pinned staging copy (reused once its last upload is done)
@@ -1057,6 +1064,7 @@ extern "C" int xsh_run_ensemble(xsh_handle* h, int npar) {
   xsh::PathArgs A = make_args(h, npar);
   A.av = h->d_av;
   HIPCHK(hipMemsetAsync(h->d_av, 0, (size_t)h->e2 * h->nchk * h->avlen * 16, h->stream));
+  HIPCHK(hipMemsetAsync(h->d_nonfinite, 0, 2 * sizeof(int), h->stream));
   HIPCHK(hipEventRecord(h->ev0, h->stream));
   int rc = launch(h, A, h->groups * h->e2 * h->nchk);
   if (rc) return rc;
@@ -1081,6 +1089,13 @@ extern "C" int xsh_read_averages(xsh_handle* h, double* av_out) {
   if (!h || !av_out) return fail(XSH_EINVAL, "NULL argument");
   if (!h->ran) return fail(XSH_ESTATE, "xsh_run_ensemble has not run");
   return copy_averages(h, h->d_av, h->e2 * h->nchk, av_out, h->stream);
+}
+
+extern "C" int xsh_nonfinite(xsh_handle* h, int* flag) {
+  if (!h || !flag) return fail(XSH_EINVAL, "NULL argument");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(flag, h->d_nonfinite, sizeof(int), hipMemcpyDeviceToHost));
+  return XSH_OK;
 }
 
 extern "C" int xsh_read_raw(xsh_handle* h, double* out) {
@@ -1341,6 +1356,7 @@ static int batch_capture(xsh_batch* b) {
   for (auto* h : b->h) {
     const long long n = (long long)h->e2 * h->nchk * h->avlen * 2;
     hipLaunchKernelGGL(zero_doubles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->s0, h->d_av, n);
+    hipLaunchKernelGGL(zero_doubles, dim3(1), dim3(256), 0, b->s0, reinterpret_cast<double*>(h->d_nonfinite), 1LL);
   }
   if (b->serial) {
     for (auto& g : b->g)
@@ -1505,6 +1521,14 @@ extern "C" int xsh_batch_read_averages(xsh_batch* b, int i, double* av_out) {
   xsh_handle* h = b->h[i];
   HIPCHK(hipStreamSynchronize(b->s0));
   return copy_averages(h, h->d_av, h->e2 * h->nchk, av_out, b->s0);
+}
+
+extern "C" int xsh_batch_nonfinite(xsh_batch* b, int i, int* flag) {
+  if (!b || !flag) return fail(XSH_EINVAL, "NULL argument");
+  if (i < 0 || i >= (int)b->h.size()) return fail(XSH_EINVAL, "instance %d outside 0..%d", i, (int)b->h.size() - 1);
+  HIPCHK(hipStreamSynchronize(b->s0));
+  HIPCHK(hipMemcpy(flag, b->h[i]->d_nonfinite, sizeof(int), hipMemcpyDeviceToHost));
+  return XSH_OK;
 }
 
 extern "C" int xsh_batch_query(const xsh_batch* b, int i, xsh_info* info) {

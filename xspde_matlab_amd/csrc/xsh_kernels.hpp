@@ -203,6 +203,7 @@ __global__ __launch_bounds__(256) void sde_path(const PathArgs* __restrict__ Ap)
     }
   };
   auto observe = [&](int n) __attribute__((always_inline)) {
+    if (A.nonfinite && active && not_finite(a)) atomicOr(A.nonfinite, 1);
 #pragma unroll
     for (int o = 0; o < XSH_MAX_OBS; ++o) {
       if (o >= A.nobs) break;
@@ -732,6 +733,11 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
   }
 
   auto observe = [&](int n) __attribute__((always_inline)) {
+    if (A.nonfinite) {  // divergence check: any non-finite site of this thread's chunks
+      bool bad = false;
+      XSH_FOR(k, 0, PPT) bad = bad || not_finite(f0[k]); XSH_END
+      if (bad) atomicOr(A.nonfinite, 1);
+    }
     if (A.raw) {  // xpath.m:73-78: the fields themselves at this point
       XSH_FOR(i, 0, CPT)
         const int pt = Lat::point(i, 0);

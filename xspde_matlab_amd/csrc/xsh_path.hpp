@@ -87,6 +87,8 @@ struct PathArgs {
                            // step fields (time-major, trajectories fastest)
   int ft_len;              // points * steps
   int tspec_mask;          // bit o: observable o is taken of the time transform
+  int* nonfinite;          // optional: set to 1 when a field value at a stored point is
+                           // not finite (divergence check; NULL: not checked)
   double* av;              // [paths][avlen] complex double
   const double* a0;        // optional [E][ns] complex double initial fields
   double* a_out;           // optional [E][ns] complex double final fields
@@ -115,6 +117,12 @@ struct Drift {
     return cpx<R>{r.x + s * xi.x, r.y + s * xi.y};
   }
 };
+
+// a complex value with a NaN or infinite part
+template <class R>
+__device__ __forceinline__ bool not_finite(cpx<R> a) {
+  return !(__builtin_isfinite(a.x) && __builtin_isfinite(a.y));
+}
 
 // block-wide sum of two doubles; every thread gets the result
 template <int NT>

@@ -335,6 +335,15 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
 
     auto observe = [&](int n) __attribute__((always_inline)) {
       __syncthreads();
+      if (A.nonfinite) {  // divergence check over this thread's site pairs
+        bool bad = false;
+#pragma unroll 1
+        for (int i = 0; i < PAIRS; ++i) {
+          const int s = S::site(i);
+          bad = bad || not_finite(F0[s]) || not_finite(F0[s + 1]);
+        }
+        if (bad) atomicOr(A.nonfinite, 1);
+      }
       if (A.raw) {  // xpath.m:73-78
 #pragma unroll 1
         for (int i = 0; i < PAIRS; ++i) {

@@ -21,6 +21,7 @@ in the MATLAB host such projects stay on the reference's CPU ``xpath``.
 """
 import math
 import time
+import warnings
 
 import numpy as np
 
@@ -29,6 +30,10 @@ from . import _lib
 
 class GpuFormError(ValueError):
     """The project cannot run on the GPU path (not a built-in form)."""
+
+
+class DivergenceWarning(RuntimeWarning):
+    """The kernels saw a NaN or infinite field value at a stored time point."""
 
 
 def _vec(x, n, default):
@@ -400,6 +405,10 @@ class Problem:
         fields as p.d.raw."""
         self.handle.run_ensemble(npar)
         self.handle.synchronize()
+        if self.handle.nonfinite():
+            warnings.warn("%s: a trajectory's field was not finite at a stored time point (parallel "
+                          "ensemble %d): its averages hold NaN or Inf" % (self.q.get("name", "project"), npar),
+                          DivergenceWarning, stacklevel=2)
         data = ensemble_moments(self.q, self.info, self.outputs, self.handle.read_averages())
         if not want_raw:
             return data
