@@ -180,6 +180,14 @@ const char* xsh_last_error(void);
  * Replaces the per-(j,nc) setup of xensemble.m:41-53. */
 int xsh_create(const xsh_desc* desc, xsh_handle** out);
 
+/* Everything xsh_create derives on the host, without a device: validates the
+ * descriptor, derives the lattice, picks the kernel, builds every table, and
+ * fills *info as xsh_query would for the handle.  XSH_OK means xsh_create
+ * would get as far as device allocation; XSH_EUNSUPPORTED is the reference's
+ * CPU xpath (no reference counterpart: lets a MATLAB host choose the path
+ * before any device is touched). */
+int xsh_plan(const xsh_desc* desc, xsh_info* info);
+
 /* Release device memory and the handle. */
 void xsh_destroy(xsh_handle* h);
 

@@ -22,7 +22,7 @@ OBSERVE = {"a": 0, "abs2": 1, "a_ave": 2, "abs2_ave": 3, "abs2_int": 4, "abs4_av
 STATUS = {0: "XSH_OK", 1: "XSH_EINVAL", 2: "XSH_EUNSUPPORTED", 3: "XSH_ENOMEM", 4: "XSH_EDEVICE",
           5: "XSH_ESTATE"}
 
-EXPORTS = ["xsh_version", "xsh_last_error", "xsh_create", "xsh_destroy", "xsh_query", "xsh_run_path",
+EXPORTS = ["xsh_version", "xsh_last_error", "xsh_create", "xsh_plan", "xsh_destroy", "xsh_query", "xsh_run_path",
            "xsh_run_path_noalloc",
            "xsh_run_ensemble", "xsh_synchronize", "xsh_read_averages", "xsh_read_trajectories", "xsh_read_raw",
            "xsh_nonfinite", "xsh_batch_nonfinite",
@@ -108,6 +108,7 @@ def load():
     lib.xsh_destroy.argtypes = [C.c_void_p]
     lib.xsh_destroy.restype = None
     lib.xsh_query.argtypes = [C.c_void_p, C.POINTER(Info)]
+    lib.xsh_plan.argtypes = [C.POINTER(Desc), C.POINTER(Info)]
     lib.xsh_run_path.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, dp]
     lib.xsh_run_path_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p]
@@ -153,6 +154,15 @@ def _ptr(a):
     if a is None:
         return None
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def plan(desc):
+    """xsh_plan: the Info xsh_create would produce, with no device (raises
+    XshError with XSH_EUNSUPPORTED for projects that stay on the CPU path)."""
+    desc.struct_size = C.sizeof(Desc)
+    info = Info()
+    check(load().xsh_plan(C.byref(desc), C.byref(info)))
+    return info
 
 
 class Handle:

@@ -268,19 +268,13 @@ static std::vector<double> dft_matrix(int N, bool inverse) {
   return T;
 }
 
-extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
-  if (!out) return fail(XSH_EINVAL, "out is NULL");
-  *out = nullptr;
-  int rc = validate(desc);
-  if (rc) return rc;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(XSH_EDEVICE, "no HIP device");
-  auto* h = new xsh_handle();
-  std::memset(h, 0, sizeof(*h));
-  if (hipGetDevice(&h->device) != hipSuccess) {
-    delete h;
-    return fail(XSH_EDEVICE, "hipGetDevice");
-  }
+// Everything xsh_create derives on the host before it touches a device:
+// the lattice (xpreferences.m:133-175, spdpreferences.m:16-72), the kernel
+// for it, the averages layout and every device table (propagators
+// xpropfactor.m:23-58, twiddles, transform matrices and their matrix-core
+// operand copies, boundary patches and values, noise filters) packed into
+// host.  No HIP call: xsh_plan runs it alone.
+static int plan_handle(xsh_handle* h, const xsh_desc* desc, std::vector<char>& host, size_t& total) {
   h->d = *desc;
   const xsh_desc& d = h->d;
   h->nd = d.dimensions;
@@ -330,7 +324,6 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
             dirichlet = dirichlet || d.boundaries[k][side] == 1;
           }
         if (values || dirichlet) {
-          delete h;
           return fail(XSH_EUNSUPPORTED, "the streaming %dx%d dense-transform kernel takes homogeneous Neumann and "
                                         "periodic dimensions (no Dirichlet ends or boundary values)", npt[0], npt[1]);
         }
@@ -347,13 +340,11 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
                   d.observe[o] == XSH_OBS_LOWK;
       if (fft_obs) {
         const int np = h->kern->nx;
-        delete h;
         return fail(XSH_EUNSUPPORTED, "corr, abs2k, lowk and noise modes need a lattice of a compiled size "
                                       "(%d x %d sites run padded to %d)", npt[0], npt[1], np);
       }
     }
     if (!h->kern) {
-      delete h;
       return fail(XSH_EUNSUPPORTED, "no %s %s kernel for a %dx%d lattice", d.precision == XSH_FP64 ? "fp64" : "fp32",
                   h->mat ? "non-periodic" : "periodic", npt[0], npt[1]);
     }
@@ -362,7 +353,6 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     h->npad[0] = h->kern->nx;
     h->npad[1] = h->kern->ny;
     if ((d.knoises || d.krandoms) && (h->mat || !h->kern->kn)) {
-      delete h;
       return fail(XSH_EUNSUPPORTED, "k-space noises run on periodic lattices with register-resident fields "
                                     "(not the %s kernel)", h->mat ? "dense-transform" : "streaming");
     }
@@ -373,7 +363,6 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
       bool lowk = false;
       for (int o = 0; o < d.nobserve; ++o) lowk = lowk || d.observe[o] == XSH_OBS_LOWK;
       if (d.nmodes > 0 || lowk) {
-        delete h;
         return fail(XSH_EUNSUPPORTED, "noise modes and the lowk observable are not on the streaming %dx%d kernel",
                     npt[0], npt[1]);
       }
@@ -477,7 +466,7 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   }
   h->cplx[0] = cplx[0];
   h->cplx[1] = cplx[1];
-  size_t total = 0;
+  total = 0;
   for (int i = 0; i < 6; ++i) {
     h->tab_off[i] = total;
     total += ((tabs[i].size() * h->esize + 255) / 256) * 256;
@@ -587,7 +576,7 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
   total += ((kfil.size() * h->esize + 255) / 256) * 256;
   h->tab_off[18] = total;
   total += ((rfil.size() * h->esize + 255) / 256) * 256;
-  std::vector<char> host(total + 256);
+  host.assign(total + 256, 0);
   for (int i = 0; i < 6; ++i) {
     if (d.precision == XSH_FP64)
       pack<double>(host, h->tab_off[i], tabs[i]);
@@ -622,6 +611,33 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     pack<float>(host, h->tab_off[17], kfil);
     pack<float>(host, h->tab_off[18], rfil);
   }
+  // per-trajectory outputs index trajectories by workgroup
+  if ((d.traj_out || d.nmodes) && h->tpb != 1)
+    return fail(XSH_EUNSUPPORTED, "per-trajectory outputs need one trajectory per workgroup (2D lattices)");
+  return XSH_OK;
+}
+
+extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
+  if (!out) return fail(XSH_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = validate(desc);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(XSH_EDEVICE, "no HIP device");
+  auto* h = new xsh_handle();
+  std::memset(h, 0, sizeof(*h));
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return fail(XSH_EDEVICE, "hipGetDevice");
+  }
+  std::vector<char> host;
+  size_t total = 0;
+  rc = plan_handle(h, desc, host, total);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  const xsh_desc& d = h->d;
   auto cleanup = [&](int code) {
     xsh_destroy(h);
     return code;
@@ -634,8 +650,6 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     return cleanup(fail(XSH_ENOMEM, "hipMalloc averages (%zu bytes)", avbytes));
   if (hipMalloc(&h->d_path_av, (size_t)h->avlen * 16) != hipSuccess) return cleanup(fail(XSH_ENOMEM, "hipMalloc path averages"));
   if (hipMalloc(&h->d_a, (size_t)h->E * h->ns * 16) != hipSuccess) return cleanup(fail(XSH_ENOMEM, "hipMalloc fields"));
-  if ((d.traj_out || d.nmodes) && h->tpb != 1)
-    return cleanup(fail(XSH_EUNSUPPORTED, "per-trajectory outputs need one trajectory per workgroup (2D lattices)"));
   const size_t paths = (size_t)h->e2 * h->nchk;
   if (d.traj_out &&
       hipMalloc(&h->d_traj, paths * h->E * d.points * d.nobserve * sizeof(double)) != hipSuccess)
@@ -704,6 +718,20 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     return cleanup(fail(XSH_ENOMEM, "hipHostMalloc arguments"));
   *out = h;
   return XSH_OK;
+}
+
+extern "C" int xsh_plan(const xsh_desc* desc, xsh_info* info) {
+  if (!info) return fail(XSH_EINVAL, "info is NULL");
+  int rc = validate(desc);
+  if (rc) return rc;
+  auto* h = new xsh_handle();
+  std::memset(h, 0, sizeof(*h));
+  std::vector<char> host;
+  size_t total = 0;
+  rc = plan_handle(h, desc, host, total);
+  if (!rc) rc = xsh_query(h, info);
+  delete h;
+  return rc;
 }
 
 extern "C" void xsh_destroy(xsh_handle* h) {
