@@ -64,6 +64,7 @@ struct xsh_handle {
                     // matrix-core operand copies of the real matrices
   size_t tab_off[19];
   bool has_tmp[4];
+  int tsym[4];         // per packed matrix: even/odd split (1 DCT-I, 2 DST-I) or 0
   bool has_patch;      // non-zero boundary values on a non-periodic dimension
   int bval_level;      // boundary-value table: elements per time level (0: one level)
   int bval_nb;         // values per boundary line
@@ -481,18 +482,31 @@ static int plan_handle(xsh_handle* h, const xsh_desc* desc, std::vector<char>& h
   // precision (used by kernels that run those passes on MFMA):
   //   FP32, N % 32 == 0, 32x32x2: Tp[kb][kk/2][lane] = T[32 kb + lane%32][kk + lane/32]
   //   FP64, N % 16 == 0, 16x16x4: Tp[kb][kk/4][lane] = T[16 kb + lane%16][kk + lane/16]
+  // A DCT-I (NN) or DST-I (DD) matrix of an unpadded dimension with N % (2 tile)
+  // == 0 is packed as its even/odd split instead, half the GEMM depth:
+  // T[k][N-1-n] = (-1)^k T[k][n] (DCT-I), -(-1)^k T[k][n] (DST-I), so the even
+  // rows act on x_n +- x_{N-1-n} and the odd rows on x_n -+ x_{N-1-n}, n < N/2:
+  //   Ts[r][n] = T[2r][n] (r < N/2), T[2(r - N/2) + 1][n] (r >= N/2), n < N/2
   std::vector<std::vector<double>> mpk(4);
   const int tile = d.precision == XSH_FP64 ? 16 : 32, kstep = 64 / tile;
   for (int i = 0; i < 4; ++i) {
     const int k = i % 2, n = h->npad[k];
+    h->tsym[i] = 0;
     h->has_tmp[i] = h->mat && k < d.dimensions - 1 && !cplx[k] && n % tile == 0 && mats[i].size() == (size_t)n * n;
     if (!h->has_tmp[i]) continue;
-    mpk[i].resize((size_t)n * n);
+    const int bt = h->btype[k];
+    if ((bt == -3 || bt == 3) && npt[k] == n && n % (2 * tile) == 0) h->tsym[i] = bt == -3 ? 1 : 2;
+    const int K = h->tsym[i] ? n / 2 : n;
+    auto Tsrc = [&](int r, int c) -> double {
+      if (!h->tsym[i]) return mats[i][(size_t)r * n + c];
+      const int row = r < n / 2 ? 2 * r : 2 * (r - n / 2) + 1;
+      return mats[i][(size_t)row * n + c];
+    };
+    mpk[i].resize((size_t)n * K);
     for (int kb = 0; kb < n / tile; ++kb)
-      for (int kk = 0; kk < n; kk += kstep)
+      for (int kk = 0; kk < K; kk += kstep)
         for (int lane = 0; lane < 64; ++lane)
-          mpk[i][((size_t)kb * (n / kstep) + kk / kstep) * 64 + lane] =
-              mats[i][(size_t)(tile * kb + lane % tile) * n + kk + lane / tile];
+          mpk[i][((size_t)kb * (K / kstep) + kk / kstep) * 64 + lane] = Tsrc(tile * kb + lane % tile, kk + lane / tile);
   }
   for (int i = 0; i < 4; ++i) {
     h->tab_off[11 + i] = total;
@@ -858,7 +872,10 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
   for (int nci = 0; nci < 2; ++nci)
     for (int k = 0; k < 2; ++k)
       for (int c = 0; c < 2; ++c) A.lpf[nci][k][c] = h->lpf[nci][k][c];
-  for (int i = 0; i < 4; ++i) A.tmp[i] = h->has_tmp[i] ? t + h->tab_off[11 + i] : nullptr;
+  for (int i = 0; i < 4; ++i) {
+    A.tmp[i] = h->has_tmp[i] ? t + h->tab_off[11 + i] : nullptr;
+    A.tsym[i] = h->tsym[i];
+  }
   A.work = h->d_work;
   A.work_planes = h->work_planes;
   A.traj = h->d_traj;

@@ -531,7 +531,7 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     if (A.has_linear) {
       if constexpr (MAT) {
         patch(f, R(-1), false, lev);
-        Lat::spectral_mat(L, f, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop);
+        Lat::spectral_mat(L, f, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop, A.tsym);
         patch(f, R(1), true, lev);
       } else {
         periodic_spectral<R, NX, NY, NT, TPB, MAT, RX, KN>(L, f, twx, twy, prop);
@@ -901,7 +901,7 @@ __global__ __launch_bounds__(NT) void probe_propagate(const PathArgs* __restrict
     }
   const auto prop = make_prop<R, NX, NY, NT, TPB, MAT, RX, false>(A, nci);
   if constexpr (MAT)
-    Lat::spectral_mat(L, f, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop);
+    Lat::spectral_mat(L, f, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop, A.tsym);
   else
     periodic_spectral<R, NX, NY, NT, TPB, MAT, RX, false>(L, f, static_cast<const C*>(A.twx),
                                                           static_cast<const C*>(A.twy), prop);

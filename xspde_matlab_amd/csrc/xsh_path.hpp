@@ -61,6 +61,12 @@ struct PathArgs {
   const void* tmp[4];      // real matrices in MFMA operand order, else NULL: FP32 (N % 32 == 0)
                            // Tp[kb][kk/2][lane] = T[32 kb + lane%32][kk + lane/32],
                            // FP64 (N % 16 == 0) Tp[kb][kk/4][lane] = T[16 kb + lane%16][kk + lane/16]
+  int tsym[4];             // per packed matrix: 0 the full N x N GEMM; 1 / 2 the
+                           // even/odd split of a DCT-I / DST-I (T[k][N-1-n] =
+                           // +-(-1)^k T[k][n]): rows [0, N/2) hold T[2r][n], rows
+                           // [N/2, N) T[2r+1][n], n < N/2, applied to
+                           // x_n + sgn x_{N-1-n} with sgn = +1 for the even rows
+                           // of a DCT-I and the odd rows of a DST-I, else -1
   int dirichlet;           // any Dirichlet end (xsetbound.m:40-53)
   double* traj;            // optional [paths][E][npts][nobs] per-trajectory lattice means
   double* raw;             // optional [paths][npts][E][ns] complex double fields
@@ -437,27 +443,37 @@ struct Lattice {
     static constexpr bool ok = std::is_same<R, float>::value && (NY > 1 ? TPB == 1 : DIM == 0) &&
                                N % 32 == 0 && NL % 32 == 0 && PW > 0 && LB % PW == 0;
   };
-  template <int DIM, class Op>
-  __device__ static __forceinline__ void mat_mfma(C* L, const float* __restrict__ Tp, const Op& op) {
+  template <int DIM, bool SYM, class Op>
+  __device__ static __forceinline__ void mat_mfma(C* L, const float* __restrict__ Tp, int sym, const Op& op) {
     using Pl = MfmaPlan<DIM>;
-    constexpr int N = Pl::N, PW = Pl::PW, LB = Pl::LB;
+    constexpr int N = Pl::N, PW = Pl::PW, LB = Pl::LB, KB = Pl::KB;
+    // SYM: half the GEMM depth; row tiles [0, KB/2) are the even k, the rest
+    // the odd k, and the input pairs (n, N-1-n) fold into x_n + sgn x_{N-1-n}
+    constexpr int K = SYM ? N / 2 : N;
+    static_assert(!SYM || KB % 2 == 0, "the even/odd split needs N % 64 == 0");
     typedef float v16f __attribute__((ext_vector_type(16)));
     const int lane = threadIdx.x & 63;
     const int p0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * PW;
     const int kb = p0 / LB;
+    const bool odd = SYM && kb >= KB / 2;
+    const float sgn = odd == (sym == 1) ? -1.f : 1.f;
     v16f acc[2 * PW];
     static_for<0, 2 * PW>([&](auto T) __attribute__((always_inline)) {
       acc[decltype(T)::value] = v16f{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     });
-    const float* __restrict__ Ta = Tp + (size_t)kb * (N / 2) * 64 + lane;
+    const float* __restrict__ Ta = Tp + (size_t)kb * (K / 2) * 64 + lane;
 #pragma unroll 4
-    for (int kk = 0; kk < N; kk += 2) {
+    for (int kk = 0; kk < K; kk += 2) {
       const float a = Ta[(kk / 2) * 64];
       const int n = kk + (lane >> 5);
       static_for<0, PW>([&](auto T) __attribute__((always_inline)) {
         constexpr int t = decltype(T)::value;
         const int ln = ((p0 + t) % LB) * 32 + (lane & 31);
-        const C x = L[DIM == 0 ? ln * RS + n : n * RS + ln];
+        C x = L[DIM == 0 ? ln * RS + n : n * RS + ln];
+        if constexpr (SYM) {
+          const C y = L[DIM == 0 ? ln * RS + (N - 1 - n) : (N - 1 - n) * RS + ln];
+          x = C{fmad(sgn, y.x, x.x), fmad(sgn, y.y, x.y)};
+        }
         acc[2 * t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, x.x, acc[2 * t], 0, 0, 0);
         acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, x.y, acc[2 * t + 1], 0, 0, 0);
       });
@@ -468,7 +484,8 @@ struct Lattice {
       const int ln = ((p0 + t) % LB) * 32 + (lane & 31);
       static_for<0, 16>([&](auto Rg) __attribute__((always_inline)) {
         constexpr int r = decltype(Rg)::value;
-        const int k = kb * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+        const int kr = kb * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+        const int k = SYM ? (kr < N / 2 ? 2 * kr : 2 * (kr - N / 2) + 1) : kr;
         const C X = op(C{acc[2 * t][r], acc[2 * t + 1][r]}, ln, k);
         L[DIM == 0 ? ln * RS + k : k * RS + ln] = X;
       });
@@ -489,27 +506,35 @@ struct Lattice {
     static constexpr bool ok = std::is_same<R, double>::value && (NY > 1 ? TPB == 1 : DIM == 0) &&
                                N % 16 == 0 && NL % 16 == 0 && PW > 0 && LB % PW == 0;
   };
-  template <int DIM, class Op>
-  __device__ static __forceinline__ void mat_mfma64(C* L, const double* __restrict__ Tp, const Op& op) {
+  template <int DIM, bool SYM, class Op>
+  __device__ static __forceinline__ void mat_mfma64(C* L, const double* __restrict__ Tp, int sym, const Op& op) {
     using Pl = MfmaPlan64<DIM>;
-    constexpr int N = Pl::N, PW = Pl::PW, LB = Pl::LB;
+    constexpr int N = Pl::N, PW = Pl::PW, LB = Pl::LB, KB = Pl::KB;
+    constexpr int K = SYM ? N / 2 : N;  // the even/odd split as in mat_mfma
+    static_assert(!SYM || KB % 2 == 0, "the even/odd split needs N % 32 == 0");
     typedef double v4d __attribute__((ext_vector_type(4)));
     const int lane = threadIdx.x & 63;
     const int p0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * PW;
     const int kb = p0 / LB;
+    const bool odd = SYM && kb >= KB / 2;
+    const double sgn = odd == (sym == 1) ? -1.0 : 1.0;
     v4d acc[2 * PW];
     static_for<0, 2 * PW>([&](auto T) __attribute__((always_inline)) {
       acc[decltype(T)::value] = v4d{0.0, 0.0, 0.0, 0.0};
     });
-    const double* __restrict__ Ta = Tp + (size_t)kb * (N / 4) * 64 + lane;
+    const double* __restrict__ Ta = Tp + (size_t)kb * (K / 4) * 64 + lane;
 #pragma unroll 4
-    for (int kk = 0; kk < N; kk += 4) {
+    for (int kk = 0; kk < K; kk += 4) {
       const double a = Ta[(kk / 4) * 64];
       const int n = kk + (lane >> 4);
       static_for<0, PW>([&](auto T) __attribute__((always_inline)) {
         constexpr int t = decltype(T)::value;
         const int ln = ((p0 + t) % LB) * 16 + (lane & 15);
-        const C x = L[DIM == 0 ? ln * RS + n : n * RS + ln];
+        C x = L[DIM == 0 ? ln * RS + n : n * RS + ln];
+        if constexpr (SYM) {
+          const C y = L[DIM == 0 ? ln * RS + (N - 1 - n) : (N - 1 - n) * RS + ln];
+          x = C{fmad(sgn, y.x, x.x), fmad(sgn, y.y, x.y)};
+        }
         acc[2 * t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, x.x, acc[2 * t], 0, 0, 0);
         acc[2 * t + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, x.y, acc[2 * t + 1], 0, 0, 0);
       });
@@ -520,7 +545,8 @@ struct Lattice {
       const int ln = ((p0 + t) % LB) * 16 + (lane & 15);
       static_for<0, 4>([&](auto Rg) __attribute__((always_inline)) {
         constexpr int r = decltype(Rg)::value;
-        const int k = kb * 16 + 4 * r + (lane >> 4);  // D[4 r + lane / 16][lane % 16]
+        const int kr = kb * 16 + 4 * r + (lane >> 4);  // D[4 r + lane / 16][lane % 16]
+        const int k = SYM ? (kr < N / 2 ? 2 * kr : 2 * (kr - N / 2) + 1) : kr;
         const C X = op(C{acc[2 * t][r], acc[2 * t + 1][r]}, ln, k);
         L[DIM == 0 ? ln * RS + k : k * RS + ln] = X;
       });
@@ -528,16 +554,29 @@ struct Lattice {
   }
 
   template <int DIM, class Op>
-  __device__ static __forceinline__ void mat_any(C* L, const void* T, const void* Tp, bool cplx, const Op& op) {
+  __device__ static __forceinline__ void mat_any(C* L, const void* T, const void* Tp, bool cplx, const Op& op,
+                                                 int sym = 0) {
     if constexpr (MfmaPlan<DIM>::ok) {
       if (!cplx && Tp) {
-        mat_mfma<DIM>(L, static_cast<const float*>(Tp), op);
+        if constexpr (MfmaPlan<DIM>::KB % 2 == 0) {
+          if (sym) {
+            mat_mfma<DIM, true>(L, static_cast<const float*>(Tp), sym, op);
+            return;
+          }
+        }
+        mat_mfma<DIM, false>(L, static_cast<const float*>(Tp), 0, op);
         return;
       }
     }
     if constexpr (MfmaPlan64<DIM>::ok) {
       if (!cplx && Tp) {
-        mat_mfma64<DIM>(L, static_cast<const double*>(Tp), op);
+        if constexpr (MfmaPlan64<DIM>::KB % 2 == 0) {
+          if (sym) {
+            mat_mfma64<DIM, true>(L, static_cast<const double*>(Tp), sym, op);
+            return;
+          }
+        }
+        mat_mfma64<DIM, false>(L, static_cast<const double*>(Tp), 0, op);
         return;
       }
     }
@@ -552,25 +591,25 @@ struct Lattice {
   template <class Op>
   __device__ static __forceinline__ void spectral_mat(C* L, C (&f)[PPT], const void* const* Tm,
                                                       const void* const* Tp, int cplx_x, int cplx_y,
-                                                      const Op& op) {
+                                                      const Op& op, const int* tsym) {
     auto id = [](C X, int, int) __attribute__((always_inline)) { return X; };
     __syncthreads();
     store(L, f);
     __syncthreads();
     if constexpr (NY == 1) {
       mat_any<0>(L, Tm[0], Tp[0], cplx_x,
-                 [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, k, 0); });
+                 [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, k, 0); }, tsym[0]);
       __syncthreads();
-      mat_any<0>(L, Tm[2], Tp[2], cplx_x, id);
+      mat_any<0>(L, Tm[2], Tp[2], cplx_x, id, tsym[2]);
     } else {
-      mat_any<0>(L, Tm[0], Tp[0], cplx_x, id);
+      mat_any<0>(L, Tm[0], Tp[0], cplx_x, id, tsym[0]);
       __syncthreads();
       mat_any<1>(L, Tm[1], Tp[1], cplx_y,
-                 [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, l % NX, k); });
+                 [&](C X, int l, int k) __attribute__((always_inline)) { return op(X, l % NX, k); }, tsym[1]);
       __syncthreads();
-      mat_any<1>(L, Tm[3], Tp[3], cplx_y, id);
+      mat_any<1>(L, Tm[3], Tp[3], cplx_y, id, tsym[3]);
       __syncthreads();
-      mat_any<0>(L, Tm[2], Tp[2], cplx_x, id);
+      mat_any<0>(L, Tm[2], Tp[2], cplx_x, id, tsym[2]);
     }
     __syncthreads();
     load(L, f);

@@ -83,14 +83,14 @@ struct Stream {
   // of a mixed lattice.  op(X, kx, ky) in the transforms' natural k order.
   template <class Op>
   __device__ static __forceinline__ void propagate_mat(C* L, C* F, const void* const* Tm, const void* const* Tp,
-                                                        int cplx_x, int cplx_y, const Op& op) {
+                                                        int cplx_x, int cplx_y, const Op& op, const int* tsym) {
     auto id = [](C X, int, int) __attribute__((always_inline)) { return X; };
 #pragma unroll 1
     for (int h = 0; h < 2; ++h) {
       __syncthreads();
       to_lds(L, F + h * NSH);
       __syncthreads();
-      Lh::template mat_any<0>(L, Tm[0], Tp[0], cplx_x, id);
+      Lh::template mat_any<0>(L, Tm[0], Tp[0], cplx_x, id, tsym[0]);
       __syncthreads();
       from_lds(L, F + h * NSH);
     }
@@ -101,9 +101,9 @@ struct Stream {
       __syncthreads();
       Lv::template mat_any<1>(L, Tm[1], Tp[1], cplx_y, [&](C X, int l, int k) __attribute__((always_inline)) {
         return op(X, c * (NX / 2) + l, k);
-      });
+      }, tsym[1]);
       __syncthreads();
-      Lv::template mat_any<1>(L, Tm[3], Tp[3], cplx_y, id);
+      Lv::template mat_any<1>(L, Tm[3], Tp[3], cplx_y, id, tsym[3]);
       __syncthreads();
       cols_from_lds(L, F, c);
     }
@@ -112,7 +112,7 @@ struct Stream {
       __syncthreads();
       to_lds(L, F + h * NSH);
       __syncthreads();
-      Lh::template mat_any<0>(L, Tm[2], Tp[2], cplx_x, id);
+      Lh::template mat_any<0>(L, Tm[2], Tp[2], cplx_x, id, tsym[2]);
       __syncthreads();
       from_lds(L, F + h * NSH);
     }
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
     auto propagate = [&](C* F) __attribute__((always_inline)) {
       if (A.has_linear) {
         if constexpr (MAT)
-          S::propagate_mat(L, F, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop);
+          S::propagate_mat(L, F, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop, A.tsym);
         else
           S::propagate(L, F, twx, twy, twh, prop);
       }
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(NT) void probe_stream(const PathArgs* __restrict__ 
   const PropOp2<R> prop{static_cast<const C*>(nci ? A.px[1] : A.px[0]),
                        static_cast<const C*>(nci ? A.py[1] : A.py[0])};
   if constexpr (MAT)
-    S::propagate_mat(L, planes + (long long)blockIdx.x * S::NS, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop);
+    S::propagate_mat(L, planes + (long long)blockIdx.x * S::NS, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop, A.tsym);
   else
     S::propagate(L, planes + (long long)blockIdx.x * S::NS, static_cast<const C*>(A.twx),
                  static_cast<const C*>(A.twy), static_cast<const C*>(A.twh), prop);
