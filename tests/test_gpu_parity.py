@@ -623,6 +623,29 @@ def test_stream_dense_fp64_128(bnd, method):
     _path_case(p, 1e-10)
 
 
+@pytest.mark.parametrize("method", ["RK4", "MP"])
+@pytest.mark.parametrize("values", ["boundval", "boundfun"])
+def test_stream_dense_fp64_128_boundary_values(method, values):
+    """FP64 128x128 with Dirichlet ends and non-zero boundary values on the
+    streaming kernel (fields in device memory): Neumann x with derivative
+    values, Dirichlet y with field values, constant (p.boundval) or changing
+    along the lines and in time (p.boundfun): the patch around every
+    propagation, the Neumann correction and the Dirichlet sites at the
+    method's stage times, against the oracle, fine and coarse paths."""
+    p = shrink(cgl2d("fp64", n=128), E=2, e2=2, points=3, steps=2)
+    p["method"] = method
+    p["boundaries"] = [[-1, -1], [1, 1]]
+    if values == "boundval":
+        p["boundval"] = [[0.02, -0.01 + 0.005j], [0.05 - 0.02j, 0.03]]
+    else:
+        p["boundfun"] = _ends_2d
+    p["gpu"]["observe"] = ["abs2_ave", "abs2", "a_ave"]
+    prob = Problem(p)
+    assert prob.info.kernel_kind == 1
+    prob.close()
+    _path_case(p, 1e-10)
+
+
 def test_unsupported_combinations_rejected():
     """Combinations outside the GPU path fail at creation with XSH_EUNSUPPORTED
     (the host then keeps the project on the CPU path), before any launch."""
@@ -639,9 +662,6 @@ def test_unsupported_combinations_rejected():
     p = shrink(cgl2d("fp32", n=64), E=2, e2=1, points=2, steps=1)
     p["boundaries"] = [[-1, -1], [-1, -1]]
     p.update(noises=0, knoises=2, nfilter=_kfilter_n)    # k-space noise on a dense-transform lattice
-    cases.append(p)
-    p = shrink(cgl2d("fp64", n=128), E=2, e2=1, points=2, steps=1)
-    p["boundaries"] = [[-1, -1], [1, 1]]                 # Dirichlet ends on the streaming kernel
     cases.append(p)
     for p in cases:
         with pytest.raises(XshError, match="XSH_EUNSUPPORTED"):

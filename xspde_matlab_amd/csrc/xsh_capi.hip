@@ -313,22 +313,8 @@ static int plan_handle(xsh_handle* h, const xsh_desc* desc, std::vector<char>& h
       h->kern = xsh::find_kernel(d.precision, h->nx, h->ny, 1);
     }
     if (!h->kern && h->mat) {
-      // fields too large for the chip: dense transforms on the streaming kernel,
-      // which has no boundary patches or Dirichlet ends (homogeneous Neumann and
-      // periodic dimensions)
+      // fields too large for the chip: dense transforms on the streaming kernel
       h->kern = xsh::find_stream_kernel(d.precision, h->nx, h->ny, 1);
-      if (h->kern) {
-        bool values = d.boundtab != nullptr, dirichlet = false;
-        for (int k = 0; k < d.dimensions - 1; ++k)
-          for (int side = 0; side < 2; ++side) {
-            values = values || d.boundval[k][side][0] != 0.0 || d.boundval[k][side][1] != 0.0;
-            dirichlet = dirichlet || d.boundaries[k][side] == 1;
-          }
-        if (values || dirichlet) {
-          return fail(XSH_EUNSUPPORTED, "the streaming %dx%d dense-transform kernel takes homogeneous Neumann and "
-                                        "periodic dimensions (no Dirichlet ends or boundary values)", npt[0], npt[1]);
-        }
-      }
     }
     if (!h->kern && h->mat && d.dimensions == 2) h->kern = xsh::find_padded_kernel(d.precision, h->nx);
     if (!h->kern && h->mat && d.dimensions == 3) h->kern = xsh::find_padded_kernel2d(d.precision, h->nx, h->ny);

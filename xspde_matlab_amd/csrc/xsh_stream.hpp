@@ -263,19 +263,88 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
         w1 = R(0.5) * (w1 + u1);
       }
     };
-    auto propagate = [&](C* F) __attribute__((always_inline)) {
-      if (A.has_linear) {
-        if constexpr (MAT)
-          S::propagate_mat(L, F, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop, A.tsym);
-        else
-          S::propagate(L, F, twx, twy, twh, prop);
+    // Dirichlet ends and boundary values (xsetbound.m:30-62, xpatch.m:24-62)
+    // on the planes in device memory, with the register-resident kernel's
+    // rules: time level lev in half fine steps, y values win at corners, the
+    // patch pa = sum_d b_lower s0(x_d) + b_upper s1(x_d) around every
+    // propagation and the Neumann correction lp after it.  Each thread touches
+    // its own sites only.
+    auto dir_end = [&](int st, int& dd, int& sd) __attribute__((always_inline)) -> bool {
+      const int iy = st / NX, ix = st - iy * NX;
+      dd = -1;
+      sd = 0;
+      if (ix == 0 && A.dir_lo[0]) dd = 0, sd = 0;
+      if (ix == NX - 1 && A.dir_hi[0]) dd = 0, sd = 1;
+      if (iy == 0 && A.dir_lo[1]) dd = 1, sd = 0;
+      if (iy == NY - 1 && A.dir_hi[1]) dd = 1, sd = 1;
+      return dd >= 0;
+    };
+    auto bvalue = [&](int lev, int dd, int sd, int o) __attribute__((always_inline)) -> C {
+      const C* bv = static_cast<const C*>(A.bval);
+      return bv[(long long)lev * A.bval_level + (dd * 2 + sd) * A.bval_nb + o];
+    };
+    // the value a Dirichlet site takes at level lev (0 without boundary values)
+    auto dir_value = [&](int st, int dd, int sd, int lev) __attribute__((always_inline)) -> C {
+      const int iy = st / NX, ix = st - iy * NX;
+      return A.has_patch ? bvalue(lev, dd, sd, dd == 0 ? iy : ix) : C{R(0), R(0)};
+    };
+    auto setbound = [&](C* F, int lev) __attribute__((always_inline)) {
+      if constexpr (!MAT) return;
+      if (!A.dirichlet) return;
+#pragma unroll 1
+      for (int i = 0; i < PAIRS; ++i) {
+        const int s = S::site(i);
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          int dd, sd;
+          if (dir_end(s + v, dd, sd)) F[s + v] = dir_value(s + v, dd, sd, lev);
+        }
       }
     };
+    auto patch = [&](C* F, R sign, bool with_lp, int lev) __attribute__((always_inline)) {
+      if constexpr (!MAT) return;
+      if (!A.has_patch) return;
+      const R* sh = static_cast<const R*>(A.bshape);
+#pragma unroll 1
+      for (int i = 0; i < PAIRS; ++i) {
+        const int s = S::site(i);
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          const int st = s + v, iy = st / NX, ix = st - iy * NX;
+          C acc = C{R(0), R(0)};
+#pragma unroll
+          for (int dd = 0; dd < 2; ++dd) {
+            const int m = dd == 0 ? ix : iy, o = dd == 0 ? iy : ix;
+            const C b0 = bvalue(lev, dd, 0, o), b1 = bvalue(lev, dd, 1, o);
+            acc = acc + sign * (sh[(dd * 2) * A.bshape_n + m] * b0 + sh[(dd * 2 + 1) * A.bshape_n + m] * b1);
+            if (with_lp) acc = acc + C{(R)A.lpf[nci][dd][0], (R)A.lpf[nci][dd][1]} * (b1 - b0);
+          }
+          F[st] = F[st] + acc;
+        }
+      }
+    };
+    // xprop.m:17-46: patch, transform, propagator, inverse transform, patch, setbound
+    auto propagate = [&](C* F, int lev) __attribute__((always_inline)) {
+      if (A.has_linear) {
+        if constexpr (MAT) {
+          patch(F, R(-1), false, lev);
+          S::propagate_mat(L, F, A.tm, A.tmp, A.cplx[0], A.cplx[1], prop, A.tsym);
+          patch(F, R(1), true, lev);
+        } else {
+          S::propagate(L, F, twx, twy, twh, prop);
+        }
+      }
+      setbound(F, lev);
+    };
 
-    auto method_step = [&](uint32_t q, bool coarse) __attribute__((always_inline)) {
+    // lev: time level of the step's start in half fine steps (a coarse step
+    // spans twice the levels of a fine one)
+    auto method_step = [&](uint32_t q, bool coarse, int lev) __attribute__((always_inline)) {
+      const int hl = coarse ? 2 : 1;
       __syncthreads();
       if (A.method == XSH_RK4) {
         // RK4.m:9-14: a1 = a + D(a) dt into F1
+        setbound(F0, lev);
 #pragma unroll 1
         for (int i = 0; i < PAIRS; ++i) {
           const int s = S::site(i);
@@ -284,9 +353,9 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
 #pragma unroll
           for (int v = 0; v < 2; ++v) F1[s + v] = F0[s + v] + hdt * D(F0[s + v], w[v]);
         }
-        propagate(F0);
-        propagate(F1);
-        // RK4.m:18-32: a3 -> F0, R -> F1
+        propagate(F0, lev);
+        propagate(F1, lev);
+        // RK4.m:18-32: a3 -> F0, R -> F1 (Dirichlet sites at the midpoint's values)
 #pragma unroll 1
         for (int i = 0; i < PAIRS; ++i) {
           const int s = S::site(i);
@@ -294,15 +363,19 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
           noise(s, q, coarse, w[0], w[1]);
 #pragma unroll
           for (int v = 0; v < 2; ++v) {
+            int dd = -1, sd = 0;
+            const bool dz = MAT && A.dirichlet && dir_end(s + v, dd, sd);
             const C A0 = F0[s + v], A1 = F1[s + v];
-            const C a2 = A0 + hdt * D(A1, w[v]);
-            const C a3 = A0 + (R(2) * hdt) * D(a2, w[v]);
+            C a2 = A0 + hdt * D(A1, w[v]);
+            if (dz) a2 = dir_value(s + v, dd, sd, lev + hl);
+            C a3 = A0 + (R(2) * hdt) * D(a2, w[v]);
+            if (dz) a3 = dir_value(s + v, dd, sd, lev + hl);
             F1[s + v] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
             F0[s + v] = a3;
           }
         }
-        propagate(F0);
-        propagate(F1);
+        propagate(F0, lev + 2 * hl);
+        propagate(F1, lev + 2 * hl);
         // RK4.m:33-41: a = P R + D(P a3) dt/3
         __syncthreads();
 #pragma unroll 1
@@ -313,9 +386,10 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
 #pragma unroll
           for (int v = 0; v < 2; ++v) F0[s + v] = F1[s + v] + (hdt / R(3)) * D(F0[s + v], w[v]);
         }
+        setbound(F0, lev + 2 * hl);
       } else {
         // MP.m:13-24
-        propagate(F0);
+        propagate(F0, lev);
 #pragma unroll 1
         for (int i = 0; i < PAIRS; ++i) {
           const int s = S::site(i);
@@ -323,13 +397,18 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
           noise(s, q, coarse, w[0], w[1]);
 #pragma unroll
           for (int v = 0; v < 2; ++v) {
+            int dd = -1, sd = 0;
+            const bool dz = MAT && A.dirichlet && dir_end(s + v, dd, sd);
             const C a0 = F0[s + v];
             C a = a0;
-            for (int it = 0; it < A.iterations; ++it) a = a0 + hdt * D(a, w[v]);
+            for (int it = 0; it < A.iterations; ++it) {
+              a = a0 + hdt * D(a, w[v]);
+              if (dz) a = dir_value(s + v, dd, sd, lev + hl);
+            }
             F0[s + v] = R(2) * a - a0;
           }
         }
-        propagate(F0);
+        propagate(F0, lev + 2 * hl);
       }
     };
 
@@ -427,14 +506,19 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
 
     // ---- the time loop of xpath.m:29-86
     uint32_t q = 0;
+    int lev = 0;
     for (int n = 0; n < A.npts; ++n) {
       for (int st = 0; st < A.steps; ++st) {
         if (n > 0 || st > 0 || A.npts == 1) {
           if (A.checks && nci == 1) {
-            method_step(q, true);
+            method_step(q, true, lev);
             q += 2;
+            lev += 4;
           } else {
-            for (int c = 0; c <= A.checks; ++c) method_step(q++, false);
+            for (int c = 0; c <= A.checks; ++c) {
+              method_step(q++, false, lev);
+              lev += 2;
+            }
           }
         }
         if (st == 0) observe(n);
