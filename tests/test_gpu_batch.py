@@ -107,3 +107,42 @@ def test_batch_families_match_oracle(autotune):
         print("instance %d %s: max rel diff vs oracle %.2e" % (i, p["points"][1:], err), flush=True)
         assert err < 5e-5, (i, p["points"], err)
     bp.close()
+
+
+def _kfilter(w, p):
+    """A k-space noise filter neither real nor symmetric in k (xnoise.m:44)."""
+    k = p.kx if p.dimensions == 2 else p.kx - 0.5 * p.ky
+    return w * np.exp(-0.5 * (k - 0.3) ** 2) * (1 + 0.2j * k)
+
+
+@pytest.mark.parametrize("autotune", [False, True])
+def test_batch_with_kspace_noise(autotune):
+    """k-space filtered noises (knoises, xnoise.m:39-50) in a batch: each
+    instance keeps its x-space noise planes per work item of its own, whatever
+    block of the persistent grid serves the item; instances with and without
+    k-space noise, 1D and 2D, against their single runs and the oracle."""
+    ps = _family_batch()
+    pick = [p for p in ps if tuple(p["points"][1:]) in ((64,), (256,), (32, 32), (64, 64))]
+    out = []
+    for i, p in enumerate(pick):
+        q = dict(p, gpu=dict(p["gpu"]))
+        if i % 2 == 0:
+            q.update(noises=0, knoises=2, nfilter=_kfilter)
+        out.append(q)
+    bp = BatchProblem(out, autotune=autotune)
+    print("plan", bp.plan())
+    for rep in range(2):
+        bp.run(1)
+        bp.synchronize()
+        for i, p in enumerate(out):
+            got = bp.read_averages(i)
+            ref = _single(p)
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            print("rep %d instance %d %s knoises %d: vs single run %.2e" % (rep, i, p["points"][1:], p.get("knoises", 0), err),
+                  flush=True)
+            assert err < (2e-5 if autotune else 1e-12), (i, err)
+            if rep == 0:
+                oerr = np.abs(got - _oracle_averages(p)).max() / np.abs(ref).max()
+                print("   vs oracle %.2e" % oerr, flush=True)
+                assert oerr < 5e-5, (i, oerr)
+    bp.close()

@@ -646,18 +646,35 @@ def test_stream_dense_fp64_128_boundary_values(method, values):
     _path_case(p, 1e-10)
 
 
+@pytest.mark.parametrize("method", ["RK4", "MP"])
+@pytest.mark.parametrize("randoms", [False, True])
+def test_stream_fp64_128_kspace_noise(method, randoms):
+    """k-space filtered noises (and initial randoms) on the FP64 128x128
+    streaming kernel: the step's x-space noise plane in the workspace, produced
+    by the streaming transform with the filtered k-space normals as its middle
+    operation; against the oracle's xnoise / xrandom (xift of the filtered
+    normals), fine and coarse paths."""
+    p = shrink(cgl2d("fp64", n=128), E=2, e2=2, points=3, steps=2)
+    p["method"] = method
+    p.update(noises=0, knoises=2, nfilter=_kfilter_n)
+    if randoms:
+        p.update(inrandoms=0, krandoms=2, rfilter=_kfilter_r, urandoms=0)
+        p["gpu"] = dict(p["gpu"], initial="coherent", alpha=0.3 + 0.1j)
+    p["gpu"]["observe"] = ["abs2_ave", "abs2", "a_ave"]
+    prob = Problem(p)
+    assert prob.info.kernel_kind == 1
+    prob.close()
+    _path_case(p, 1e-10)
+
+
 def test_unsupported_combinations_rejected():
     """Combinations outside the GPU path fail at creation with XSH_EUNSUPPORTED
     (the host then keeps the project on the CPU path), before any launch."""
     from xspde_matlab_amd._lib import XshError
-    from xspde_matlab_amd.xsim import BatchProblem
     cases = []
     p = shrink(nls1d("fp64", nx=101), E=4, e2=1, points=2, steps=1)
     p["boundaries"] = [[-1, -1]]
     p["gpu"]["observe"] = ["corr"]                  # periodic FFT of a padded lattice
-    cases.append(p)
-    p = shrink(cgl2d("fp64", n=128), E=2, e2=1, points=2, steps=1)
-    p.update(noises=0, knoises=2, nfilter=_kfilter_n)    # k-space noise on the streaming kernel
     cases.append(p)
     p = shrink(cgl2d("fp32", n=64), E=2, e2=1, points=2, steps=1)
     p["boundaries"] = [[-1, -1], [-1, -1]]
@@ -666,10 +683,6 @@ def test_unsupported_combinations_rejected():
     for p in cases:
         with pytest.raises(XshError, match="XSH_EUNSUPPORTED"):
             Problem(p)
-    p = shrink(cgl2d("fp32", n=32), E=2, e2=1, points=2, steps=1)
-    p.update(noises=0, knoises=1, nfilter=_kfilter_n)
-    with pytest.raises(XshError, match="XSH_EUNSUPPORTED"):
-        BatchProblem([p])                           # k-space noise in a batch
 
 
 def test_averages_in_matlab_order():

@@ -339,9 +339,8 @@ static int plan_handle(xsh_handle* h, const xsh_desc* desc, std::vector<char>& h
     h->tpb = h->kern->tpb;
     h->npad[0] = h->kern->nx;
     h->npad[1] = h->kern->ny;
-    if ((d.knoises || d.krandoms) && (h->mat || !h->kern->kn)) {
-      return fail(XSH_EUNSUPPORTED, "k-space noises run on periodic lattices with register-resident fields "
-                                    "(not the %s kernel)", h->mat ? "dense-transform" : "streaming");
+    if ((d.knoises || d.krandoms) && (h->mat || !(h->kern->kn || h->kern->stream))) {
+      return fail(XSH_EUNSUPPORTED, "k-space noises run on periodic lattices (not the dense-transform kernel)");
     }
     // corr, lowk and the noise modes are periodic FFTs of the field whatever the
     // boundaries (the dense-transform kernels have power-of-two lattices and the
@@ -697,15 +696,22 @@ extern "C" int xsh_create(const xsh_desc* desc, xsh_handle** out) {
     bool corr = false;
     for (int o = 0; o < d.nobserve; ++o)
       corr = corr || d.observe[o] == XSH_OBS_CORR || d.observe[o] == XSH_OBS_ABS2K;
-    h->work_planes = corr ? 3 : 2;
+    h->work_planes = (corr ? 3 : 2) + (d.knoises || d.krandoms ? 1 : 0);
     const long long blocks = std::min<long long>((long long)paths * h->groups, xsh::kStreamMaxBlocks);
     const size_t wbytes = (size_t)blocks * h->work_planes * h->ns * h->esize;
     if (hipMalloc(&h->d_work, wbytes) != hipSuccess)
       return cleanup(fail(XSH_ENOMEM, "hipMalloc streaming workspace (%zu bytes)", wbytes));
   }
-  if (d.knoises) {
-    // the largest launch (every (j, nc) path of an ensemble) has one workgroup per item
-    const size_t kbytes = paths * h->groups * (size_t)h->tpb * h->ns * h->esize;
+  if (d.knoises && !h->kern->stream) {
+    // x-space noise planes per work item of the largest launch (every (j, nc)
+    // path of an ensemble), for whichever compiled variant of this lattice runs
+    // it (a batch's autotuner times them all): ceil(E / TPB) items of TPB planes
+    // per path, at most E + TPB - 1 planes
+    int tpb_max = h->tpb;
+    const xsh::KernelEntry* v[16];
+    const int nv = xsh::kernel_variants(d.precision, h->nx, h->ny, h->mat, v, 16);
+    for (int k = 0; k < nv; ++k) tpb_max = std::max(tpb_max, v[k]->tpb);
+    const size_t kbytes = paths * ((size_t)h->E + tpb_max - 1) * h->ns * h->esize;
     if (hipMalloc(&h->d_kwork, kbytes) != hipSuccess)
       return cleanup(fail(XSH_ENOMEM, "hipMalloc k-space noise planes (%zu bytes)", kbytes));
   }
@@ -1038,7 +1044,8 @@ extern "C" int xsh_run_path_noalloc(xsh_handle* h, int npar, int j, int nc, cons
                   (int)XSH_SCRATCH_BYTES, args_bytes + av_bytes + per_block);
     work_bytes = align256((size_t)grid * per_block);
   }
-  const size_t kn_bytes = h->d.knoises ? align256((size_t)items * h->tpb * h->ns * h->esize) : 0;
+  const size_t kn_bytes =
+      h->d.knoises && !h->kern->stream ? align256((size_t)items * h->tpb * h->ns * h->esize) : 0;
   const size_t need = args_bytes + av_bytes + work_bytes + kn_bytes;
   if (need > (size_t)XSH_SCRATCH_BYTES)
     return fail(XSH_EINVAL, "path needs %zu bytes of scratch, more than the %d available (averages %zu, "
@@ -1424,10 +1431,6 @@ extern "C" int xsh_batch_create(int n, const xsh_desc* descs, int flags, xsh_bat
       batch_free(b);
       return fail(XSH_EUNSUPPORTED, "instance %d: batches hold lattice problems (dimensions >= 2)", i);
     }
-    if (descs[i].knoises || descs[i].krandoms) {
-      batch_free(b);
-      return fail(XSH_EUNSUPPORTED, "instance %d: k-space noises are not in batches", i);
-    }
     int rc = xsh_create(&descs[i], &b->h[i]);
     if (rc) {
       const std::string msg = g_err;
@@ -1476,7 +1479,7 @@ extern "C" int xsh_batch_create(int n, const xsh_desc* descs, int flags, xsh_bat
     const int nv = g.kern->stream ? 0 : xsh::kernel_variants(g.precision, g.nx, g.ny, g.mat, v, 16);
     g.cand.assign(1, g.kern);
     for (int k = 0; k < nv; ++k)
-      if (v[k] != g.kern && (!g.need_tpb1 || v[k]->tpb == 1)) g.cand.push_back(v[k]);
+      if (v[k] != g.kern && (!g.need_tpb1 || v[k]->tpb == 1) && (!g.kern->kn || v[k]->kn)) g.cand.push_back(v[k]);
     int rc = batch_layout(g, b->h, 0);
     if (rc) return bail(rc);
   }

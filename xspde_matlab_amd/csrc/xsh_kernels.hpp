@@ -407,7 +407,10 @@ __device__ __forceinline__ void spde_work(const PathArgs& A, int lb, cpx<R>* __r
     Lat::inverse_lds(L, twx, twy);
   };
   // this workgroup's planes of the step's x-space k-noise
-  C* const kws = KN && A.knoises ? static_cast<C*>(A.kn_work) + (long long)blockIdx.x * TPB * NS : nullptr;
+  // (indexed by the instance's work item, not the block: in a batch's
+  // persistent grid a block serves the items of several instances, and each
+  // instance's planes cover its own items)
+  C* const kws = KN && A.knoises ? static_cast<C*>(A.kn_work) + (long long)lb * TPB * NS : nullptr;
 
   XSH_FOR(k, 0, PPT) f1[k] = C{R(0), R(0)}; XSH_END
   if constexpr (!MAT && KN) {

@@ -38,7 +38,7 @@ struct PathArgs {
                            // sites in FFT order, times sqrt(nscale knoisefactor) / ns
   const void* kr_filter;   // complex R [krandoms][ns]: random filter, times
                            // sqrt(knoisefactor) / ns
-  void* kn_work;           // complex R [grid * TPB][ns]: x-space k-noise of the step
+  void* kn_work;           // complex R [work items * TPB][ns]: x-space k-noise of the step
   double noise_scale;      // sqrt(nscale * noisefactor), xnoise.m:32
   double init_scale;       // sqrt(noisefactor), xrandom.m:22
   int deriv;

@@ -223,7 +223,10 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
     const PathArgs& A = args[inst];
     C* const F0 = static_cast<C*>(A.work) + (long long)blockIdx.x * A.work_planes * NS;
     C* const F1 = F0 + NS;
-    C* const F2 = F1 + NS;  // corr observable scratch (present when work_planes == 3)
+    C* const F2 = F1 + NS;  // corr observable scratch (present with corr or abs2k)
+    // x-space k-space noise of the step (or the initial k-space randoms): the
+    // last plane when the problem has knoises or krandoms
+    C* const FK = A.knoises || A.krandoms ? F0 + (long long)(A.work_planes - 1) * NS : nullptr;
     const C* twx = static_cast<const C*>(A.twx);
     const C* twy = static_cast<const C*>(A.twy);
     const C* twh = static_cast<const C*>(A.twh);
@@ -237,8 +240,45 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
                          static_cast<const C*>(nci ? A.py[1] : A.py[0])};
     double* av = A.av + 2 * (long long)pidx * A.avlen;
 
+    // k-space filtered noise (xnoise.m:39-50) or randoms (xrandom.m:24-29) into
+    // FK: Z(k) = F0(k) z0(k) + i F1(k) z1(k) from the counter normals of every k
+    // site (coarse: the mean of draws q and q + 1), inverse transformed.  The
+    // propagation machinery does it: its middle operation returns Z(kx, ky) in
+    // place of the plane's own spectrum (the filter tables hold the 1/N of the
+    // inverse), so the plane's former content never reaches the result.
+    auto kfield = [&](const C* __restrict__ Fk, int ncomp, uint32_t tag, uint32_t q, bool coarse)
+        __attribute__((always_inline)) {
+      auto zop = [&](C X, int kx, int ky) __attribute__((always_inline)) -> C {
+        const int ks = kx + NX * ky;
+        R z0, z1;
+        knormals<R>(A, nsp, tag, q, (uint32_t)traj, ks, ncomp, z0, z1);
+        if (coarse) {
+          R u0, u1;
+          knormals<R>(A, nsp, tag, q + 1, (uint32_t)traj, ks, ncomp, u0, u1);
+          z0 = R(0.5) * (z0 + u0);
+          z1 = R(0.5) * (z1 + u1);
+        }
+        const C f0 = Fk[ks];
+        C Z = C{f0.x * z0, f0.y * z0};
+        if (ncomp >= 2) {
+          const C f1 = Fk[NS + ks];
+          Z = Z + C{-f1.y * z1, f1.x * z1};
+        }
+        return Z;
+      };
+      if constexpr (!MAT) S::propagate(L, FK, twx, twy, twh, zop);
+    };
+
     // ---- initial field (xensemble.m:53-65)
     __syncthreads();
+    if (FK) {
+#pragma unroll 1
+      for (int i = 0; i < PAIRS; ++i) {  // finite content for the discarded forward pass
+        const int s = S::site(i);
+        FK[s] = FK[s + 1] = C{R(0), R(0)};
+      }
+      if (A.krandoms && !A.a0) kfield(static_cast<const C*>(A.kr_filter), A.krandoms, TAG_KRAND_INIT, 0u, false);
+    }
 #pragma unroll 1
     for (int i = 0; i < PAIRS; ++i) {
       const int s = S::site(i);
@@ -249,12 +289,18 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
           const long long g = (long long)traj * NS + st;
           F0[st] = C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]};
         } else {
-          F0[st] = initial_value<R>(A, nsp, (uint32_t)traj, st, ix, iy);
+          F0[st] = initial_value<R>(A, nsp, (uint32_t)traj, st, ix, iy,
+                                    A.krandoms ? FK[st] : C{R(0), R(0)});
         }
       }
     }
 
     auto noise = [&](int s, uint32_t q, bool coarse, C& w0, C& w1) __attribute__((always_inline)) {
+      if (A.knoises) {  // the step's x-space k-noise, transformed once per step
+        w0 = FK[s];
+        w1 = FK[s + 1];
+        return;
+      }
       draw_pair<R>(A, nsp, q, (uint32_t)traj, s, w0, w1);
       if (coarse) {
         C u0, u1;
@@ -342,6 +388,7 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
     auto method_step = [&](uint32_t q, bool coarse, int lev) __attribute__((always_inline)) {
       const int hl = coarse ? 2 : 1;
       __syncthreads();
+      if (A.knoises) kfield(static_cast<const C*>(A.kn_filter), A.knoises, TAG_KNOISE, q, coarse);
       if (A.method == XSH_RK4) {
         // RK4.m:9-14: a1 = a + D(a) dt into F1
         setbound(F0, lev);
