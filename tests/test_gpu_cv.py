@@ -177,3 +177,42 @@ def test_cross_fitted_estimate():
     print("cross-fit %.5f +- %.5f, plain %.5f +- %.5f" % (mu, sig, plain, psig))
     assert abs(mu - plain) < 4 * math.hypot(sig, psig)
     assert sig < 0.5 * psig
+
+
+@pytest.mark.parametrize("translate", [False, True])
+def test_training_graph_matches_eager_epochs(translate):
+    """Training epochs replay one captured HIP graph (several epochs in a call)
+    or launch eagerly (one epoch per call): same order arrays, translation
+    epochs and Adam steps, so the parameters after every call agree bitwise.
+    The graph is replayed across two calls with host work and synchronisations
+    between the replays."""
+    rng = np.random.default_rng(11)
+    th = OM.init(3).astype(np.float32)
+    x = rng.standard_normal((512, 64)).astype(np.float32)
+    t = rng.standard_normal(512).astype(np.float32)
+    a, b = _lib.MLP(th, batch_max=1024), _lib.MLP(th, batch_max=1024)
+    if translate:
+        modes = lowest_modes(16, 16, 32)
+        a.set_translations(modes, 16, 16, seed=4)
+        b.set_translations(modes, 16, 16, seed=4)
+    import ctypes as C
+    lib = _lib.load()
+    fp = lambda v: v.ctypes.data_as(C.POINTER(C.c_float))
+    for call in range(2):
+        E = 5
+        order = np.concatenate([rng.permutation(512) for _ in range(E)]).astype(np.int32)
+        la, lb = np.zeros(E), np.zeros(E)
+        _lib.mcheck(lib.xsh_mlp_train(a._h, 512, fp(x), fp(t), E, 64, order.ctypes.data_as(C.POINTER(C.c_int)),
+                                      1e-3, la.ctypes.data_as(C.POINTER(C.c_double))))
+        for e in range(E):
+            oe = np.ascontiguousarray(order[e * 512:(e + 1) * 512])
+            l1 = np.zeros(1)
+            _lib.mcheck(lib.xsh_mlp_train(b._h, 512, fp(x), fp(t), 1, 64, oe.ctypes.data_as(C.POINTER(C.c_int)),
+                                          1e-3, l1.ctypes.data_as(C.POINTER(C.c_double))))
+            lb[e] = l1[0]
+        pa, pb = a.params(), b.params()
+        print("call %d: max |graph - eager| = %.3e, losses %s vs %s" % (call, np.abs(pa - pb).max(), la, lb))
+        assert np.array_equal(pa, pb)
+        assert np.allclose(la, lb, rtol=1e-6)
+    a.close()
+    b.close()

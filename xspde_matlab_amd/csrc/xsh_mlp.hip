@@ -95,6 +95,11 @@ hipError_t gemm(hipStream_t s, int M, int N, int K, Mat A, Mat B, float* C, long
   return hipGetLastError();
 }
 
+__global__ void zero_dbl(double* p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
 __global__ void fill_kernel(float* p, long long n, float v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -116,10 +121,24 @@ __global__ void loss_kernel(const float* y, const float* t, float* dy, int n, fl
   if (threadIdx.x == 0) unsafeAtomicAdd(loss, 0.5 * (red[0] + red[1] + red[2] + red[3]) * inv_n);
 }
 
+// Per-epoch training state in device memory, written by set_state before
+// each epoch (so one captured epoch replays with new values):
+//   [0] epoch index into the caller's order array, [1] translation epoch,
+//   [2] Adam steps taken before this epoch
+__global__ void set_state(int* st, int e, int tepoch, int t0) {
+  st[0] = e;
+  st[1] = tepoch;
+  st[2] = t0;
+}
+
+// Adam step number st[2] + b + 1 (b: the minibatch within the epoch), bias
+// corrections 1 - beta^t computed here
 __global__ void adam_kernel(float* th, const float* g, float* m, float* v, int n, float lr, float b1, float b2,
-                            float c1, float c2, float eps) {
+                            const int* st, int b, float eps) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const float t = (float)(st[2] + b + 1);
+  const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
   const float gi = g[i];
   const float mi = b1 * m[i] + (1.f - b1) * gi;
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
@@ -129,9 +148,12 @@ __global__ void adam_kernel(float* th, const float* g, float* m, float* v, int n
 }
 
 // gather rows idx[0..n) of X (nin wide) and t into contiguous buffers
-__global__ void gather_kernel(const float* X, const float* t, const int* idx, int n, int nin, float* Xb, float* tb) {
+// rows of position r0 .. r0 + n of epoch st[0] in the order array (ntot rows per epoch)
+__global__ void gather_kernel(const float* X, const float* t, const int* order, const int* st, long long ntot, int r0,
+                              int n, int nin, float* Xb, float* tb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * nin) return;
+  const int* idx = order + (long long)st[0] * ntot + r0;
   const int r = i / nin, c = i % nin;
   Xb[i] = X[(long long)idx[r] * nin + c];
   if (c == 0) tb[r] = t[idx[r]];
@@ -142,11 +164,14 @@ __global__ void gather_kernel(const float* X, const float* t, const int* idx, in
 // of an nx x ny periodic lattice, so a translation by (sx, sy) multiplies it by
 // exp(-2 pi i (kx sx / nx + ky sy / ny)).  The shift of sample idx in epoch e is
 // drawn from the counter stream, key (seed, 4), counter (e, idx, 0, 0).
-__global__ void gather_translate(const float* X, const float* t, const int* idx, int n, int nin, float* Xb,
-                                 float* tb, const int* kxy, int nx, int ny, uint32_t seed, uint32_t epoch) {
+__global__ void gather_translate(const float* X, const float* t, const int* order, const int* st, long long ntot,
+                                 int r0, int n, int nin, float* Xb, float* tb, const int* kxy, int nx, int ny,
+                                 uint32_t seed) {
   const int np = nin / 2;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * np) return;
+  const int* idx = order + (long long)st[0] * ntot + r0;
+  const uint32_t epoch = (uint32_t)st[1];
   const int r = i / np, m = i % np;
   const int src = idx[r];
   const xsh::u4 u = xsh::philox10(xsh::u4{epoch, (uint32_t)src, 0u, 0u}, seed, 4u);
@@ -230,6 +255,22 @@ struct xsh_mlp {
   int* kxy;                 // translation augmentation: mode wavenumbers (device), or NULL
   int tnx, tny;
   uint32_t tseed, tepoch;   // stream key and epochs drawn so far
+  int* st;                  // per-epoch state (set_state)
+  hipGraphExec_t ep_exec;   // one captured training epoch, or NULL
+  hipGraph_t ep_graph;
+  long long idx_cap;        // ints allocated for the order array
+  // what the captured epoch was built with (buffers and baked-in arguments)
+  struct EpochKey {
+    long long n;
+    int batch, tnx, tny;
+    uint32_t tseed;
+    float lr;
+    const void *X, *T, *idx, *kxy;
+    bool operator==(const EpochKey& o) const {
+      return n == o.n && batch == o.batch && tnx == o.tnx && tny == o.tny && tseed == o.tseed && lr == o.lr &&
+             X == o.X && T == o.T && idx == o.idx && kxy == o.kxy;
+    }
+  } ep_key;
 };
 
 extern "C" const char* xsh_mlp_last_error(void) { return g_mlp_err.c_str(); }
@@ -243,6 +284,9 @@ static void mlp_free(xsh_mlp* h) {
   if (h->idx) (void)hipFree(h->idx);
   if (h->kxy) (void)hipFree(h->kxy);
   if (h->dacc) (void)hipFree(h->dacc);
+  if (h->st) (void)hipFree(h->st);
+  if (h->ep_exec) (void)hipGraphExecDestroy(h->ep_exec);
+  if (h->ep_graph) (void)hipGraphDestroy(h->ep_graph);
   if (h->s) (void)hipStreamDestroy(h->s);
   delete h;
 }
@@ -271,7 +315,8 @@ extern "C" int xsh_mlp_create(int nin, int nh, int batch_max, const float* theta
       mlp_free(h);
       return mfail(XSH_ENOMEM, "hipMalloc (%zu floats)", a.n);
     }
-  if (hipMalloc(&h->dacc, 4 * sizeof(double)) != hipSuccess || hipStreamCreate(&h->s) != hipSuccess) {
+  if (hipMalloc(&h->dacc, 4 * sizeof(double)) != hipSuccess || hipMalloc(&h->st, 4 * sizeof(int)) != hipSuccess ||
+      hipStreamCreate(&h->s) != hipSuccess) {
     mlp_free(h);
     return mfail(XSH_EDEVICE, "allocation");
   }
@@ -380,41 +425,86 @@ extern "C" int xsh_mlp_train(xsh_mlp* h, int n, const float* x, const float* t, 
   if (h->nall < n) {
     if (h->Xall) (void)hipFree(h->Xall);
     if (h->Tall) (void)hipFree(h->Tall);
-    if (h->idx) (void)hipFree(h->idx);
     h->Xall = h->Tall = nullptr;
-    h->idx = nullptr;
+    h->nall = 0;
     if (hipMalloc(&h->Xall, (size_t)n * h->nin * sizeof(float)) != hipSuccess ||
-        hipMalloc(&h->Tall, (size_t)n * sizeof(float)) != hipSuccess ||
-        hipMalloc(&h->idx, (size_t)n * epochs * sizeof(int)) != hipSuccess)
+        hipMalloc(&h->Tall, (size_t)n * sizeof(float)) != hipSuccess)
       return mfail(XSH_ENOMEM, "training set");
     h->nall = n;
-  } else {
+  }
+  if (h->idx_cap < (long long)n * epochs) {
     if (h->idx) (void)hipFree(h->idx);
+    h->idx = nullptr;
+    h->idx_cap = 0;
     if (hipMalloc(&h->idx, (size_t)n * epochs * sizeof(int)) != hipSuccess) return mfail(XSH_ENOMEM, "order");
+    h->idx_cap = (long long)n * epochs;
   }
   MCHK(hipMemcpyAsync(h->Xall, x, (size_t)n * h->nin * sizeof(float), hipMemcpyHostToDevice, h->s));
   MCHK(hipMemcpyAsync(h->Tall, t, (size_t)n * sizeof(float), hipMemcpyHostToDevice, h->s));
   MCHK(hipMemcpyAsync(h->idx, order, (size_t)n * epochs * sizeof(int), hipMemcpyHostToDevice, h->s));
-  const float b1c = 0.9f, b2c = 0.999f, eps = 1e-8f;
-  for (int e = 0; e < epochs; ++e) {
-    MCHK(hipMemsetAsync(h->dacc, 0, sizeof(double), h->s));
-    int nbat = 0;
-    for (int r0 = 0; r0 + batch <= n; r0 += batch, ++nbat) {
+  // One epoch: clear the loss, then per minibatch gather, forward, backward
+  // and Adam - about 15 small kernels each, so the epoch is captured once as a
+  // HIP graph and replayed (the per-epoch values live in h->st, set by a
+  // kernel before each replay; the loss is cleared by a fill kernel, not a
+  // memset node).  The captured sequence is exactly the eager one.
+  const int nbat = n / batch;
+  auto epoch_body = [&](hipStream_t s) -> int {
+    hipLaunchKernelGGL(zero_dbl, dim3(1), dim3(64), 0, s, h->dacc, 1LL);
+    for (int b = 0; b < nbat; ++b) {
+      const int r0 = b * batch;
       if (h->kxy)
-        hipLaunchKernelGGL(gather_translate, dim3((batch * (h->nin / 2) + 255) / 256), dim3(256), 0, h->s, h->Xall,
-                           h->Tall, h->idx + (size_t)e * n + r0, batch, h->nin, h->X, h->T, h->kxy, h->tnx, h->tny,
-                           h->tseed, h->tepoch + (uint32_t)e);
+        hipLaunchKernelGGL(gather_translate, dim3((batch * (h->nin / 2) + 255) / 256), dim3(256), 0, s, h->Xall,
+                           h->Tall, h->idx, h->st, (long long)n, r0, batch, h->nin, h->X, h->T, h->kxy, h->tnx,
+                           h->tny, h->tseed);
       else
-        hipLaunchKernelGGL(gather_kernel, dim3((batch * h->nin + 255) / 256), dim3(256), 0, h->s, h->Xall, h->Tall,
-                           h->idx + (size_t)e * n + r0, batch, h->nin, h->X, h->T);
+        hipLaunchKernelGGL(gather_kernel, dim3((batch * h->nin + 255) / 256), dim3(256), 0, s, h->Xall, h->Tall,
+                           h->idx, h->st, (long long)n, r0, batch, h->nin, h->X, h->T);
       int rc = forward_dev(h, batch);
       if (!rc) rc = backward_dev(h, batch);
       if (rc) return rc;
-      ++h->adam_t;
-      const float c1 = 1.f - std::pow(b1c, (float)h->adam_t), c2 = 1.f - std::pow(b2c, (float)h->adam_t);
-      hipLaunchKernelGGL(adam_kernel, dim3((h->P + 255) / 256), dim3(256), 0, h->s, h->th, h->g, h->m, h->v, h->P, lr,
-                         b1c, b2c, c1, c2, eps);
+      hipLaunchKernelGGL(adam_kernel, dim3((h->P + 255) / 256), dim3(256), 0, s, h->th, h->g, h->m, h->v, h->P, lr,
+                         0.9f, 0.999f, h->st, b, 1e-8f);
     }
+    MCHK(hipGetLastError());
+    return XSH_OK;
+  };
+  const xsh_mlp::EpochKey key{n, batch, h->tnx, h->tny, h->tseed, lr, h->Xall, h->Tall, h->idx, h->kxy};
+  if (h->ep_exec && !(h->ep_key == key)) {
+    (void)hipGraphExecDestroy(h->ep_exec);
+    (void)hipGraphDestroy(h->ep_graph);
+    h->ep_exec = nullptr;
+    h->ep_graph = nullptr;
+  }
+  const bool graph = epochs > 1 && nbat > 0;
+  if (graph && !h->ep_exec) {
+    MCHK(hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal));
+    const int rc = epoch_body(h->s);
+    hipGraph_t gr = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(h->s, &gr);
+    if (rc || e2 != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc ? rc : mfail(XSH_EDEVICE, "capture of the training epoch: %s", hipGetErrorString(e2));
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e3 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) {
+      (void)hipGraphDestroy(gr);
+      return mfail(XSH_EDEVICE, "instantiate the training epoch: %s", hipGetErrorString(e3));
+    }
+    h->ep_graph = gr;
+    h->ep_exec = ex;
+    h->ep_key = key;
+  }
+  for (int e = 0; e < epochs; ++e) {
+    hipLaunchKernelGGL(set_state, dim3(1), dim3(1), 0, h->s, h->st, e, (int)(h->tepoch + (uint32_t)e), h->adam_t);
+    MCHK(hipGetLastError());
+    if (graph) {
+      MCHK(hipGraphLaunch(h->ep_exec, h->s));
+    } else {
+      const int rc = epoch_body(h->s);
+      if (rc) return rc;
+    }
+    h->adam_t += nbat;
     if (epoch_loss) {
       double l = 0.0;
       MCHK(hipMemcpyAsync(&l, h->dacc, sizeof(double), hipMemcpyDeviceToHost, h->s));
