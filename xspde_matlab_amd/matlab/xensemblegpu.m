@@ -30,6 +30,9 @@ end
 cleanup = onCleanup(@() xsh_mex('destroy', h));
 info = xsh_mex('query', h);
 xsh_mex('run_ensemble', h, npar);
+if xsh_mex('nonfinite', h)                       % the kernels' divergence check
+  warning('xSPDE:gpu:nonfinite', '%s: a field was not finite at a stored point (ensemble %d)', p.name, npar);
+end
 avall = xsh_mex('read_averages', h);             % [avlen, ensembles(2)*nchk]
 raw = cell(1,p.nchk,p.ensembles(2));
 if p.rawdata                                     % raw{s,nc,j} of xensemble.m:79

@@ -74,6 +74,13 @@ d.observe = zeros(1,numel(obs));
 for o = 1:numel(obs)
   d.observe(o) = find(strcmp(onames, obs{o})) - 1;
 end
+% time-domain spectra: p.transforms{o}(1) (xpath.m:90-106), SDEs
+d.tspec = zeros(1,numel(obs));
+if isfield(p,'transforms')
+  for o = 1:min(numel(obs), numel(p.transforms))
+    d.tspec(o) = double(p.transforms{o}(1) ~= 0);
+  end
+end
 d.raw = double(isfield(p,'rawdata') && p.rawdata);
 bv = zeros(2,2);
 if isfield(p,'boundval')

@@ -16,6 +16,10 @@
  *   xsh_mex('run_ensemble', h, npar)        every (j, nc) path of ensemble npar
  *   av   = xsh_mex('read_averages', h)      [avlen, ensembles(2)*nchk] complex,
  *                                           column (j-1)*nchk + nc
+ *   flag = xsh_mex('nonfinite', h)          1 if a stored field value of the last run
+ *                                           was NaN or Inf (xsh_nonfinite)
+ *   info = xsh_mex('plan', d)               what create would build, with no device
+ *                                           (xsh_plan; xSPDE:gpu:unsupported: CPU path)
  *   xsh_mex('destroy', h)
  *
  * Errors map the library's status to an identifier: xSPDE:gpu:unsupported
@@ -159,6 +163,34 @@ static void make_desc(const mxArray* s, xsh_desc* d) {
   d->knoises = d->nfilter ? (int)(n / ((mwSize)d->nx * d->ny)) : 0;
   d->rfilter = ctable(s, "rfilter", &n);
   d->krandoms = d->rfilter ? (int)(n / ((mwSize)d->nx * d->ny)) : 0;
+  /* tspec: per observable, 1 for the time-domain spectrum (p.transforms{o}(1)) */
+  memset(t, 0, sizeof t);
+  vec(s, "tspec", t, XSH_MAX_OBS);
+  for (int o = 0; o < XSH_MAX_OBS; ++o) d->tspec[o] = (int)t[o];
+}
+
+static mxArray* info_struct(const xsh_info* info) {
+  const char* names[] = {"nsites", "nchk", "avlen", "obs_offset", "obs_sites", "traj_steps", "kernel_kind",
+                         "ensembles", "points"};
+  mxArray* r = mxCreateStructMatrix(1, 1, 9, names);
+  mxSetField(r, 0, "nsites", mxCreateDoubleScalar(info->nsites));
+  mxSetField(r, 0, "nchk", mxCreateDoubleScalar(info->nchk));
+  mxSetField(r, 0, "avlen", mxCreateDoubleScalar((double)info->avlen));
+  mxArray* off = mxCreateDoubleMatrix(1, XSH_MAX_OBS, mxREAL);
+  mxArray* ns = mxCreateDoubleMatrix(1, XSH_MAX_OBS, mxREAL);
+  for (int o = 0; o < XSH_MAX_OBS; ++o) {
+    mxGetDoubles(off)[o] = (double)info->obs_offset[o];
+    mxGetDoubles(ns)[o] = info->obs_sites[o];
+  }
+  mxSetField(r, 0, "obs_offset", off);
+  mxSetField(r, 0, "obs_sites", ns);
+  mxSetField(r, 0, "traj_steps", mxCreateDoubleScalar((double)info->traj_steps));
+  mxSetField(r, 0, "kernel_kind", mxCreateDoubleScalar(info->kernel_kind));
+  mxArray* ens = mxCreateDoubleMatrix(1, 3, mxREAL);
+  for (int i = 0; i < 3; ++i) mxGetDoubles(ens)[i] = info->ensembles[i];
+  mxSetField(r, 0, "ensembles", ens);
+  mxSetField(r, 0, "points", mxCreateDoubleScalar(info->points));
+  return r;
 }
 
 static xsh_handle* handle_arg(const mxArray* a) {
@@ -182,6 +214,16 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     *(uint64_t*)mxGetData(plhs[0]) = (uint64_t)(uintptr_t)h;
     return;
   }
+  if (!strcmp(cmd, "plan")) {
+    if (nrhs < 2) mexErrMsgIdAndTxt("xSPDE:gpu:invalid", "plan needs the descriptor struct");
+    xsh_desc d;
+    make_desc(prhs[1], &d);
+    xsh_info pinfo;
+    const int rc = xsh_plan(&d, &pinfo);
+    if (rc) fail_status(rc, "xsh_plan");
+    plhs[0] = info_struct(&pinfo);
+    return;
+  }
   if (nrhs < 2) mexErrMsgIdAndTxt("xSPDE:gpu:invalid", "%s needs a handle", cmd);
   xsh_handle* h = handle_arg(prhs[1]);
   xsh_info info;
@@ -192,26 +234,14 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   int rc = xsh_query(h, &info);
   if (rc) fail_status(rc, "xsh_query");
   if (!strcmp(cmd, "query")) {
-    const char* names[] = {"nsites", "nchk", "avlen", "obs_offset", "obs_sites", "traj_steps", "kernel_kind",
-                           "ensembles", "points"};
-    plhs[0] = mxCreateStructMatrix(1, 1, 9, names);
-    mxSetField(plhs[0], 0, "nsites", mxCreateDoubleScalar(info.nsites));
-    mxSetField(plhs[0], 0, "nchk", mxCreateDoubleScalar(info.nchk));
-    mxSetField(plhs[0], 0, "avlen", mxCreateDoubleScalar((double)info.avlen));
-    mxArray* off = mxCreateDoubleMatrix(1, XSH_MAX_OBS, mxREAL);
-    mxArray* ns = mxCreateDoubleMatrix(1, XSH_MAX_OBS, mxREAL);
-    for (int o = 0; o < XSH_MAX_OBS; ++o) {
-      mxGetDoubles(off)[o] = (double)info.obs_offset[o];
-      mxGetDoubles(ns)[o] = info.obs_sites[o];
-    }
-    mxSetField(plhs[0], 0, "obs_offset", off);
-    mxSetField(plhs[0], 0, "obs_sites", ns);
-    mxSetField(plhs[0], 0, "traj_steps", mxCreateDoubleScalar((double)info.traj_steps));
-    mxSetField(plhs[0], 0, "kernel_kind", mxCreateDoubleScalar(info.kernel_kind));
-    mxArray* ens = mxCreateDoubleMatrix(1, 3, mxREAL);
-    for (int i = 0; i < 3; ++i) mxGetDoubles(ens)[i] = info.ensembles[i];
-    mxSetField(plhs[0], 0, "ensembles", ens);
-    mxSetField(plhs[0], 0, "points", mxCreateDoubleScalar(info.points));
+    plhs[0] = info_struct(&info);
+    return;
+  }
+  if (!strcmp(cmd, "nonfinite")) {
+    int flag = 0;
+    rc = xsh_nonfinite(h, &flag);
+    if (rc) fail_status(rc, "xsh_nonfinite");
+    plhs[0] = mxCreateDoubleScalar(flag);
     return;
   }
   if (!strcmp(cmd, "run_path")) {
