@@ -4,7 +4,8 @@ Array convention: a field cell is ``a[field, x, (y,) traj]`` (the reference's
 ``p.d.ca = [fields, space..., ens]``, xpreferences.m:145).  Noise and random
 cells are ``w[comp, x, (y,) traj]`` (xpreferences.m:153-158).  Only one field
 cell is supported (``p.fieldcells = 1``), which covers every example on the
-path.
+path; its ``p.fields`` components may each have their own boundary types and
+values (``p.boundaries{2}`` with one row per field, spdpreferences.m:47-66).
 
 Functions and the reference code they restate:
 
@@ -19,7 +20,8 @@ xpropfactor           Applications/SPDE/xpropfactor.m:23-58       linear-only ex
 xprop / xtransform    xprop.m:17-46, xtransform.m:21-70          transforms brute force,
                                                                     Batchtest rows
 xsetbound             xsetbound.m:30-59 (boundval constants or     Batchtest #21 (NN),
-                      boundfun(a, c, d, p) at p.t)                  #42 (Peregrine), heat
+                      boundfun(a, c, d, p) at p.t; per field)       #42 (Peregrine, as 4
+                                                                    fields), #33, heat
                                                                     closed forms
 xpatch, da            xpatch.m:24-62, xpropfactor.m:40-51           inhomogeneous heat
                                                                     closed forms (DD, NN,
@@ -43,8 +45,12 @@ xensemble             Simulator/xensemble.m:12-108                  Batchtest #2
                                                                     extrapolation)
 xsim statistics       Simulator/xsim.m:64-188                       Batchtest rows,
                                                                     synthetic sigma test
+xsim sequences        xsim.m:33-38, xensemble.m:48-52 (transfer),   Batchtest #33
+                      xpreferences.m:27-28,197                      (Boundaries2)
 xcompress             Applications/SPDE/xcompress.m:18-69           Batchtest #16
 Ave / Int / D1        Applications/SPDE/Ave.m, Int.m, D1.m          Batchtest #14, #16, #21
+D2                    Applications/SPDE/D2.m:40-83                  Batchtest #33, quadratic
+                                                                    with ghost-point ends
 xnft                  Applications/SPDE/xnft.m                      Batchtest #16
 ====================  ===========================================  =======================
 
@@ -125,7 +131,7 @@ def xpreferences(p_in):
     pts[1:] = 1 + (pts[1:] - 1) * p.steps[1:]                    # xpreferences.m:137
     p.points = pts
     og = p.get("origins", None)
-    og = [0.0] + list(-p.ranges[1:] / 2) if og is None else _listify(og)
+    og = [float(p.get("_origin0", 0.0))] + list(-p.ranges[1:] / 2) if og is None else _listify(og)
     p.origins = np.asarray(list(og) + list(-p.ranges[len(og):] / 2), float)[:nd]
     ch = p.get("checks", None)
     ch = [1] + [0] * (nd - 1) if ch is None else _listify(ch)
@@ -161,49 +167,82 @@ def xpreferences(p_in):
     p.d_ca = (p.fields,) + p.space + (E,)
     # spdpreferences.m:16-72 and xgrid.m:18-48
     bnd = p.get("boundaries", None)
-    p.btypes = []
-    p.setboundaries = 0
+    F = p.fields
+    # per dimension, one [lower, upper] row per field (spdpreferences.m:17,47-66:
+    # p.boundaries{c,d} is fields x 2; a single row here applies to every field)
+    rows = []
+    for d in range(1, nd):
+        b = [0, 0]
+        if bnd is not None and len(bnd) > d - 1 and bnd[d - 1] is not None:
+            b = bnd[d - 1]
+        b = np.asarray(b, dtype=int)
+        if b.ndim == 1:
+            b = np.tile(b, (F, 1))
+        if b.shape != (F, 2):
+            raise ValueError("boundaries of dimension %d: need [lower, upper] or one row per field" % (d + 1))
+        rows.append(b)
+    # constant boundary values p.boundval (spdpreferences.m:26-43, xboundfun default 0):
+    # per space dimension (lower, upper) - field values at Dirichlet ends,
+    # derivatives at Neumann (Robin) ends; one pair for every field or one per field
+    bv = p.get("boundval", None)
+    fbv = [[(0.0, 0.0)] * (nd - 1) for _ in range(F)]
+    for d in range(1, nd):
+        if bv is None or len(bv) <= d - 1 or bv[d - 1] is None:
+            continue
+        v = bv[d - 1]
+        per_field = F > 1 and len(v) == F and isinstance(v[0], (list, tuple)) and len(v[0]) == 2
+        for f in range(F):
+            lo, hi = v[f] if per_field else v
+            fbv[f][d - 1] = (lo, hi)
+    uniform = all((r == r[0]).all() for r in rows) and all(fbv[f] == fbv[0] for f in range(F)) and \
+        (F == 1 or p.get("boundfun", None) is None)
+    p.setboundaries = int(any((r != 0).any() for r in rows))
     sh = (1,) + p.space + (1,)
-    p.xs, p.ks = [], []
+    p.xs = []
+    ks_f = [[] for _ in range(F)]
     for d in range(1, nd):
         n = int(pts[d])
         p0 = np.arange(n)
         xc = p.origins[d] + p0 * dx[d]
-        kc = (p0 - n // 2) * p.dk[d]
-        dkp = np.pi / ((n - 1) * dx[d]) if n > 1 else 0.0
-        b = [0, 0]
-        if bnd is not None and len(bnd) > d - 1 and bnd[d - 1] is not None:
-            b = list(bnd[d - 1])
-        t = int(b[0]) + 2 * int(b[1])
-        if t == -3:
-            kcp = p0 * dkp
-        elif t == -1:
-            kcp = np.concatenate([[0.0], np.arange(1, n) - 0.5]) * dkp
-        elif t == 0:
-            kcp = np.fft.ifftshift(kc)
-        elif t == 1:
-            kcp = np.concatenate([np.arange(1, n) - 0.5, [0.0]]) * dkp
-        elif t == 3:
-            kcp = p0 * dkp
-        else:
-            raise ValueError("Invalid boundary type: [%d,%d]" % (b[0], b[1]))
-        if t != 0:
-            p.setboundaries = 1
-        p.btypes.append((t, b))
         shape = [1] * (nd + 1)
         shape[d] = n
         p.xs.append(np.broadcast_to(xc.reshape(shape), sh).copy())
-        p.ks.append(np.broadcast_to(kcp.reshape(shape), sh).copy())
-    # constant boundary values p.boundval (spdpreferences.m:26-43, xboundfun default 0):
-    # per space dimension (lower, upper) - field values at Dirichlet ends,
-    # derivatives at Neumann (Robin) ends
-    bv = p.get("boundval", None)
-    p.bvals = []
-    for d in range(1, nd):
-        v = (0.0, 0.0)
-        if bv is not None and len(bv) > d - 1 and bv[d - 1] is not None:
-            v = (complex(bv[d - 1][0]), complex(bv[d - 1][1]))
-        p.bvals.append(v)
+        kc = (p0 - n // 2) * p.dk[d]
+        dkp = np.pi / ((n - 1) * dx[d]) if n > 1 else 0.0
+        for f in range(F):
+            t = int(rows[d - 1][f, 0]) + 2 * int(rows[d - 1][f, 1])
+            if t == -3:
+                kcp = p0 * dkp
+            elif t == -1:
+                kcp = np.concatenate([[0.0], np.arange(1, n) - 0.5]) * dkp
+            elif t == 0:
+                kcp = np.fft.ifftshift(kc)
+            elif t == 1:
+                kcp = np.concatenate([np.arange(1, n) - 0.5, [0.0]]) * dkp
+            elif t == 3:
+                kcp = p0 * dkp
+            else:
+                raise ValueError("Invalid boundary type: [%d,%d]" % tuple(rows[d - 1][f]))
+            ks_f[f].append(np.broadcast_to(kcp.reshape(shape), sh).copy())
+    # field groups sharing one boundary setup: all fields at once when every field
+    # has the same boundaries (the single-field path), else one group per field
+    # (xtransform.m:25-70 and xsetbound.m:37-55 loop over the field index)
+    groups = range(1) if uniform else range(F)
+    p.bgroups = []
+    for f in groups:
+        g = P()
+        g.sl = slice(None) if uniform else slice(f, f + 1)
+        g.field = None if uniform else f
+        g.btypes = [(int(r[f, 0]) + 2 * int(r[f, 1]), [int(r[f, 0]), int(r[f, 1])]) for r in rows]
+        g.bvals = [(complex(lo), complex(hi)) for lo, hi in fbv[f]]
+        g.da = [0.0] * (nd - 1)
+        p.bgroups.append(g)
+    p.btypes = p.bgroups[0].btypes
+    p.bvals = p.bgroups[0].bvals
+    if uniform:
+        p.ks = ks_f[0]
+    else:
+        p.ks = [np.concatenate([ks_f[f][d] for f in range(F)], axis=0) for d in range(nd - 1)]
     names = ["x", "y", "z"]
     for i, (xg, kg) in enumerate(zip(p.xs, p.ks)):
         p[names[i]] = xg
@@ -214,6 +253,7 @@ def xpreferences(p_in):
            [p.origins[d] + np.arange(pts[d]) * dx[d] for d in range(1, nd)]
     p.kc = [None] + [(np.arange(pts[d]) - pts[d] // 2) * p.dk[d] for d in range(1, nd)]
     p.deriv = p.get("deriv", None) or (lambda a, w, p: 0 * a)
+    p.transfer = p.get("transfer", None) or (lambda b, *v: b)         # xpreferences.m:94
     p.linear = p.get("linear", None)
     p.initial = p.get("initial", None) or (lambda v, p: 0)
     obs = p.get("observe", None)
@@ -245,19 +285,27 @@ def xpreferences(p_in):
 def xpropfactor(p):
     """propagator = exp(L * dtr / ipsteps) (xpropfactor.m:26-39); also the
     Neumann patch correction da_d = -(L2 - L1) dt range_d^2 / pi^2 from the first
-    two k values of each non-periodic dimension (xpropfactor.m:40-51), into p.da."""
-    p.da = [0.0] * len(p.btypes)
+    two k values of each non-periodic dimension (xpropfactor.m:40-51), per field
+    group into g.da (p.da for the first group).  L may carry one row per field
+    (p.Dx does when fields have different boundaries)."""
+    p.bgroups = [g.copy() for g in p.bgroups]
+    for g in p.bgroups:
+        g.da = [0.0] * len(g.btypes)
+    p.da = p.bgroups[0].da
     if p.linear is None or (not callable(p.linear) and p.linear == 0):
         return None
     L = np.asarray(p.linear(p))
     dt = p.dtr / p.ipsteps
     if p.setboundaries:
-        Lb = np.broadcast_to(L, (1,) + p.space + (1,))
-        for d in range(1, len(p.btypes) + 1):
-            i1 = [0] * Lb.ndim
-            i2 = list(i1)
-            i2[d] = 1
-            p.da[d - 1] = -(Lb[tuple(i2)] - Lb[tuple(i1)]) * dt / np.pi ** 2 * p.ranges[d] ** 2
+        Lf = np.broadcast_to(L, (L.shape[0] if L.ndim == len(p.space) + 2 else 1,) + p.space + (1,))
+        for g in p.bgroups:
+            Lb = Lf if Lf.shape[0] == 1 else Lf[g.sl]
+            for d in range(1, len(g.btypes) + 1):
+                i1 = [0] * Lb.ndim
+                i2 = list(i1)
+                i2[d] = 1
+                g.da[d - 1] = -(Lb[tuple(i2)] - Lb[tuple(i1)]) * dt / np.pi ** 2 * p.ranges[d] ** 2
+        p.da = p.bgroups[0].da
     return np.exp(L * dt)
 
 
@@ -281,73 +329,102 @@ def boundary_line(v, a_shape, d):
     return v.reshape(shape)
 
 
-def boundary_values(a, p, d):
-    """(lower, upper) boundary values of space dimension d at time p.t:
-    p.boundfun(a, c, d, p) (reference numbering: c = 1, d = 2, 3 for x, y;
-    xpatch.m:37, xsetbound.m:39) when given, else the constants p.boundval
-    (spdpreferences.m:26-43; default 0)."""
+def boundary_values(a, p, d, g=None, shape=None):
+    """(lower, upper) boundary values of space dimension d at time p.t for field
+    group g (default: the first): p.boundfun(a, c, d, p) (reference numbering:
+    c = 1, d = 2, 3 for x, y; xpatch.m:37, xsetbound.m:39) when given - one
+    (lower, upper) pair, or one per field like the reference's fields x 2 cell -
+    else the constants p.boundval (spdpreferences.m:26-43; default 0).  The
+    values are shaped for the group's part of the field (``shape``)."""
+    g = g or p.bgroups[0]
+    shape = shape or a.shape
     f = p.get("boundfun", None)
     if f is None:
-        lo, hi = p.bvals[d - 1]
+        lo, hi = g.bvals[d - 1]
     else:
         b = f(a, 1, d + 1, p)
+        if p.fields > 1 and len(b) == p.fields and isinstance(b[0], (list, tuple)):
+            b = b[g.field if g.field is not None else 0]
         lo, hi = b[0], b[1]
-    return boundary_line(lo, a.shape, d), boundary_line(hi, a.shape, d)
+    return boundary_line(lo, shape, d), boundary_line(hi, shape, d)
+
+
+def _by_group(p, a, fn):
+    """Apply fn(sub_array, group) to every field group and stack the results
+    along the field axis (one group: the whole field at once)."""
+    if len(p.bgroups) == 1:
+        return fn(a, p.bgroups[0])
+    parts = [fn(a[g.sl], g) for g in p.bgroups]
+    if isinstance(parts[0], tuple):
+        return tuple(np.concatenate([np.broadcast_to(q[k], a[g.sl].shape) for q, g in zip(parts, p.bgroups)],
+                                    axis=0) for k in range(len(parts[0])))
+    return np.concatenate(parts, axis=0)
 
 
 def xpatch(a, p):
     """Boundary patch (xpatch.m:24-62): returns (a - pa, pa + lp) where pa is the
     per-dimension polynomial matching the boundary values and lp the Neumann
-    (type -3) correction db * da_d."""
-    pa = np.zeros(a.shape, dtype=complex)
-    lp = 0.0
-    for d, (t, _) in enumerate(p.btypes, start=1):
-        if t == 0 or a.shape[d] <= 1:
-            continue
-        b1, b2 = boundary_values(a, p, d)
-        rng = p.ranges[d]
-        x = p.xs[d - 1] - p.origins[d]
-        if t == -3:
-            db = (b2 - b1) / rng
-            pa = pa + b1 * x + 0.5 * x ** 2 * db
-            lp = lp + db * p.da[d - 1]
-        elif t == -1:
-            pa = pa + b1 + x * b2
-        elif t == 1:
-            pa = pa + b2 + (x - rng) * b1
-        elif t == 3:
-            pa = pa + b1 + x * (b2 - b1) / rng
-    return a - pa, pa + lp
+    (type -3) correction db * da_d, per field group."""
+    a_full = a
+
+    def one(a, g):
+        pa = np.zeros(a.shape, dtype=complex)
+        lp = 0.0
+        for d, (t, _) in enumerate(g.btypes, start=1):
+            if t == 0 or a.shape[d] <= 1:
+                continue
+            b1, b2 = boundary_values(a_full, p, d, g, a.shape)
+            rng = p.ranges[d]
+            x = p.xs[d - 1] - p.origins[d]
+            if t == -3:
+                db = (b2 - b1) / rng
+                pa = pa + b1 * x + 0.5 * x ** 2 * db
+                lp = lp + db * g.da[d - 1]
+            elif t == -1:
+                pa = pa + b1 + x * b2
+            elif t == 1:
+                pa = pa + b2 + (x - rng) * b1
+            elif t == 3:
+                pa = pa + b1 + x * (b2 - b1) / rng
+        return a - pa, pa + lp
+    return _by_group(p, a, one)
 
 
 def xsetbound(a, p):
     """Dirichlet ends set to their boundary values, dimension by dimension (later
-    dimensions win at corners) (xsetbound.m:30-59): p.boundfun at p.t, else the
-    p.boundval constants (default 0, spdpreferences.m:26-43,75-93)."""
+    dimensions win at corners), per field group (xsetbound.m:30-59): p.boundfun
+    at p.t, else the p.boundval constants (default 0, spdpreferences.m:26-43,75-93)."""
     if not p.setboundaries:
         return a
-    a = np.array(a, dtype=complex)
-    for d, (t, b) in enumerate(p.btypes, start=1):
-        if t == 0:
-            continue
-        lo, hi = boundary_values(a, p, d)
-        if b[0] == 1:
-            idx = [slice(None)] * a.ndim
-            idx[d] = slice(0, 1)
-            a[tuple(idx)] = np.broadcast_to(lo, a[tuple(idx)].shape)
-        if b[1] == 1:
-            idx = [slice(None)] * a.ndim
-            idx[d] = slice(a.shape[d] - 1, a.shape[d])
-            a[tuple(idx)] = np.broadcast_to(hi, a[tuple(idx)].shape)
-    return a
+    a_full = np.array(a, dtype=complex)
+
+    def one(a, g):
+        a = np.array(a, dtype=complex)
+        for d, (t, b) in enumerate(g.btypes, start=1):
+            if t == 0:
+                continue
+            lo, hi = boundary_values(a_full, p, d, g, a.shape)
+            if b[0] == 1:
+                idx = [slice(None)] * a.ndim
+                idx[d] = slice(0, 1)
+                a[tuple(idx)] = np.broadcast_to(lo, a[tuple(idx)].shape)
+            if b[1] == 1:
+                idx = [slice(None)] * a.ndim
+                idx[d] = slice(a.shape[d] - 1, a.shape[d])
+                a[tuple(idx)] = np.broadcast_to(hi, a[tuple(idx)].shape)
+        return a
+    return _by_group(p, a_full, one)
 
 
 def xtransform(a, p, inverse):
-    """Per space dimension spectral transform of xtransform.m:33-70."""
-    for d, (t, _) in enumerate(p.btypes, start=1):
-        if a.shape[d] > 1:
-            a = transform(a, d, t, inverse)
-    return a
+    """Per space dimension spectral transform of xtransform.m:33-70, with each
+    field's own boundary type."""
+    def one(a, g):
+        for d, (t, _) in enumerate(g.btypes, start=1):
+            if a.shape[d] > 1:
+                a = transform(a, d, t, inverse)
+        return a
+    return _by_group(p, np.asarray(a), one)
 
 
 def xprop(a, p, prop):
@@ -557,8 +634,10 @@ def initial_field(p, v):
 
 
 # -------------------------------------------------------------------- ensemble
-def run_path(p, j, nc, npar=1, seq=0, inst=0, noise_fn=None, a0=None):
-    """One (sub-ensemble j, check nc) call of p.path as xensemble.m:26-79 sets it up."""
+def run_path(p, j, nc, npar=1, seq=0, inst=0, noise_fn=None, a0=None, prev=None):
+    """One (sub-ensemble j, check nc) call of p.path as xensemble.m:26-79 sets it up.
+    ``prev``: the final field of the previous element of a sequence, handed to
+    p.transfer(b, v, p) for the initial field (xensemble.m:48-52)."""
     p = p.copy()
     nsp = p.seed + npar + (j - 1) * p.ensembles[2]                      # xensemble.m:26
     if p.checks[0] and nc != 2:
@@ -568,7 +647,13 @@ def run_path(p, j, nc, npar=1, seq=0, inst=0, noise_fn=None, a0=None):
     prop = xpropfactor(p)
     v = xrandom(p, nsp, seq, inst)
     p.t = float(p.origins[0])
-    a = initial_field(p, v) if a0 is None else np.array(a0, dtype=complex)
+    if a0 is not None:
+        a = np.array(a0, dtype=complex)
+    elif prev is not None:
+        a = np.asarray(p.transfer(prev, *v, p), dtype=complex)
+        a = np.broadcast_to(a, p.d_ca).copy() if a.shape != p.d_ca else a.copy()
+    else:
+        a = initial_field(p, v)
     return xpath(a, nc, p, prop, nsp, seq, inst, noise_fn)
 
 
@@ -604,6 +689,40 @@ def xensemble(npar, p, path_fn=None):
             for n1 in range(1, m + 1):
                 data[o][:, n1 - 1] += D[(n1, o)] * nes
             data[o][:, m1 - 1] += D[(1, o)] ** 2 * nes
+    return data
+
+
+def xensemble_seq(npar, ps):
+    """xensemble.m:25-108 over a sequence of problems: per (j, nc) the elements run
+    in order, each after the first starting from p.transfer of the previous
+    element's final field; D, D^2 per element as in xensemble.  Returns one
+    data list per element."""
+    p1 = ps[0]
+    e2, e3 = p1.ensembles[1], p1.ensembles[2]
+    nes = 1.0 / (e2 * e3)
+    m = p1.fbsw + p1.nchk
+    m1 = max(3, m + 1)
+    data = [[None] * len(p.output) for p in ps]
+    for j in range(1, e2 + 1):
+        D = [{} for _ in ps]
+        for nc in range(1, p1.nchk + 1):
+            b = None
+            for s, p in enumerate(ps):
+                b, av = run_path(p, j, nc, npar, seq=s, prev=b if s > 0 else None)
+                for o, out in enumerate(p.output):
+                    D[s][(nc, o)] = np.real(np.asarray(out(av, p))).reshape(-1)
+        for s, p in enumerate(ps):
+            for o in range(len(p.output)):
+                Ds = D[s]
+                if p.checks[0] and p.order > 0:                          # xensemble.m:93-97
+                    dl = (Ds[(1, o)] - Ds[(2, o)]) / (2.0 ** p.order - 1.0)
+                    Ds[(2, o)] = Ds[(1, o)]
+                    Ds[(1, o)] = Ds[(1, o)] + dl
+                if data[s][o] is None:
+                    data[s][o] = np.zeros((Ds[(1, o)].size, m1))
+                for n1 in range(1, m + 1):
+                    data[s][o][:, n1 - 1] += Ds[(n1, o)] * nes
+                data[s][o][:, m1 - 1] += Ds[(1, o)] ** 2 * nes
     return data
 
 
@@ -653,7 +772,7 @@ def xcompress_idx(axes_o, shape):
     return idx
 
 
-def xsim_stats(p, data, elapsed=0.0):
+def xsim_stats(p, data, elapsed=0.0, rows=False):
     """Final sampling/step/difference errors of xsim.m:64-188 (no bins/scatters).
 
     ``data[o]`` is [ndata_o, m1] as returned by xensemble summed over npar.
@@ -721,6 +840,21 @@ def xsim_stats(p, data, elapsed=0.0):
         er[o, 0] = np.sum(seg) / (1e-99 + np.sum(seg > 0))
         er[o, :sx + 1] = np.sqrt(er[o, :sx + 1])
         outd.append(d.reshape(cshape + (d.shape[1],)))
+    if rows:
+        return er, em, outd
+    return _combine_errors(p, er, elapsed), outd
+
+
+def _stats_rows(p, data):
+    """Per-output rows (er, em) and reduced data of xsim.m:64-173 for one element."""
+    return xsim_stats(p, data, rows=True)
+
+
+def _combine_errors(p, er, elapsed=0.0):
+    """The error vector of xsim.m:174-188 from the per-output rows of every
+    element of the sequence."""
+    nc1 = p.nchk + p.fbsw
+    sx = max(3, 1 + nc1)
     error = np.zeros(max(6, sx + 3))
     for c in range(2, sx + 2):
         pts = 1e-99 + np.sum(er[:, c - 1] > p.tol)
@@ -729,11 +863,16 @@ def xsim_stats(p, data, elapsed=0.0):
     error[sx + 1:sx + 3] = np.sum(er[:, sx + 1:sx + 3], axis=0)
     error[sx + 1] = error[sx + 1] / (1e-20 + error[sx + 2])
     error[sx + 2] = elapsed
-    return error, outd
+    return error
 
 
 def xsim(p_in, path_fn=None):
-    """[error, data, p] = xsim(p) for one sequence element (xsim.m:28-188)."""
+    """[error, data, p] = xsim(p) (xsim.m:28-188).  A list of problems is a
+    sequence (xsim.m:33-38, xpreferences.m:11-13,197): checks from the first
+    element, each default time origin where the previous element ended, the
+    error vector over the outputs of every element; data and p are then lists."""
+    if isinstance(p_in, (list, tuple)):
+        return _xsim_sequence(p_in)
     t0 = _time.time()
     p = xpreferences(p_in)
     data = None
@@ -745,6 +884,32 @@ def xsim(p_in, path_fn=None):
         p._avshape = _probe_shapes(p)
     err, outd = xsim_stats(p, data, _time.time() - t0)
     return err, outd, p
+
+
+def _xsim_sequence(plist):
+    t0 = _time.time()
+    ps = []
+    origin = 0.0
+    for k, pin in enumerate(plist):
+        q = dict(pin)
+        if k > 0:
+            q["checks"] = list(ps[0].checks)                         # xpreferences.m:27-28
+        q["_origin0"] = origin
+        p = xpreferences(q)
+        origin = p.origins[0] + p.ranges[0]                           # xpreferences.m:197
+        ps.append(p)
+    data = None
+    for npar in range(1, ps[0].ensembles[2] + 1):
+        dp = xensemble_seq(npar, ps)
+        data = dp if data is None else [[x + y for x, y in zip(u, w)] for u, w in zip(data, dp)]
+    ers, outs = [], []
+    for p, d in zip(ps, data):
+        p._avshape = _probe_shapes(p)
+        er, _, outd = _stats_rows(p, d)
+        ers.append(er)
+        outs.append(outd)
+    err = _combine_errors(ps[0], np.concatenate(ers, axis=0), _time.time() - t0)
+    return err, outs, ps
 
 
 def _probe_shapes(p):
@@ -788,16 +953,59 @@ def Int(o, *args):
 
 
 def D1(a, d, p):
-    """Central first derivative along space axis d (D1.m), Neumann ends = boundval 0."""
+    """Central first derivative along space axis d (D1.m), Neumann ends = boundval 0,
+    per field group."""
     ax = d - 1
-    r = (np.roll(a, -1, axis=ax) - np.roll(a, 1, axis=ax)) / (2 * p.dx[d - 1])
-    t, b = p.btypes[d - 2]
-    if b[0] == -1:
-        idx = [slice(None)] * a.ndim
-        idx[ax] = 0
-        r[tuple(idx)] = 0
-    if b[1] == -1:
-        idx = [slice(None)] * a.ndim
-        idx[ax] = -1
-        r[tuple(idx)] = 0
-    return r
+
+    def one(a, g):
+        r = (np.roll(a, -1, axis=ax) - np.roll(a, 1, axis=ax)) / (2 * p.dx[d - 1])
+        t, b = g.btypes[d - 2]
+        if b[0] == -1:
+            idx = [slice(None)] * a.ndim
+            idx[ax] = 0
+            r[tuple(idx)] = 0
+        if b[1] == -1:
+            idx = [slice(None)] * a.ndim
+            idx[ax] = -1
+            r[tuple(idx)] = 0
+        return r
+    return _by_group(p, np.asarray(a), one)
+
+
+def D2(a, d, p):
+    """Central second derivative along space axis d (Applications/SPDE/D2.m:40-83,
+    a derivative function: p.indext = 0, cell 1).  Periodic wrap; a field whose
+    lower end is not periodic gets end values from its boundary types: at a
+    Neumann end the ghost-point form 2((a_1 - a_0)/dx - b_lower)/dx (upper:
+    2(b_upper + (a_{N-2} - a_{N-1})/dx)/dx) with the derivative values of
+    p.boundval; at any other end the neighbouring interior value (a continuous
+    D2).  With a p.boundfun, the values are those stored before the path, at
+    t = origins(1) - 1 (xensemble.m:53-57)."""
+    ax = d - 1
+    dx = p.dx[d - 1]
+    a = np.asarray(a)
+
+    def one(a, g):
+        r = (np.roll(a, -1, axis=ax) + np.roll(a, 1, axis=ax) - 2 * a) / dx ** 2
+        t, b = g.btypes[d - 2]
+        if b[0] == 0:
+            return r
+        q = p.copy()
+        q.t = float(p.origins[0]) - 1
+        lo, hi = boundary_values(a, q, d - 1, g, a.shape)
+
+        def at(k):
+            idx = [slice(None)] * a.ndim
+            idx[ax] = slice(k, k + 1) if k != -1 else slice(a.shape[ax] - 1, a.shape[ax])
+            return tuple(idx)
+        n = a.shape[ax]
+        if b[0] == -1:
+            r[at(0)] = 2 * ((a[at(1)] - a[at(0)]) / dx - lo) / dx
+        else:
+            r[at(0)] = r[at(1)]
+        if b[1] == -1:
+            r[at(-1)] = 2 * (hi + (a[at(n - 2)] - a[at(-1)]) / dx) / dx
+        else:
+            r[at(-1)] = r[at(n - 2)]
+        return r
+    return _by_group(p, a, one)

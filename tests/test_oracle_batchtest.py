@@ -184,3 +184,86 @@ def test_batchtest_peregrine_time_dependent_boundaries():
     err = X.xsim_stats(q, data)[0]
     print("Peregrine error %.7e (Batchtest 4.588574e-04)" % err[0])
     assert 4.588574e-04 / 1.001 < err[0] < 4.588574e-04 * 1.001
+
+
+def test_peregrine_as_one_four_field_problem():
+    """Peregrine.m as the reference writes it: one 4-field problem, one row of
+    p.boundaries{2} per field and a boundary function returning one
+    (lower, upper) pair per field (the fields x 2 cell of Peregrine.m:45-50)."""
+    def bfun(a, c, d, p):
+        _, v = PEREGRINE_ENDS["DD"]
+        lo, _ = v(p.origins[1], p.t)
+        dv = peregrine_dx(p.origins[1], p.t)
+        return [(lo, lo), (dv, -dv), (lo, -dv), (dv, lo)]
+    p = peregrine_problem("DD", nobs=4)
+    p.update(fields=4, boundaries=[[b for b, _ in PEREGRINE_ENDS.values()]], boundfun=bfun,
+             initial=lambda v, p: peregrine(p.x, p.origins[0]) + np.zeros((4, 1, 1)),
+             observe=[lambda a, p, _k=k: a[_k][None] for k in range(4)])
+    err = X.xsim(p)[0]
+    print("Peregrine 4 fields: error %.7e (Batchtest 4.588574e-04)" % err[0])
+    assert 4.588574e-04 / 1.001 < err[0] < 4.588574e-04 * 1.001
+
+
+# Examples/SPDE_2D/Boundaries2.m (Batchtest #33): five uncoupled heat-equation
+# fields on [0, pi], one boundary type per field (DD, NN, DN, ND, periodic),
+# as a sequence of two problems: (I) the spectral interaction picture with
+# L = Dx^2, (II) finite differences D2 with 80 steps, restarted from the
+# initial fields by p.transfer.  The error vector runs over the ten outputs.
+def heat_fields(v, p):
+    x = p.x[0]
+    return np.stack([4 * np.sin(x) + np.sin(2 * x), 5 + 4 * np.cos(x) + np.cos(2 * x),
+                     4 * np.sin(x / 2) + np.sin(3 * x / 2), 4 * np.cos(x / 2) + np.cos(3 * x / 2),
+                     2 + np.cos(2 * x / 1.02) + np.sin(4 * x / 1.02)])
+
+
+HEAT_EXACT = [
+    lambda p: 4 * np.sin(p.x) * np.exp(-p.t) + np.sin(2 * p.x) * np.exp(-4 * p.t),
+    lambda p: 5 + 4 * np.cos(p.x) * np.exp(-p.t) + np.cos(2 * p.x) * np.exp(-4 * p.t),
+    lambda p: 4 * np.sin(p.x / 2) * np.exp(-p.t / 4) + np.sin(3 * p.x / 2) * np.exp(-9 * p.t / 4),
+    lambda p: 4 * np.cos(p.x / 2) * np.exp(-p.t / 4) + np.cos(3 * p.x / 2) * np.exp(-9 * p.t / 4),
+    lambda p: 2 + np.cos(2 * p.x / 1.02) * np.exp(-4 * p.t / 1.02 ** 2)
+    + np.sin(4 * p.x / 1.02) * np.exp(-16 * p.t / 1.02 ** 2)]
+
+
+def boundaries2():
+    p = dict(dimensions=2, points=[51, 51], method="MP", order=0, fields=5, ranges=[4, np.pi],
+             origins=[0, 0], initial=heat_fields, compare=HEAT_EXACT,
+             observe=[lambda a, p, _k=k: a[_k][None] for k in range(5)],
+             boundaries=[[[1, 1], [-1, -1], [1, -1], [-1, 1], [0, 0]]], linear=lambda p: p.Dx ** 2)
+    p1 = dict(p, linear=None, deriv=lambda a, w, p: X.D2(a, 2, p), steps=80,
+              transfer=lambda b, v, p: heat_fields(v, p))
+    return p, p1
+
+
+def test_boundaries2_spectral_fields_exact():
+    """Dataset (I) alone: each field's own transform reproduces its closed form
+    to rounding (a field transformed with another field's boundary type does
+    not)."""
+    p, _ = boundaries2()
+    err = X.xsim(p)[0]
+    assert err[0] < 1e-13
+    bad = dict(p, boundaries=[[[1, 1], [1, 1], [1, -1], [-1, 1], [0, 0]]])
+    assert X.xsim(bad)[0][0] > 1e-2
+
+
+def test_batchtest_boundaries2_sequence():
+    p, p1 = boundaries2()
+    err = X.xsim([p, p1])[0]
+    print("Boundaries2 error %.7e (Batchtest 2.185585e-05)" % err[0])
+    assert 2.185585e-05 / 1.001 < err[0] < 2.185585e-05 * 1.001
+
+
+def test_d2_matches_second_difference_with_ghost_points():
+    """D2 on a quadratic: interior exact; Neumann ends with boundary derivative
+    values use the ghost-point form, which is exact for a quadratic whose end
+    slopes are those values; a Dirichlet end copies its neighbour."""
+    n, L = 11, 2.0
+    bval = 2 * 0.0 - 1.0, 2 * L - 1.0      # slopes of x^2 - x at 0 and L
+    p = X.xpreferences(dict(dimensions=2, points=[3, n], ranges=[1.0, L], origins=[0, 0], fields=2,
+                            boundaries=[[[-1, -1], [1, 1]]], boundval=[[bval, (0.0, 0.0)]]))
+    x = p.xc[1]
+    a = np.stack([x ** 2 - x, x ** 2 - x])[:, :, None].astype(complex)
+    d2 = X.D2(a, 2, p)
+    assert np.allclose(d2[0, :, 0], 2.0, atol=1e-10)
+    assert np.allclose(d2[1, 1:-1, 0], 2.0, atol=1e-10)
+    assert d2[1, 0, 0] == d2[1, 1, 0] and d2[1, -1, 0] == d2[1, -2, 0]
