@@ -93,7 +93,7 @@ def test_per_site_output_layout():
     for o, out in enumerate(outputs):
         v = np.real(np.asarray(out(split_averages(q, info, flat), q))).reshape(-1)
         outd.append(np.stack([v, 10 * v, 100 * v], axis=1))
-    arr = matlab_data(q, info, outputs, outd)
+    arr = matlab_data(q, split_averages(q, info, np.zeros(info.avlen, dtype=complex)), outputs, outd)
     assert arr[0].shape == (1, npts, nx, ny, 3) and arr[1].shape == (1, npts, 1, 3)
     block = flat[:npts * ns].reshape(1, npts, nx, ny, order="F")
     for t in range(npts):

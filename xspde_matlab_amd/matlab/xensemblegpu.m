@@ -8,6 +8,9 @@ function [data,raw] = xensemblegpu(npar,input)
 %   xpath.m:79-83 one after another in MATLAB order, so a reshape recovers
 %   them.  If the library refuses the project (xSPDE:gpu:unsupported) this
 %   falls back to the CPU xensemble.
+%   A project with p.fields > 1 runs one handle per field component
+%   (xgpudesc(p,k)), all launched before any is read; observable o comes
+%   from component p.gpu.ofield(o).
 %   Called by xsim (Simulator/xsim.m:53,59) in place of xensemble when
 %   isfield(p,'gpu').  Not executed in the repository's tests (MATLAB is not
 %   part of its image); the same bookkeeping in Python is
@@ -18,28 +21,51 @@ if numel(input) > 1
   return;
 end
 p = input{1};
+F = p.fields{1};
+nobs = numel(p.gpu.observe);
+of = ones(1,nobs);
+if isfield(p.gpu,'ofield'), of = p.gpu.ofield; end
+h = zeros(1,F,'uint64');
 try
-  h = xsh_mex('create', xgpudesc(p));
+  for k = 1:F
+    h(k) = xsh_mex('create', xgpudesc(p,k));
+  end
 catch err
+  destroyall(h);                                 % the components created so far
   if strcmp(err.identifier,'xSPDE:gpu:unsupported')
     [data,raw] = xensemble(npar,input);
     return;
   end
   rethrow(err);
 end
-cleanup = onCleanup(@() xsh_mex('destroy', h));
-info = xsh_mex('query', h);
-xsh_mex('run_ensemble', h, npar);
-if xsh_mex('nonfinite', h)                       % the kernels' divergence check
-  warning('xSPDE:gpu:nonfinite', '%s: a field was not finite at a stored point (ensemble %d)', p.name, npar);
+cleanup = onCleanup(@() destroyall(h));          % h holds every handle from here on
+info = cell(1,F);
+for k = 1:F
+  info{k} = xsh_mex('query', h(k));
+  xsh_mex('run_ensemble', h(k), npar);           % each on its own stream
 end
-avall = xsh_mex('read_averages', h);             % [avlen, ensembles(2)*nchk]
+avall = cell(1,F);
+for k = 1:F
+  if xsh_mex('nonfinite', h(k))                  % the kernels' divergence check
+    warning('xSPDE:gpu:nonfinite', '%s: field %d was not finite at a stored point (ensemble %d)', ...
+            p.name, k, npar);
+  end
+  avall{k} = xsh_mex('read_averages', h(k));     % [avlen, ensembles(2)*nchk]
+end
 raw = cell(1,p.nchk,p.ensembles(2));
 if p.rawdata                                     % raw{s,nc,j} of xensemble.m:79
-  rawall = xsh_mex('read_raw', h);               % [prod(p.d.raw{1}), ensembles(2)*nchk]
+  sz = p.d.raw{1};
+  rawall = cell(1,F);
+  for k = 1:F
+    rawall{k} = xsh_mex('read_raw', h(k));       % [prod(sz)/F, ensembles(2)*nchk]
+  end
   for j = 1:p.ensembles(2)
     for nc = 1:p.nchk
-      raw{1,nc,j} = {reshape(rawall(:,(j-1)*p.nchk+nc), p.d.raw{1})};
+      parts = cell(1,F);
+      for k = 1:F
+        parts{k} = reshape(rawall{k}(:,(j-1)*p.nchk+nc), [1, sz(2:end)]);
+      end
+      raw{1,nc,j} = {cat(1, parts{:})};
     end
   end
 end
@@ -50,15 +76,16 @@ data = cell(1);
 for o = p.outputs
   data{1}{o} = zeros([p.d.data{o},m1]);
 end
-nobs = numel(p.gpu.observe);
 for j = 1:p.ensembles(2)
   D = cell(p.nchk, max(p.outputs));
   for nc = 1:p.nchk
-    col = avall(:,(j-1)*p.nchk+nc);
     av = cell(1,nobs);
     for o = 1:nobs
-      n = info.points*info.obs_sites(o);
-      av{o} = reshape(col(info.obs_offset(o)+(1:n)), p.d.av{o});
+      k = of(o);
+      i = nnz(of(1:o) == k);                     % o's place among component k's observables
+      col = avall{k}(:,(j-1)*p.nchk+nc);
+      n = info{k}.points*info{k}.obs_sites(i);
+      av{o} = reshape(col(info{k}.obs_offset(i)+(1:n)), p.d.av{o});
     end
     for o = p.outputs
       D{nc,o} = reshape(real(p.output{o}(av,p)),[p.d.data{o},1]);
@@ -76,5 +103,11 @@ for j = 1:p.ensembles(2)
     end
     data{1}{o}(:,m1) = data{1}{o}(:,m1)+D{1,o}.^2*nes;
   end
+end
+end
+
+function destroyall(h)
+for k = 1:numel(h)
+  if h(k) ~= 0, xsh_mex('destroy', h(k)); end
 end
 end

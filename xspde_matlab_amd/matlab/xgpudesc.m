@@ -1,8 +1,14 @@
-function d = xgpudesc(p)
+function d = xgpudesc(p, k)
 %   d = XGPUDESC(p) maps a prepared xSPDE structure (after xpreferences) whose
 %   drift, linear term, initial state and observables are built-in GPU forms
 %   named in p.gpu onto the descriptor of the MI355X library (xsh_desc in
 %   include/xspde_hip.h), for xsh_mex('create', d).
+%   d = XGPUDESC(p, k) is the descriptor of field component k of a project
+%   with p.fields > 1: the built-in forms act on each component by itself, so
+%   each component is its own descriptor, with row k of p.boundaries{1,d} and
+%   p.boundval{1,d}, row k of the fields x 2 cell p.boundfun returns, the
+%   overrides p.gpu.fields{k}, p.noises/p.fields noise components, the
+%   observables o with p.gpu.ofield(o) == k and the streams of instance k-1.
 %   p.gpu fields (the names of xspde_matlab_amd/xsim.py make_desc):
 %     precision 'fp32' | 'fp64'
 %     deriv     'zero' | 'kubo' (c) | 'gp' (g, u, s)
@@ -17,7 +23,18 @@ function d = xgpudesc(p)
 %   raises, and the project then stays on the CPU xensemble.
 %   Not executed in the repository's tests (MATLAB is not part of its image).
 
+if nargin < 2, k = 1; end
 g = p.gpu;
+F = p.fields{1};
+if isfield(g,'fields') && numel(g.fields) >= k  % per-component overrides
+  o = g.fields{k};
+  fn = fieldnames(o);
+  for i = 1:numel(fn), g.(fn{i}) = o.(fn{i}); end
+end
+obs = g.observe;
+of = ones(1,numel(obs));
+if isfield(g,'ofield'), of = g.ofield; end
+mine = find(of == k);
 nd = p.dimensions;
 pts = p.points{1};
 d.precision = double(isfield(g,'precision') && any(strcmpi(g.precision,{'fp64','double'})));
@@ -32,12 +49,14 @@ d.method = double(~strcmp(func2str(p.method),'RK4'));
 d.iterations = p.iterations;
 d.ensembles = p.ensembles;
 d.seed = p.seed;
-d.noises = field0(p,'noises');
-d.inrandoms = field0(p,'inrandoms');
-d.urandoms = field0(p,'urandoms');
+d.noises = field0(p,'noises')/F;
+d.inrandoms = field0(p,'inrandoms')/F;
+d.urandoms = field0(p,'urandoms')/F;
+d.inst = k-1;
 b = zeros(2,2);
-for k = 1:nd-1
-  b(k,:) = p.boundaries{1,k+1}(1,:);
+for kd = 1:nd-1
+  bd = p.boundaries{1,kd+1};
+  b(kd,:) = bd(min(k,size(bd,1)),:);
 end
 d.boundaries = b;
 d.ranges = zeros(1,3);  d.ranges(1:nd) = p.ranges(1:nd);
@@ -69,24 +88,26 @@ switch g.initial
 end
 d.init_par = ip;
 onames = {'a','abs2','a_ave','abs2_ave','abs2_int','abs4_ave','corr','lowk','abs2k'};
-obs = g.observe;
-d.observe = zeros(1,numel(obs));
-for o = 1:numel(obs)
-  d.observe(o) = find(strcmp(onames, obs{o})) - 1;
-end
-% time-domain spectra: p.transforms{o}(1) (xpath.m:90-106), SDEs
-d.tspec = zeros(1,numel(obs));
-if isfield(p,'transforms')
-  for o = 1:min(numel(obs), numel(p.transforms))
-    d.tspec(o) = double(p.transforms{o}(1) ~= 0);
+if isempty(mine)                                 % a component nobody observes
+  d.observe = find(strcmp(onames,'a_ave')) - 1;
+  d.tspec = 0;
+else
+  d.observe = zeros(1,numel(mine));
+  d.tspec = zeros(1,numel(mine));
+  for i = 1:numel(mine)
+    d.observe(i) = find(strcmp(onames, obs{mine(i)})) - 1;
+    % time-domain spectra: p.transforms{o}(1) (xpath.m:90-106), SDEs
+    if isfield(p,'transforms') && mine(i) <= numel(p.transforms)
+      d.tspec(i) = double(p.transforms{mine(i)}(1) ~= 0);
+    end
   end
 end
 d.raw = double(isfield(p,'rawdata') && p.rawdata);
 bv = zeros(2,2);
 if isfield(p,'boundval')
-  for k = 1:nd-1
-    v = p.boundval{1,k+1};
-    if ~isempty(v), bv(k,:) = [v{1,1}(1), v{1,2}(1)]; end
+  for kd = 1:nd-1
+    v = p.boundval{1,kd+1};
+    if ~isempty(v), r = min(k,size(v,1)); bv(kd,:) = [v{r,1}(1), v{r,2}(1)]; end
   end
 end
 d.boundval = complex(bv);
@@ -99,11 +120,12 @@ if isfield(p,'boundfun') && ~isequal(func2str(p.boundfun),'xboundfun')
   tab = complex(zeros(nb,2,2,levels));
   for m = 1:levels
     p.t = p.origins(1) + (m-1)*dtr/2;
-    for k = 1:nd-1
-      v = p.boundfun({[]},1,k+1,p);
+    for kd = 1:nd-1
+      v = p.boundfun({[]},1,kd+1,p);
+      r = min(k,size(v,1));                      % row k of a fields x 2 cell
       for side = 1:2
-        line = v{1,side}(:);
-        tab(1:numel(line),side,k,m) = line;
+        line = v{r,side}(:);
+        tab(1:numel(line),side,kd,m) = line;
       end
     end
   end
@@ -111,10 +133,10 @@ if isfield(p,'boundfun') && ~isequal(func2str(p.boundfun),'xboundfun')
 end
 % k-space filters as multipliers at every k site in FFT order (xnoise.m:42, xrandom.m:26)
 if field0(p,'knoises') > 0
-  d.nfilter = kfilter(p, p.nfilter{1}, field0(p,'knoises'));
+  d.nfilter = kfilter(p, p.nfilter{1}, field0(p,'knoises')/F);
 end
 if field0(p,'krandoms') > 0
-  d.rfilter = kfilter(p, p.rfilter{1}, field0(p,'krandoms'));
+  d.rfilter = kfilter(p, p.rfilter{1}, field0(p,'krandoms')/F);
 end
 end
 

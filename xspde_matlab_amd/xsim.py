@@ -18,6 +18,9 @@ D, D^2 accumulation (Simulator/xensemble.m:12-108), ``xsim`` the parallel
 ensemble loop and the final error statistics (Simulator/xsim.m:28-188).
 A drift given as a Python callable instead of a ``gpu`` form is rejected:
 in the MATLAB host such projects stay on the reference's CPU ``xpath``.
+A project with several field components (``fields > 1``) runs each
+component as its own handle (``FieldsProblem``): the built-in forms do not
+couple components.
 """
 import math
 import time
@@ -58,8 +61,9 @@ def preferences(p):
     nd = max(1, int(q.get("dimensions", 1)))
     q["dimensions"] = nd
     fields = int(q.get("fields", 1))
-    if fields != 1:
-        raise GpuFormError("the built-in forms integrate one field component")
+    if fields < 1:
+        raise GpuFormError("fields must be at least 1")
+    q["fields"] = fields
     q["knoises"] = int(q.get("knoises") or 0)
     q["krandoms"] = int(q.get("krandoms") or 0)
     if q.get("noises") is None and not q["knoises"]:
@@ -103,6 +107,8 @@ def preferences(p):
 
 
 def make_desc(q):
+    if q.get("fields", 1) != 1:
+        raise GpuFormError("one descriptor integrates one field component (FieldsProblem splits a project)")
     g = q["gpu"]
     d = _lib.Desc()
     d.precision = _lib.FP64 if str(g.get("precision", "fp32")).lower() in ("fp64", "double") else _lib.FP32
@@ -350,12 +356,12 @@ def _unflatten_sites(x, axis, dims):
     return np.transpose(x, perm)
 
 
-def matlab_data(q, info, outputs, outd):
+def matlab_data(q, zero, outputs, outd):
     """Per output, the [ndata, 3] statistics as the array xsim.m:150-157 stores:
     the output's own shape with space dimensions unflattened when it is a
     per-site observable ([lines, points, space..., 3]), the mean, step error and
-    sampling error along the last axis."""
-    zero = split_averages(q, info, np.zeros(info.avlen, dtype=complex))
+    sampling error along the last axis.  ``zero``: the av list of a zero path,
+    for the outputs' shapes."""
     dims = space_dims(q)
     ns = int(np.prod(dims)) if dims else 1
     res = []
@@ -384,6 +390,8 @@ class Problem:
 
     def __init__(self, p):
         self.q = preferences(p)
+        if self.q["fields"] != 1:
+            raise GpuFormError("Problem integrates one field component; FieldsProblem several")
         self.handle = _lib.Handle(make_desc(self.q))
         self.info = self.handle.info
         outs = self.q.get("output")
@@ -416,17 +424,167 @@ class Problem:
         raw = [[matlab_raw(self.q, r[j, nc]) for j in range(r.shape[0])] for nc in range(r.shape[1])]
         return data, raw
 
+    def zero_av(self):
+        return split_averages(self.q, self.info, np.zeros(self.info.avlen, dtype=complex))
+
     def close(self):
         self.handle.close()
 
 
-def ensemble_moments(q, info, outputs, av_all):
+def _field_rows(v, F, pair_len=2):
+    """One entry per field: v itself when it is a single [lower, upper] pair (or
+    None), else v's rows (the reference's fields x 2 p.boundaries{d} and
+    p.boundval{d}, spdpreferences.m:17,26-66)."""
+    if v is None:
+        return [None] * F
+    rows = list(v)
+    if len(rows) == F and F > 1 and all(isinstance(r, (list, tuple, np.ndarray)) and np.size(r) == pair_len
+                                        for r in rows):
+        return [list(r) for r in rows]
+    return [v] * F
+
+
+def field_problems(p, q):
+    """A project with p.fields = F components split into F one-field projects
+    (the built-in drift forms act on each component by itself, so the
+    components do not couple).  Field k takes row k of p.boundaries{d} and
+    p.boundval{d}, the k-th (lower, upper) pair of a p.boundfun that returns one
+    per field, its noise and random components (p.noises / F of them), the
+    overrides p.gpu.fields[k] of the gpu forms, and the observables o with
+    p.gpu.ofield[o] = k + 1 (default 1).  Its streams are those of instance k.
+    p is the project as written, q its preferences."""
+    F = q["fields"]
+    g = dict(q["gpu"])
+    per = list(g.pop("fields", None) or [{}] * F)
+    if len(per) != F:
+        raise GpuFormError("gpu.fields needs one entry per field (%d)" % F)
+    obs = list(g.get("observe", ["a"]))
+    ofield = [int(f) for f in (g.pop("ofield", None) or [1] * len(obs))]
+    if len(ofield) != len(obs) or any(not 1 <= f <= F for f in ofield):
+        raise GpuFormError("gpu.ofield: one field index 1..%d per observable" % F)
+    tr = q.get("transforms")
+    tr = (list(tr) if isinstance(tr, (list, tuple)) else [tr]) if tr is not None else None
+    nd = q["dimensions"]
+    bnd = list(q.get("boundaries") or [])
+    bval = list(q.get("boundval") or [])
+    brows = [_field_rows(bnd[d] if d < len(bnd) else None, F) for d in range(nd - 1)]
+    vrows = [_field_rows(bval[d] if d < len(bval) else None, F) for d in range(nd - 1)]
+    for name in ("noises", "inrandoms", "urandoms", "knoises", "krandoms"):
+        if q[name] % F:
+            raise GpuFormError("%s = %d is not a whole number of components per field (%d fields)"
+                               % (name, q[name], F))
+    bf = q.get("boundfun")
+    subs, omap = [], []
+    for k in range(F):
+        sk = dict(p)
+        sk["fields"] = 1
+        for name in ("noises", "inrandoms", "urandoms", "knoises", "krandoms"):
+            sk[name] = q[name] // F
+        gk = dict(g)
+        gk.update(per[k])
+        mine = [o for o in range(len(obs)) if ofield[o] == k + 1]
+        gk["observe"] = [obs[o] for o in mine] or ["a_ave"]
+        gk["inst"] = int(g.get("inst", 0)) * F + k
+        sk["gpu"] = gk
+        if tr is not None:
+            sk["transforms"] = [tr[o] if o < len(tr) else 0 for o in mine] or [0]
+        sk["boundaries"] = [brows[d][k] for d in range(nd - 1)] if bnd else q.get("boundaries")
+        sk["boundval"] = [vrows[d][k] for d in range(nd - 1)] if bval else q.get("boundval")
+        if bf is not None:
+            def fk(a, c, d, p, _f=bf, _k=k):
+                b = _f(a, c, d, p)
+                if len(b) == F and F > 1 and isinstance(b[0], (list, tuple)):
+                    return b[_k]
+                return b
+            sk["boundfun"] = fk
+        subs.append(sk)
+        omap.append(mine)
+    where = {}
+    for k, mine in enumerate(omap):
+        for i, o in enumerate(mine):
+            where[o] = (k, i)
+    return subs, [where[o] for o in range(len(obs))]
+
+
+class FieldsProblem:
+    """A project with several field components (p.fields > 1,
+    xpreferences.m:44-50): one GPU handle per component, launched together on
+    their own streams, the averages reassembled in the project's observable
+    order.  Same interface as Problem."""
+
+    def __init__(self, p):
+        self.q = preferences(p)
+        subs, self.where = field_problems(p, self.q)
+        self.parts = []
+        try:
+            for sk in subs:
+                self.parts.append(Problem(sk))
+        except Exception:
+            self.close()
+            raise
+        self.info = self.parts[0].info
+        outs = self.q.get("output")
+        nobs = len(self.where)
+        if outs is None:
+            outs = [None] * nobs
+        self.outputs = [o if o is not None else (lambda av, p, _n=n: av[_n]) for n, o in enumerate(outs)]
+
+    def _assemble(self, per_field):
+        return [per_field[k][i] for k, i in self.where]
+
+    def xpath(self, nc, npar=1, j=1, a0=None, want_final=False):
+        """p.path for one (j, nc): av in the project's observable order; with
+        want_final also the final fields [F, E, sites]. a0: [F, E, sites] or None."""
+        res = [part.xpath(nc, npar, j, a0=None if a0 is None else a0[k], want_final=want_final)
+               for k, part in enumerate(self.parts)]
+        if want_final:
+            E, ns = self.q["ensembles"][0], self.info.nsites
+            return self._assemble([r[0] for r in res]), np.stack([r[1].reshape(E, ns) for r in res])
+        return self._assemble(res)
+
+    def xensemble(self, npar, want_raw=False):
+        for part in self.parts:
+            part.handle.run_ensemble(npar)
+        for part in self.parts:
+            part.handle.synchronize()
+        for k, part in enumerate(self.parts):
+            if part.handle.nonfinite():
+                warnings.warn("%s: field %d was not finite at a stored time point (parallel ensemble %d): "
+                              "its averages hold NaN or Inf" % (self.q.get("name", "project"), k + 1, npar),
+                              DivergenceWarning, stacklevel=2)
+        avs = [part.handle.read_averages() for part in self.parts]
+        split = lambda j, nc: self._assemble([split_averages(part.q, part.info, av[j, nc])
+                                              for part, av in zip(self.parts, avs)])
+        data = ensemble_moments(self.q, self.info, self.outputs, None, split=split)
+        if not want_raw:
+            return data
+        rs = [part.handle.read_raw() for part in self.parts]
+        raw = [[np.concatenate([matlab_raw(part.q, r[j, nc]) for part, r in zip(self.parts, rs)], axis=0)
+                for j in range(rs[0].shape[0])] for nc in range(rs[0].shape[1])]
+        return data, raw
+
+    def zero_av(self):
+        return self._assemble([part.zero_av() for part in self.parts])
+
+    def close(self):
+        for part in self.parts:
+            part.close()
+        self.parts = []
+
+
+def make_problem(p):
+    """Problem for one field component, FieldsProblem for several."""
+    return FieldsProblem(p) if int(p.get("fields", 1)) > 1 else Problem(p)
+
+
+def ensemble_moments(q, info, outputs, av_all, split=None):
     """D and D^2 sums of one parallel ensemble from the averages of its paths
     (xensemble.m:84-105): av_all[j, nc] is the flat average of path (j, nc);
     each output's D is extrapolated by (D_fine - D_coarse) / (2^order - 1) when
     checks and order > 0; the sums carry the weight 1 / (ensembles(2) ensembles(3)).
     Returns data[o] of shape [ndata_o, max(3, nchk + 1)]: sum D per check, then
-    sum D_fine^2."""
+    sum D_fine^2.  ``split(j, nc)``, when given, returns path (j, nc)'s av list
+    instead (several field components, each from its own handle)."""
     e2, e3 = q["ensembles"][1], q["ensembles"][2]
     nchk = 1 + q["checks"][0]
     w = 1.0 / (e2 * e3)
@@ -435,7 +593,7 @@ def ensemble_moments(q, info, outputs, av_all):
     for j in range(e2):
         D = {}
         for nc in range(nchk):
-            av = split_averages(q, info, av_all[j, nc])
+            av = split(j, nc) if split is not None else split_averages(q, info, av_all[j, nc])
             for o, out in enumerate(outputs):
                 D[(nc + 1, o)] = np.real(np.asarray(out(av, q))).reshape(-1)
         for o in range(len(outputs)):
@@ -497,7 +655,7 @@ def xsim(p):
     (xsim.m:42,56-57), or None without p.rawdata.  A non-blank p.file saves
     error, data, input and raw to that '.mat' file (xsim.m:207-210)."""
     t0 = time.time()
-    prob = Problem(p)
+    prob = make_problem(p)
     q = prob.q
     raw = [[] for _ in range(1 + q["checks"][0])] if q["rawdata"] else None
     try:
@@ -514,7 +672,7 @@ def xsim(p):
         error[-1] = time.time() - t0
         if q["file"].strip():
             from .xfile import xsave
-            xsave(q["file"], error, matlab_data(q, prob.info, prob.outputs, outd), q, raw)
+            xsave(q["file"], error, matlab_data(q, prob.zero_av(), prob.outputs, outd), q, raw)
     finally:
         prob.close()
     return error, outd, q, raw
