@@ -45,11 +45,11 @@ xensemble             Simulator/xensemble.m:12-108                  Batchtest #2
                                                                     extrapolation)
 xsim statistics       Simulator/xsim.m:64-188                       Batchtest rows,
                                                                     synthetic sigma test
-xsim sequences        xsim.m:33-38, xensemble.m:48-52 (transfer),   Batchtest #33
+xsim sequences        xsim.m:33-38, xensemble.m:59-63 (transfer),   Batchtest #33
                       xpreferences.m:27-28,197                      (Boundaries2)
 xcompress             Applications/SPDE/xcompress.m:18-69           Batchtest #16
 Ave / Int / D1        Applications/SPDE/Ave.m, Int.m, D1.m          Batchtest #14, #16, #21
-D2                    Applications/SPDE/D2.m:40-83                  Batchtest #33, quadratic
+D2                    Applications/SPDE/D2.m:38-78                  Batchtest #33, quadratic
                                                                     with ghost-point ends
 xnft                  Applications/SPDE/xnft.m                      Batchtest #16
 ====================  ===========================================  =======================
@@ -637,7 +637,7 @@ def initial_field(p, v):
 def run_path(p, j, nc, npar=1, seq=0, inst=0, noise_fn=None, a0=None, prev=None):
     """One (sub-ensemble j, check nc) call of p.path as xensemble.m:26-79 sets it up.
     ``prev``: the final field of the previous element of a sequence, handed to
-    p.transfer(b, v, p) for the initial field (xensemble.m:48-52)."""
+    p.transfer(b, v, p) for the initial field (xensemble.m:59-63)."""
     p = p.copy()
     nsp = p.seed + npar + (j - 1) * p.ensembles[2]                      # xensemble.m:26
     if p.checks[0] and nc != 2:
@@ -973,14 +973,14 @@ def D1(a, d, p):
 
 
 def D2(a, d, p):
-    """Central second derivative along space axis d (Applications/SPDE/D2.m:40-83,
+    """Central second derivative along space axis d (Applications/SPDE/D2.m:38-78,
     a derivative function: p.indext = 0, cell 1).  Periodic wrap; a field whose
     lower end is not periodic gets end values from its boundary types: at a
     Neumann end the ghost-point form 2((a_1 - a_0)/dx - b_lower)/dx (upper:
     2(b_upper + (a_{N-2} - a_{N-1})/dx)/dx) with the derivative values of
     p.boundval; at any other end the neighbouring interior value (a continuous
     D2).  With a p.boundfun, the values are those stored before the path, at
-    t = origins(1) - 1 (xensemble.m:53-57)."""
+    t = origins(1) - 1 (xensemble.m:66-72)."""
     ax = d - 1
     dx = p.dx[d - 1]
     a = np.asarray(a)
