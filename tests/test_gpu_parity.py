@@ -206,7 +206,7 @@ def test_stream_kernel_persistent_grid(method):
     base["method"] = method
     base["gpu"] = dict(base["gpu"], observe=["abs2_ave"])
     prob = Problem(base)
-    assert prob.info.threads == 512 and prob.info.traj_per_block == 1
+    assert prob.info.threads == 1024 and prob.info.traj_per_block == 1
     av3, a3 = prob.xpath(1, npar=1, j=1, want_final=True)
     prob.close()
     big = dict(base, ensembles=[1100, 1, 1])

@@ -6,7 +6,7 @@
 namespace xsh {
 
 static const KernelEntry kStreamEntries[] = {
-    XSH_STREAM_ENTRY(double, 128, 128, 512),
+    XSH_STREAM_ENTRY(double, 128, 128, 1024),
 };
 
 const KernelEntry* table_stream_mat(int* n);

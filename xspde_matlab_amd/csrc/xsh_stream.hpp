@@ -39,7 +39,7 @@ struct Stream {
   __device__ static __forceinline__ int site(int i) { return (opaque_tid() + NT * i) * 2; }
 
   __device__ static __forceinline__ void to_lds(C* L, const C* F) {
-#pragma unroll 4
+#pragma unroll
     for (int i = 0; i < HPAIRS; ++i) {
       const int s = site(i), y = s / NX, x = s - y * NX;
       const C a = F[s], b = F[s + 1];
@@ -48,7 +48,7 @@ struct Stream {
     }
   }
   __device__ static __forceinline__ void from_lds(const C* L, C* F) {
-#pragma unroll 4
+#pragma unroll
     for (int i = 0; i < HPAIRS; ++i) {
       const int s = site(i), y = s / NX, x = s - y * NX;
       F[s] = L[y * RS + x];
@@ -121,10 +121,14 @@ struct Stream {
 
   // F -> T^-1 op(T F) in place (F one plane in device memory).  twx: W_NX^m,
   // twy: W_NY^m (m < NY), twh: W_H^m (m < H).  op(X, kx, ky) as in spectral().
+  // Five plane transfers: the second half-plane's forward x result stays in
+  // LDS for the y split, and the first half-plane's merge result goes straight
+  // into LDS for its inverse x pass (every LDS position is read and rewritten
+  // by the one thread that owns its site, between barriers).
   template <class Op>
   __device__ static __forceinline__ void propagate(C* L, C* F, const C* twx, const C* twy, const C* twh,
                                                     const Op& op) {
-    // forward x on each half-plane
+    // forward x on each half-plane; the second stays in LDS
 #pragma unroll 1
     for (int h = 0; h < 2; ++h) {
       __syncthreads();
@@ -134,10 +138,10 @@ struct Stream {
       __syncthreads();
       Lh::template dif2<0>(L);
       __syncthreads();
-      from_lds(L, F + h * NSH);
+      if (h == 0) from_lds(L, F);
     }
-    __syncthreads();
-    // y: radix-2 split across the halves; odd part parked in registers
+    // y: radix-2 split across the halves (first half from F, second from LDS,
+    // in place); odd part parked in registers
     C w[2 * HPAIRS];
 #pragma unroll
     for (int i = 0; i < HPAIRS; ++i) {
@@ -145,7 +149,7 @@ struct Stream {
       const C t = twy[y];
 #pragma unroll
       for (int v = 0; v < 2; ++v) {
-        const C a0 = F[s + v], a1 = F[NSH + s + v];
+        const C a0 = F[s + v], a1 = L[y * RS + x + v];
         L[y * RS + x + v] = a0 + a1;
         w[2 * i + v] = (a0 - a1) * t;
       }
@@ -162,7 +166,8 @@ struct Stream {
       }
     }
     half_y(L, twh, op, 1);
-    // radix-2 merge: a[y] = e + W^-y o, a[y+H] = e - W^-y o
+    // radix-2 merge: a[y] = e + W^-y o into LDS (in place of o), a[y+H] = e - W^-y o
+    // to the second half of F
 #pragma unroll
     for (int i = 0; i < HPAIRS; ++i) {
       const int s = site(i), y = s / NX, x = s - y * NX;
@@ -170,16 +175,18 @@ struct Stream {
 #pragma unroll
       for (int v = 0; v < 2; ++v) {
         const C e = F[s + v], o = L[y * RS + x + v] * t;
-        F[s + v] = e + o;
+        L[y * RS + x + v] = e + o;
         F[NSH + s + v] = e - o;
       }
     }
-    // inverse x on each half-plane
+    // inverse x on each half-plane (the first is in LDS already)
 #pragma unroll 1
     for (int h = 0; h < 2; ++h) {
       __syncthreads();
-      to_lds(L, F + h * NSH);
-      __syncthreads();
+      if (h == 1) {
+        to_lds(L, F + NSH);
+        __syncthreads();
+      }
       Lh::template dita<0>(L, twx);
       __syncthreads();
       Lh::template ditb<0>(L);
@@ -384,45 +391,73 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
     };
 
     // lev: time level of the step's start in half fine steps (a coarse step
-    // spans twice the levels of a fine one)
+    // spans twice the levels of a fine one).  The propagations of both methods
+    // run from one code site (two phases, each a pointwise stage and then one
+    // or two propagations): the 2D transform is inlined once, not once per call.
     auto method_step = [&](uint32_t q, bool coarse, int lev) __attribute__((always_inline)) {
       const int hl = coarse ? 2 : 1;
+      const bool rk = A.method == XSH_RK4;
       __syncthreads();
       if (A.knoises) kfield(static_cast<const C*>(A.kn_filter), A.knoises, TAG_KNOISE, q, coarse);
-      if (A.method == XSH_RK4) {
-        // RK4.m:9-14: a1 = a + D(a) dt into F1
-        setbound(F0, lev);
 #pragma unroll 1
-        for (int i = 0; i < PAIRS; ++i) {
-          const int s = S::site(i);
-          C w[2];
-          noise(s, q, coarse, w[0], w[1]);
-#pragma unroll
-          for (int v = 0; v < 2; ++v) F1[s + v] = F0[s + v] + hdt * D(F0[s + v], w[v]);
-        }
-        propagate(F0, lev);
-        propagate(F1, lev);
-        // RK4.m:18-32: a3 -> F0, R -> F1 (Dirichlet sites at the midpoint's values)
+      for (int ph = 0; ph < 2; ++ph) {
+        if (rk && ph == 0) {
+          // RK4.m:9-14: a1 = a + D(a) dt into F1
+          setbound(F0, lev);
 #pragma unroll 1
-        for (int i = 0; i < PAIRS; ++i) {
-          const int s = S::site(i);
-          C w[2];
-          noise(s, q, coarse, w[0], w[1]);
+          for (int i = 0; i < PAIRS; ++i) {
+            const int s = S::site(i);
+            C w[2];
+            noise(s, q, coarse, w[0], w[1]);
 #pragma unroll
-          for (int v = 0; v < 2; ++v) {
-            int dd = -1, sd = 0;
-            const bool dz = MAT && A.dirichlet && dir_end(s + v, dd, sd);
-            const C A0 = F0[s + v], A1 = F1[s + v];
-            C a2 = A0 + hdt * D(A1, w[v]);
-            if (dz) a2 = dir_value(s + v, dd, sd, lev + hl);
-            C a3 = A0 + (R(2) * hdt) * D(a2, w[v]);
-            if (dz) a3 = dir_value(s + v, dd, sd, lev + hl);
-            F1[s + v] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
-            F0[s + v] = a3;
+            for (int v = 0; v < 2; ++v) F1[s + v] = F0[s + v] + hdt * D(F0[s + v], w[v]);
+          }
+        } else if (rk) {
+          // RK4.m:18-32: a3 -> F0, R -> F1 (Dirichlet sites at the midpoint's values)
+#pragma unroll 1
+          for (int i = 0; i < PAIRS; ++i) {
+            const int s = S::site(i);
+            C w[2];
+            noise(s, q, coarse, w[0], w[1]);
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+              int dd = -1, sd = 0;
+              const bool dz = MAT && A.dirichlet && dir_end(s + v, dd, sd);
+              const C A0 = F0[s + v], A1 = F1[s + v];
+              C a2 = A0 + hdt * D(A1, w[v]);
+              if (dz) a2 = dir_value(s + v, dd, sd, lev + hl);
+              C a3 = A0 + (R(2) * hdt) * D(a2, w[v]);
+              if (dz) a3 = dir_value(s + v, dd, sd, lev + hl);
+              F1[s + v] = R(1.0 / 3.0) * (A1 + R(2) * a2 + a3 - A0);
+              F0[s + v] = a3;
+            }
+          }
+        } else if (ph == 1) {
+          // MP.m:13-24 (after the first propagation)
+#pragma unroll 1
+          for (int i = 0; i < PAIRS; ++i) {
+            const int s = S::site(i);
+            C w[2];
+            noise(s, q, coarse, w[0], w[1]);
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+              int dd = -1, sd = 0;
+              const bool dz = MAT && A.dirichlet && dir_end(s + v, dd, sd);
+              const C a0 = F0[s + v];
+              C a = a0;
+              for (int it = 0; it < A.iterations; ++it) {
+                a = a0 + hdt * D(a, w[v]);
+                if (dz) a = dir_value(s + v, dd, sd, lev + hl);
+              }
+              F0[s + v] = R(2) * a - a0;
+            }
           }
         }
-        propagate(F0, lev + 2 * hl);
-        propagate(F1, lev + 2 * hl);
+        // RK4: P a, P a1 (phase 0), P a3, P R (phase 1); MP: P a (phase 0), P (2a - a0) (phase 1)
+#pragma unroll 1
+        for (int k = 0; k < (rk ? 2 : 1); ++k) propagate(k ? F1 : F0, lev + ph * 2 * hl);
+      }
+      if (rk) {
         // RK4.m:33-41: a = P R + D(P a3) dt/3
         __syncthreads();
 #pragma unroll 1
@@ -434,28 +469,6 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
           for (int v = 0; v < 2; ++v) F0[s + v] = F1[s + v] + (hdt / R(3)) * D(F0[s + v], w[v]);
         }
         setbound(F0, lev + 2 * hl);
-      } else {
-        // MP.m:13-24
-        propagate(F0, lev);
-#pragma unroll 1
-        for (int i = 0; i < PAIRS; ++i) {
-          const int s = S::site(i);
-          C w[2];
-          noise(s, q, coarse, w[0], w[1]);
-#pragma unroll
-          for (int v = 0; v < 2; ++v) {
-            int dd = -1, sd = 0;
-            const bool dz = MAT && A.dirichlet && dir_end(s + v, dd, sd);
-            const C a0 = F0[s + v];
-            C a = a0;
-            for (int it = 0; it < A.iterations; ++it) {
-              a = a0 + hdt * D(a, w[v]);
-              if (dz) a = dir_value(s + v, dd, sd, lev + hl);
-            }
-            F0[s + v] = R(2) * a - a0;
-          }
-        }
-        propagate(F0, lev + 2 * hl);
       }
     };
 
@@ -557,15 +570,13 @@ __global__ __launch_bounds__(NT) void spde_stream(const PathArgs* __restrict__ a
     for (int n = 0; n < A.npts; ++n) {
       for (int st = 0; st < A.steps; ++st) {
         if (n > 0 || st > 0 || A.npts == 1) {
-          if (A.checks && nci == 1) {
-            method_step(q, true, lev);
-            q += 2;
-            lev += 4;
-          } else {
-            for (int c = 0; c <= A.checks; ++c) {
-              method_step(q++, false, lev);
-              lev += 2;
-            }
+          // a coarse step (draws q, q + 1), or 1 + checks fine steps: one call site
+          const bool coarse = A.checks && nci == 1;
+#pragma unroll 1
+          for (int c = 0; c < (coarse ? 1 : 1 + A.checks); ++c) {
+            method_step(q, coarse, lev);
+            q += coarse ? 2 : 1;
+            lev += coarse ? 4 : 2;
           }
         }
         if (st == 0) observe(n);
