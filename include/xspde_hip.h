@@ -48,7 +48,7 @@ typedef enum {
 
 enum { XSH_FP32 = 0, XSH_FP64 = 1 };
 enum { XSH_RK4 = 0, XSH_MP = 1 };
-enum { XSH_DERIV_ZERO = 0, XSH_DERIV_KUBO = 1, XSH_DERIV_GP = 2 };
+enum { XSH_DERIV_ZERO = 0, XSH_DERIV_KUBO = 1, XSH_DERIV_GP = 2, XSH_DERIV_MATRIX = 3 };
 enum {
   XSH_INIT_CONST = 0,
   XSH_INIT_SECH = 1,
@@ -72,6 +72,7 @@ enum {
 };
 #define XSH_MAX_OBS 8
 #define XSH_MAX_MODES 32
+#define XSH_MAX_FIELDS 4
 
 /* Problem descriptor: the subset of the reference's struct p that the path
  * reads (xpreferences.m:42-219, spdpreferences.m:10-72).  Lattice sizes are
@@ -149,6 +150,27 @@ typedef struct {
      Its averages fill the observable's [points] block with frequency in place of
      time.  SDEs (dimensions = 1); not in batches or xsh_run_path_noalloc. */
   int tspec[XSH_MAX_OBS];
+  /* Coupled field components of an SDE (p.fields > 1 with dimensions = 1,
+     xpreferences.m:44-50; e.g. Examples/SDE_1D/SHO.m): fields 2..XSH_MAX_FIELDS
+     (0 or 1: one component and the forms above).  The drift is then
+     deriv = XSH_DERIV_MATRIX,
+       da_i/dt = sum_j M[i][j] a_j + g[i] |a_i|^2 a_i + sum_k S[i][k] w_k
+     (fmat = M and fnl = g complex, fnoise = S real over the noises <= 4 real
+     noise components w_k of xnoise.m:30-32); the interaction picture
+     propagator exp(fl0[i] dt / ipsteps) per component (p.linear returning one
+     row per field, xpropfactor.m:26-39); the initial state the constants fc0[i]
+     (initial = XSH_INIT_CONST) or a0 as [ensembles(1)][fields] (components
+     fastest, p.d.ca); observable o (XSH_OBS_A / _A_AVE: a_i, _ABS2 / _ABS2_AVE:
+     |a_i|^2, _ABS4_AVE: |a_i|^4) reads component obs_field[o] (0-based).
+     xsh_info.nsites is then the number of components (the size of a0, a_out
+     per trajectory).  Not with tspec, in batches or xsh_run_path_noalloc. */
+  int fields;
+  double fmat[XSH_MAX_FIELDS][XSH_MAX_FIELDS][2];
+  double fnl[XSH_MAX_FIELDS][2];
+  double fnoise[XSH_MAX_FIELDS][4];
+  double fl0[XSH_MAX_FIELDS][2];
+  double fc0[XSH_MAX_FIELDS][2];
+  int obs_field[XSH_MAX_OBS];
 } xsh_desc;
 
 typedef struct xsh_handle xsh_handle;

@@ -15,6 +15,8 @@ deriv    'zero'                        D = 0
          'kubo'   c                    D = i c w0 a                       (Kubo.m:17)
          'gp'     g, u, s              D = g |a|^2 a + u a + s (w0 + i w1)   [noises >= 2]
                                        D = g |a|^2 a + u a + s w0            [noises == 1]
+         'matrix' M, g, S              D_i = sum_j M_ij a_j + g_i |a_i|^2 a_i + sum_k S_ik w_k
+                                       (p.fields coupled SDE components; l0 and c0 per component)
 linear   l0, l1[d], l2[d]              L = l0 + sum_d l1[d] D_d + l2[d] D_d^2  (xpropfactor.m:37-39)
 initial  'const'  c0                   a = c0
          'sech'   amp, width, eps, center
@@ -24,12 +26,13 @@ initial  'const'  c0                   a = c0
          'phase'  amp                  a = amp exp(2 pi i u0),  u0 = uniform(0,1)
          'coherent' alpha              a = alpha + (v0 + i v1)/2  (Wigner vacuum)
 observe  'a', 'abs2', 'a_ave', 'abs2_ave', 'abs2_int', 'abs4_ave', 'corr'
+         (of component ofield[o] - 1 of a multi-component field, default the first)
 """
 import math
 
 import numpy as np
 
-DERIVS = ("zero", "kubo", "gp")
+DERIVS = ("zero", "kubo", "gp", "matrix")
 INITIALS = ("const", "sech", "gauss", "phase", "coherent")
 OBSERVES = ("a", "abs2", "a_ave", "abs2_ave", "abs2_int", "abs4_ave", "corr")
 
@@ -60,6 +63,19 @@ def make_deriv(g):
                 d = d + s * w[0:1]
             return d
         return deriv
+    if kind == "matrix":
+        M = np.asarray(g.get("M"), complex)
+        F = M.shape[0]
+        gn = np.broadcast_to(np.asarray(g.get("g", 0.0), complex), (F,)).reshape(F, 1)
+        S = np.asarray(g.get("S", np.zeros((F, 1))), float).reshape(F, -1)
+
+        def deriv(a, w, p):
+            d = np.tensordot(M, a, axes=(1, 0)) + gn * (np.conj(a) * a) * a
+            nw = min(S.shape[1], w.shape[0]) if np.ndim(w) else 0
+            if nw:
+                d = d + np.tensordot(S[:, :nw], w[:nw], axes=(1, 0))
+            return d
+        return deriv
     raise ValueError("unknown drift form %r" % (kind,))
 
 
@@ -67,6 +83,9 @@ def make_linear(g):
     lin = g.get("linear", None)
     if lin is None:
         return None
+    if g.get("deriv") == "matrix":                      # one l0 per component (an SDE)
+        l0v = np.asarray(lin.get("l0", 0.0), complex).reshape(-1, 1)
+        return lambda p: l0v
     l0 = complex(lin.get("l0", 0.0))
     l1 = lin.get("l1", 0.0)
     l2 = lin.get("l2", 0.0)
@@ -85,6 +104,9 @@ def make_linear(g):
 def make_initial(g):
     ini = g.get("initial", "const")
     if ini == "const":
+        if np.ndim(g.get("c0", 1.0)) > 0:             # one constant per component
+            cv = np.asarray(g.get("c0"), complex).reshape(-1, 1)
+            return lambda *args: cv + 0j
         c = complex(g.get("c0", 1.0))
         return lambda *args: c + 0 * args[0]
     if ini == "sech":
@@ -136,7 +158,10 @@ def lowest_modes(nx, ny, count):
     return [(a % nx, b % ny) for _, _, a, b in cand[:count]]
 
 
-def make_observe(name, g=None):
+def make_observe(name, g=None, field=0):
+    if field:
+        f0 = make_observe(name, g)
+        return lambda a, p: f0(a[field:field + 1], p)
     if name == "lowk":
         # occupation of the feature modes other than k = 0:
         # sum_k (|a_k|^2 - 1/2), a_k = sqrt(dv/Ns) sum_x a(x) exp(-i k.x)
@@ -189,7 +214,8 @@ def apply_forms(p_in):
     p["linear"] = make_linear(g)
     p["initial"] = make_initial(g)
     names = g.get("observe", ["a"])
-    p["observe"] = [make_observe(o, g) for o in names]
+    of = list(g.get("ofield") or [1] * len(names))
+    p["observe"] = [make_observe(o, g, int(f) - 1) for o, f in zip(names, of)]
     if "abs2k" in names:
         nd = p["dimensions"]
         given = p.get("transforms") or [0] * len(names)

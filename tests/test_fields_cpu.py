@@ -66,3 +66,71 @@ def test_refusals():
         field_problems(p, preferences(p))
     with pytest.raises(GpuFormError, match="one field component"):
         make_desc(preferences(_four_fields()))
+
+
+def _coupled(**kw):
+    p = dict(dimensions=1, fields=3, noises=2, points=[6], ranges=[1.0], steps=[2],
+             gpu=dict(deriv="matrix", M=np.eye(3) * 0.1j, S=[[1, 0], [0, 1], [0.5, 0.5]], initial="const",
+                      c0=[1, 2, 3], observe=["a", "abs2"], ofield=[3, 1], precision="fp64"))
+    p.update(kw)
+    return p
+
+
+def test_coupled_components_descriptor_and_plan():
+    """Coupled SDE components: one descriptor (fields, M, g, S, l0 and c0 per
+    component, the component of each observable), planned without a device:
+    a0 / a_out hold the components of a trajectory (nsites = fields), each
+    observable one value per time point."""
+    d = make_desc(preferences(_coupled()))
+    assert d.fields == 3 and d.deriv == _lib.DERIV["matrix"]
+    assert [d.obs_field[o] for o in range(2)] == [2, 0]
+    assert d.fnoise[2][0] == 0.5 and d.fmat[1][1][1] == 0.1 and d.fc0[2][0] == 3.0
+    info = _lib.plan(d)
+    assert info.nsites == 3 and list(info.obs_sites)[:2] == [1, 1] and info.avlen == 12
+
+
+def _set(d, **kw):
+    for k, v in kw.items():
+        if k == "obs_field0":
+            d.obs_field[0] = v
+        elif k == "tspec0":
+            d.tspec[0] = v
+        else:
+            setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize("change,status", [
+    (dict(fields=1), "EINVAL"),                          # the matrix form with one component
+    (dict(fields=5), "EUNSUPPORTED"),                    # more components than the kernels hold
+    (dict(dimensions=2, nx=32), "EUNSUPPORTED"),          # coupled lattice components
+    (dict(noises=6), "EUNSUPPORTED"),                    # more than 4 noise components
+    (dict(obs_field0=3), "EINVAL"),                      # observable of a component that is not there
+    (dict(tspec0=1), "EUNSUPPORTED"),                    # time-domain spectra of coupled components
+    (dict(deriv=_lib.DERIV["gp"]), "EINVAL"),            # fields > 1 without the matrix form
+])
+def test_coupled_components_refusals(change, status):
+    """The C ABI refuses coupled descriptors it cannot integrate (no device needed)."""
+    d = _set(make_desc(preferences(_coupled())), **change)
+    if "dimensions" in change:
+        d.ranges[1] = 5.0
+    with pytest.raises(_lib.XshError, match=status):
+        _lib.plan(d)
+    with pytest.raises(GpuFormError, match="4 noise components"):
+        make_desc(preferences(_coupled(noises=6)))
+
+
+def test_oracle_sho_closed_form():
+    """Examples/SDE_1D/SHO.m's deterministic project in the oracle (the matrix
+    form, MP, order 2): the mean of a_1 is cos(t) within the extrapolated step
+    error, that of a_2 is sin(t), and a wrong sign of the coupling fails."""
+    from oracle import forms, xspde as X
+    p = dict(dimensions=1, fields=2, noises=0, points=[51], ranges=[10.0], steps=[20], checks=1, order=2,
+             ensembles=[3, 1, 1], method="MP",
+             gpu=dict(deriv="matrix", M=[[0, -1], [1, 0]], initial="const", c0=[1.0, 0.0], observe=["a", "a"],
+                      ofield=[1, 2]),
+             compare=[lambda p: np.cos(p.t), lambda p: np.sin(p.t)])
+    err = X.xsim(forms.apply_forms(p))[0]
+    assert err[0] < 1e-4                                # MP, dt = 0.025, order 2: 5.6e-6
+    bad = dict(p, gpu=dict(p["gpu"], M=[[0, 1], [-1, 0]]))
+    assert X.xsim(forms.apply_forms(bad))[0][0] > 1e-2

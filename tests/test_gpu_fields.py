@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from oracle import xspde as X
-from xspde_matlab_amd.xsim import FieldsProblem, field_problems, preferences, xsim
+from xspde_matlab_amd.xsim import FieldsProblem, Problem, field_problems, preferences, xsim
 
 from _problems import oracle_problem
 
@@ -175,3 +175,77 @@ def test_sde_fields_and_xsim():
         z = (data[o][:, 0] - np.exp(-c * c * t / 2)) / np.maximum(data[o][:, 2], 1e-12)
         print("Kubo2 field c=%.1f: max |z| %.2f" % (c, np.abs(z[1:]).max()), flush=True)
         assert np.abs(z[1:]).max() < 5
+
+
+# ---------------------------------------------------------------- coupled SDE components
+# The coupled-component kernel (sde_fields) was written after GPU access closed
+# for this build's last session: these parity tests have not run on an MI355X.
+# They stay skipped until they do; the oracle side is pinned on the CPU
+# (tests/test_fields_cpu.py: the SHO closed form, descriptor planning, refusals).
+COUPLED_NOT_RUN = pytest.mark.skip(reason="sde_fields has not run on an MI355X yet (no GPU access after it was "
+                                          "written); oracle side pinned in test_fields_cpu.py")
+
+
+def sho(prec, method="MP"):
+    """Examples/SDE_1D/SHO.m:9-20: da = (-a2, a1) dt from (1, 0), 20 steps per
+    point, order 2, ensembles 3 (MP by the xpreferences default)."""
+    return dict(name="SHO", dimensions=1, fields=2, noises=0, points=[51], ranges=[10.0], steps=[20], checks=1,
+                order=2, ensembles=[3, 1, 1], method=method,
+                gpu=dict(deriv="matrix", M=[[0, -1], [1, 0]], initial="const", c0=[1.0, 0.0],
+                         observe=["a", "a", "abs2"], ofield=[1, 2, 1], precision=prec))
+
+
+@COUPLED_NOT_RUN
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
+@pytest.mark.parametrize("method", ["MP", "RK4"])
+def test_coupled_sho_matches_oracle_and_cosine(prec, tol, method):
+    """Coupled components on the GPU (one descriptor, the matrix drift form)
+    against the oracle path, fine and coarse, and the extrapolated ensemble mean
+    of a_1 against the closed form cos(t) (the compare of SHO.m:16)."""
+    p = sho(prec, method)
+    prob = Problem(p)
+    assert prob.info.nsites == 2 and prob.info.kernel_kind == 3
+    q = X.xpreferences(oracle_problem(p))
+    gpu_av = {}
+    for nc in (1, 2):
+        av_g, a_g = prob.xpath(nc, npar=1, j=1, want_final=True)
+        a_r, av_r = X.run_path(q, 1, nc)
+        _cmp(np.asarray(a_g).reshape(3, 2), a_r.T, tol, "SHO %s %s final fields nc=%d" % (prec, method, nc))
+        for o in range(3):
+            _cmp(av_g[o], av_r[o], tol, "SHO %s %s observable %d nc=%d" % (prec, method, o + 1, nc))
+        gpu_av[nc] = av_g
+    prob.close()
+    q.compare = [lambda p: np.cos(p.t), lambda p: np.sin(p.t), lambda p: np.cos(p.t) ** 2]
+    q._avshape = X._probe_shapes(q)
+    data = X.xensemble(1, q, path_fn=lambda p_, j, nc, npar: gpu_av[nc])
+    err = X.xsim_stats(q, data)[0]
+    print("SHO %s %s on the GPU: error vector %s" % (prec, method, np.array2string(err[:5], precision=3)), flush=True)
+    # MP's extrapolated error vs cos(t) is 5.6e-6 (oracle), RK4's 9e-11
+    assert err[0] < (1e-4 if method == "MP" else (1e-8 if prec == "fp64" else 2e-6))
+
+
+@COUPLED_NOT_RUN
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 5e-5)])
+@pytest.mark.parametrize("method", ["RK4", "MP"])
+def test_coupled_stochastic_components_match_oracle(prec, tol, method):
+    """Three coupled components with a complex coupling matrix, a nonlinearity
+    per component, propagators per component and two noise components mixed by
+    S: every trajectory's final components and every observable against the
+    oracle path (same counter stream), fine and coarse, two sub-ensembles."""
+    M = [[-0.2 + 0.5j, 0.3, 0.0], [-0.3, -0.1j, 0.2 + 0.1j], [0.1j, -0.2, -0.3]]
+    p = dict(name="Coupled3", dimensions=1, fields=3, noises=2, points=[6], ranges=[1.5], steps=[3], checks=1,
+             ensembles=[300, 2, 1], method=method, seed=9,
+             gpu=dict(deriv="matrix", M=M, g=[-0.2j, 0.1, -0.05 + 0.1j], S=[[0.3, 0.0], [0.1, 0.2], [0.0, -0.25]],
+                      linear=dict(l0=[-0.1 + 0.4j, 0.2j, -0.05]), initial="const", c0=[1.0, 0.5j, -0.3 + 0.2j],
+                      observe=["a", "abs2", "a_ave", "abs4_ave"], ofield=[3, 1, 2, 3], precision=prec))
+    prob = Problem(p)
+    q = X.xpreferences(oracle_problem(p))
+    for j, nc in ((1, 1), (2, 2)):
+        av_g, a_g = prob.xpath(nc, npar=1, j=j, want_final=True)
+        a_r, av_r = X.run_path(q, j, nc)
+        _cmp(np.asarray(a_g).reshape(300, 3), a_r.T, tol, "Coupled3 %s %s final j=%d nc=%d" % (prec, method, j, nc))
+        for o in range(4):
+            _cmp(av_g[o], av_r[o], tol, "Coupled3 %s %s observable %d j=%d nc=%d" % (prec, method, o + 1, j, nc))
+    prob.close()
+    err, data, _, _ = xsim(p)
+    assert len(data) == 4 and all(np.all(np.isfinite(d)) for d in data)

@@ -14,9 +14,10 @@ LIB_PATH = os.path.join(HERE, "libxspde_hip.so")
 
 XSH_MAX_OBS = 8
 XSH_MAX_MODES = 32
+XSH_MAX_FIELDS = 4
 FP32, FP64 = 0, 1
 RK4, MP = 0, 1
-DERIV = {"zero": 0, "kubo": 1, "gp": 2}
+DERIV = {"zero": 0, "kubo": 1, "gp": 2, "matrix": 3}
 INITIAL = {"const": 0, "sech": 1, "gauss": 2, "phase": 3, "coherent": 4}
 OBSERVE = {"a": 0, "abs2": 1, "a_ave": 2, "abs2_ave": 3, "abs2_int": 4, "abs4_ave": 5, "corr": 6, "lowk": 7, "abs2k": 8}
 STATUS = {0: "XSH_OK", 1: "XSH_EINVAL", 2: "XSH_EUNSUPPORTED", 3: "XSH_ENOMEM", 4: "XSH_EDEVICE",
@@ -53,6 +54,13 @@ class Desc(C.Structure):
         ("knoises", C.c_int), ("krandoms", C.c_int),
         ("nfilter", C.POINTER(C.c_double)), ("rfilter", C.POINTER(C.c_double)),
         ("tspec", C.c_int * XSH_MAX_OBS),
+        ("fields", C.c_int),
+        ("fmat", ((C.c_double * 2) * XSH_MAX_FIELDS) * XSH_MAX_FIELDS),
+        ("fnl", (C.c_double * 2) * XSH_MAX_FIELDS),
+        ("fnoise", (C.c_double * 4) * XSH_MAX_FIELDS),
+        ("fl0", (C.c_double * 2) * XSH_MAX_FIELDS),
+        ("fc0", (C.c_double * 2) * XSH_MAX_FIELDS),
+        ("obs_field", C.c_int * XSH_MAX_OBS),
     ]
 
     def set_filters(self, nfilter=None, rfilter=None):

@@ -116,9 +116,32 @@ static int validate(const xsh_desc* d) {
   if (d->method == XSH_MP && d->iterations < 1) return fail(XSH_EINVAL, "iterations must be >= 1");
   for (int i = 0; i < 3; ++i)
     if (d->ensembles[i] < 1) return fail(XSH_EINVAL, "ensembles(%d) must be >= 1", i + 1);
-  if (d->noises < 0 || d->noises > 2) return fail(XSH_EUNSUPPORTED, "noises %d: built-in forms use 0..2", d->noises);
+  if (d->fields < 0 || d->fields > XSH_MAX_FIELDS)
+    return fail(XSH_EUNSUPPORTED, "fields %d: the GPU path integrates 1..%d components", d->fields, XSH_MAX_FIELDS);
+  const int nf = d->fields > 1 ? d->fields : 1;
+  if (d->noises < 0 || d->noises > (nf > 1 ? 4 : 2))
+    return fail(XSH_EUNSUPPORTED, "noises %d: built-in forms use 0..%d", d->noises, nf > 1 ? 4 : 2);
+  if (nf > 1) {
+    // coupled components: the matrix drift on SDEs (lattice projects with several
+    // components split them into one-component descriptors)
+    if (d->dimensions != 1)
+      return fail(XSH_EUNSUPPORTED, "coupled field components run on SDEs (dimensions = 1); a lattice project's "
+                                    "uncoupled components run as one descriptor each");
+    if (d->deriv != XSH_DERIV_MATRIX) return fail(XSH_EINVAL, "fields > 1 needs the matrix drift form");
+    if (d->initial != XSH_INIT_CONST) return fail(XSH_EUNSUPPORTED, "coupled components start from constants (or a0)");
+    for (int o = 0; o < d->nobserve && o < XSH_MAX_OBS; ++o) {
+      const int k = d->observe[o];
+      if (k != XSH_OBS_A && k != XSH_OBS_A_AVE && k != XSH_OBS_ABS2 && k != XSH_OBS_ABS2_AVE && k != XSH_OBS_ABS4_AVE)
+        return fail(XSH_EUNSUPPORTED, "observe form %d: coupled components observe a_i, |a_i|^2 or |a_i|^4", k);
+      if (d->obs_field[o] < 0 || d->obs_field[o] >= nf)
+        return fail(XSH_EINVAL, "obs_field[%d] = %d outside 0..%d", o, d->obs_field[o], nf - 1);
+      if (d->tspec[o]) return fail(XSH_EUNSUPPORTED, "time-domain spectra of coupled components are not on the GPU path");
+    }
+  } else if (d->deriv == XSH_DERIV_MATRIX) {
+    return fail(XSH_EINVAL, "the matrix drift form couples fields >= 2 components");
+  }
   if (d->inrandoms < 0 || d->urandoms < 0) return fail(XSH_EINVAL, "negative random count");
-  if (d->deriv < XSH_DERIV_ZERO || d->deriv > XSH_DERIV_GP) return fail(XSH_EUNSUPPORTED, "unknown drift form %d", d->deriv);
+  if (d->deriv < XSH_DERIV_ZERO || d->deriv > XSH_DERIV_MATRIX) return fail(XSH_EUNSUPPORTED, "unknown drift form %d", d->deriv);
   if (d->deriv == XSH_DERIV_KUBO && d->noises < 1) return fail(XSH_EINVAL, "Kubo drift needs noises >= 1");
   if (d->initial < XSH_INIT_CONST || d->initial > XSH_INIT_COHERENT) return fail(XSH_EUNSUPPORTED, "unknown initial form %d", d->initial);
   if (d->knoises < 0 || d->knoises > 2) return fail(XSH_EUNSUPPORTED, "knoises %d: built-in forms use 0..2", d->knoises);
@@ -282,6 +305,7 @@ static int plan_handle(xsh_handle* h, const xsh_desc* desc, std::vector<char>& h
   h->nx = d.dimensions > 1 ? d.nx : 1;
   h->ny = d.dimensions > 2 ? d.ny : 1;
   h->ns = h->nx * h->ny;
+  if (d.dimensions == 1 && d.fields > 1) h->ns = d.fields;  // a0, a_out: [E][fields]
   h->E = d.ensembles[0];
   h->e2 = d.ensembles[1];
   h->e3 = d.ensembles[2];
@@ -366,7 +390,7 @@ static int plan_handle(xsh_handle* h, const xsh_desc* desc, std::vector<char>& h
   for (int o = 0; o < d.nobserve; ++o) {
     const int kind = d.observe[o];
     const bool site = kind == XSH_OBS_A || kind == XSH_OBS_ABS2 || kind == XSH_OBS_CORR || kind == XSH_OBS_ABS2K;
-    h->obs_ns[o] = site ? h->ns : 1;
+    h->obs_ns[o] = site && !(d.dimensions == 1 && d.fields > 1) ? h->ns : 1;
     h->obs_off[o] = off;
     off += (long long)d.points * h->obs_ns[o];
   }
@@ -858,6 +882,26 @@ static xsh::PathArgs make_args(const xsh_handle* h, int npar) {
   A.has_patch = h->has_patch;
   A.bshape = t + h->tab_off[15];
   A.bshape_n = h->bshape_n;
+  A.fields = d.fields > 1 ? d.fields : 1;
+  if (A.fields > 1) {
+    for (int i = 0; i < A.fields; ++i) {
+      for (int jj = 0; jj < A.fields; ++jj)
+        for (int c = 0; c < 2; ++c) A.fm[i][jj][c] = d.fmat[i][jj][c];
+      for (int c = 0; c < 2; ++c) {
+        A.fg[i][c] = d.fnl[i][c];
+        A.fc0[i][c] = d.fc0[i][c];
+      }
+      for (int k = 0; k < 4; ++k) A.fs[i][k] = k < d.noises ? d.fnoise[i][k] : 0.0;
+      // per-component interaction picture propagator exp(l0_i dtr / ipsteps)
+      // (xpropfactor.m:26-39 with ipsteps = 2)
+      for (int nci = 0; nci < 2; ++nci) {
+        const cd P = d.has_linear ? std::exp(cd(d.fl0[i][0], d.fl0[i][1]) * (h->dtr[nci] / 2.0)) : cd(1.0, 0.0);
+        A.fp[nci][i][0] = P.real();
+        A.fp[nci][i][1] = P.imag();
+      }
+    }
+    for (int o = 0; o < d.nobserve; ++o) A.obs_field[o] = d.obs_field[o];
+  }
   A.bval = t + h->tab_off[16];
   A.bval_level = h->bval_level;
   A.bval_nb = h->bval_nb;
@@ -897,7 +941,8 @@ static int launch(xsh_handle* h, const xsh::PathArgs& A, int nblocks) {
   if (rc) return rc;
   hipError_t e;
   if (h->nd == 1) {
-    e = xsh::launch_sde_path(h->d.precision, h->d_args, nblocks, h->stream);
+    e = h->d.fields > 1 ? xsh::launch_sde_fields(h->d.precision, h->d.fields, h->d_args, nblocks, h->stream)
+                        : xsh::launch_sde_path(h->d.precision, h->d_args, nblocks, h->stream);
     if (e == hipSuccess && h->tspec_mask)
       e = xsh::launch_sde_spectrum(h->d_args, h->d_twt, h->fsc, h->E, h->d.points, A.j_only > 0 ? 1 : h->e2 * h->nchk,
                                    h->stream);
@@ -1026,6 +1071,7 @@ extern "C" int xsh_run_path_noalloc(xsh_handle* h, int npar, int j, int nc, cons
   if (!av_dev) return fail(XSH_EINVAL, "av_dev is NULL");
   if (h->tspec_mask)
     return fail(XSH_EUNSUPPORTED, "time-domain spectra keep every step's fields: not in the scratch of this entry");
+  if (h->d.fields > 1) return fail(XSH_EUNSUPPORTED, "coupled SDE components are not on this entry");
   int dev = -1;
   HIPCHK(hipGetDevice(&dev));
   if (dev != h->device) return fail(XSH_ESTATE, "handle belongs to device %d, current device is %d", h->device, dev);

@@ -81,6 +81,16 @@ hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStr
   return precision == XSH_FP64 ? launch_sde<double>(A, nblocks, s) : launch_sde<float>(A, nblocks, s);
 }
 
+hipError_t launch_sde_fields(int precision, int fields, const PathArgs* A, int nblocks, hipStream_t s) {
+  const bool f64 = precision == XSH_FP64;
+  switch (fields) {
+    case 2: return f64 ? launch_sde_f<double, 2>(A, nblocks, s) : launch_sde_f<float, 2>(A, nblocks, s);
+    case 3: return f64 ? launch_sde_f<double, 3>(A, nblocks, s) : launch_sde_f<float, 3>(A, nblocks, s);
+    case 4: return f64 ? launch_sde_f<double, 4>(A, nblocks, s) : launch_sde_f<float, 4>(A, nblocks, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // Time-domain spectra of an SDE ensemble (xpath.m:90-106): per trajectory and
 // kept frequency m, a(w_m) = fsc/M sum_n a_n exp(+2 pi i j_m n / M) (MATLAB's
 // ifft times fsc, fftshifted, at specpts: j_m = m - floor(points/2) mod M) over

@@ -271,6 +271,164 @@ __global__ __launch_bounds__(256) void sde_path(const PathArgs* __restrict__ Ap)
   }
 }
 
+// Coupled components of an SDE (p.fields = F > 1): one trajectory per lane with
+// its F components in registers; the matrix drift
+//   D_i(a, w) = sum_j M_ij a_j + g_i |a_i|^2 a_i + sum_k S_ik w_k
+// (w: the noises <= 4 real components of one draw, xnoise.m:30-32, from one
+// Philox block), the diagonal interaction picture propagator P_i, and RK4 /
+// MP exactly as sde_path does them with vector-valued a (RK4.m:7-41, MP.m:10-26).
+template <class R, int F>
+__global__ __launch_bounds__(256) void sde_fields(const PathArgs* __restrict__ Ap) {
+  const PathArgs& A = *Ap;
+  using C = cpx<R>;
+  __shared__ double red[8];
+  int pidx, nci, j, group;
+  path_identity<R>(A, blockIdx.x, pidx, nci, j, group);
+  const int traj = group * 256 + threadIdx.x;
+  const bool active = traj < A.E;
+  const uint32_t nsp = (uint32_t)(A.seed + A.npar + (j - 1) * A.e3);
+  const R hdt = (R)(nci ? A.hdt[1] : A.hdt[0]);
+  double* av = A.av + 2 * (long long)pidx * A.avlen;
+  C P[F];
+#pragma unroll
+  for (int i = 0; i < F; ++i) P[i] = C{(R)A.fp[nci][i][0], (R)A.fp[nci][i][1]};
+
+  C a[F];
+#pragma unroll
+  for (int i = 0; i < F; ++i) {
+    if (A.a0) {
+      const long long g = (long long)(active ? traj : 0) * F + i;
+      a[i] = C{(R)A.a0[2 * g], (R)A.a0[2 * g + 1]};
+    } else {
+      a[i] = C{(R)A.fc0[i][0], (R)A.fc0[i][1]};
+    }
+  }
+  // one draw: noise components k < noises, scaled (xnoise.m:32)
+  auto draw = [&](uint32_t q, R (&w)[4]) __attribute__((always_inline)) {
+    const R sc = (R)A.noise_scale;
+    R n[4];
+    normals4<R>(u4{0u, q, (uint32_t)traj, (uint32_t)A.inst}, nsp, key1(TAG_NOISE, (uint32_t)A.seq), n);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = k < A.noises ? sc * n[k] : R(0);
+  };
+  auto D = [&](const C (&x)[F], const R (&w)[4], C (&out)[F]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < F; ++i) {
+      C acc = C{R(0), R(0)};
+#pragma unroll
+      for (int jj = 0; jj < F; ++jj) acc = acc + C{(R)A.fm[i][jj][0], (R)A.fm[i][jj][1]} * x[jj];
+      acc = acc + (norm2(x[i]) * C{(R)A.fg[i][0], (R)A.fg[i][1]}) * x[i];
+      R sw = R(0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sw += (R)A.fs[i][k] * w[k];
+      out[i] = acc + C{sw, R(0)};
+    }
+  };
+  auto step = [&](const R (&w)[4]) __attribute__((always_inline)) {
+    C d[F];
+    if (A.method == XSH_RK4) {
+      C a1[F], a2[F], a3[F];
+      D(a, w, d);
+#pragma unroll
+      for (int i = 0; i < F; ++i) {
+        a1[i] = P[i] * (a[i] + hdt * d[i]);
+        a[i] = P[i] * a[i];
+      }
+      D(a1, w, d);
+#pragma unroll
+      for (int i = 0; i < F; ++i) a2[i] = a[i] + hdt * d[i];
+      D(a2, w, d);
+#pragma unroll
+      for (int i = 0; i < F; ++i) {
+        a3[i] = a[i] + (R(2) * hdt) * d[i];
+        a[i] = P[i] * (R(1.0 / 3.0) * (a1[i] + R(2) * a2[i] + a3[i] - a[i]));
+        a3[i] = P[i] * a3[i];
+      }
+      D(a3, w, d);
+#pragma unroll
+      for (int i = 0; i < F; ++i) a[i] = a[i] + (hdt / R(3)) * d[i];
+    } else {
+      C a0[F];
+#pragma unroll
+      for (int i = 0; i < F; ++i) a0[i] = a[i] = P[i] * a[i];
+      for (int it = 0; it < A.iterations; ++it) {
+        D(a, w, d);
+#pragma unroll
+        for (int i = 0; i < F; ++i) a[i] = a0[i] + hdt * d[i];
+      }
+#pragma unroll
+      for (int i = 0; i < F; ++i) a[i] = P[i] * (R(2) * a[i] - a0[i]);
+    }
+  };
+  auto observe = [&](int n) __attribute__((always_inline)) {
+    if (A.nonfinite && active) {
+      bool bad = false;
+#pragma unroll
+      for (int i = 0; i < F; ++i) bad = bad || not_finite(a[i]);
+      if (bad) atomicOr(A.nonfinite, 1);
+    }
+#pragma unroll
+    for (int o = 0; o < XSH_MAX_OBS; ++o) {
+      if (o >= A.nobs) break;
+      double re = 0.0, im = 0.0;
+      if (active) {
+        C x = a[0];
+#pragma unroll
+        for (int i = 1; i < F; ++i)
+          if (A.obs_field[o] == i) x = a[i];
+        switch (A.obs[o]) {
+          case XSH_OBS_A:
+          case XSH_OBS_A_AVE:
+            re = (double)x.x;
+            im = (double)x.y;
+            break;
+          case XSH_OBS_ABS4_AVE: {
+            const double m = (double)norm2(x);
+            re = m * m;
+            break;
+          }
+          default:
+            re = (double)norm2(x);
+        }
+      }
+      block_sum2<256>(re, im, red);
+      if (threadIdx.x == 0) atomic_add_c(av + 2 * (A.obs_off[o] + n), re * A.inv_E, im * A.inv_E);
+    }
+  };
+
+  uint32_t q = 0;
+  for (int n = 0; n < A.npts; ++n) {
+    for (int st = 0; st < A.steps; ++st) {
+      if (n > 0 || st > 0 || A.npts == 1) {
+        if (A.checks && nci == 1) {
+          R w0[4], w1[4], w[4];
+          draw(q, w0);
+          draw(q + 1, w1);
+          q += 2;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) w[k] = R(0.5) * (w0[k] + w1[k]);
+          step(w);
+        } else {
+          for (int c = 0; c <= A.checks; ++c) {
+            R w[4];
+            draw(q++, w);
+            step(w);
+          }
+        }
+      }
+      if (st == 0) observe(n);
+    }
+  }
+  if (A.a_out && active) {
+#pragma unroll
+    for (int i = 0; i < F; ++i) {
+      const long long g = (long long)traj * F + i;
+      A.a_out[2 * g] = (double)a[i].x;
+      A.a_out[2 * g + 1] = (double)a[i].y;
+    }
+  }
+}
+
 // ==================================================================== SPDE
 // Instance of work item wk in a batch: the last i with wstart[i] <= wk
 // (wstart[0] = 0; uniform over the workgroup).
@@ -934,6 +1092,12 @@ __global__ void probe_normals(uint32_t nsp, uint32_t tag, uint32_t seq, uint32_t
 template <class R>
 static hipError_t launch_sde(const PathArgs* A, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(sde_path<R>, dim3(nblocks), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+
+template <class R, int F>
+static hipError_t launch_sde_f(const PathArgs* A, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((sde_fields<R, F>), dim3(nblocks), dim3(256), 0, s, A);
   return hipGetLastError();
 }
 

@@ -43,6 +43,8 @@ const KernelEntry* find_padded_kernel(int precision, int nx);
 // those that handle padding (fewer than kPadMaxSites: 64 x 64, 128 x 64, 64 x 128)
 const KernelEntry* find_padded_kernel2d(int precision, int nx, int ny);
 hipError_t launch_sde_path(int precision, const PathArgs* A, int nblocks, hipStream_t s);
+// coupled SDE components (A->fields = fields, 2..XSH_MAX_FIELDS)
+hipError_t launch_sde_fields(int precision, int fields, const PathArgs* A, int nblocks, hipStream_t s);
 // time-domain spectra of the step fields sde_path stored (tw: exp(2 pi i k / M), k < M)
 hipError_t launch_sde_spectrum(const PathArgs* A, const double* tw, double fsc, int E, int npts, int npaths,
                                hipStream_t s);

@@ -98,6 +98,15 @@ struct PathArgs {
   double* av;              // [paths][avlen] complex double
   const double* a0;        // optional [E][ns] complex double initial fields
   double* a_out;           // optional [E][ns] complex double final fields
+  // coupled SDE components (fields > 1, ns = fields): da_i/dt = sum_j fm[i][j] a_j
+  // + fg[i] |a_i|^2 a_i + sum_k fs[i][k] w_k; propagator fp[nc][i]; constants fc0[i]
+  int fields;
+  double fm[XSH_MAX_FIELDS][XSH_MAX_FIELDS][2];
+  double fg[XSH_MAX_FIELDS][2];
+  double fs[XSH_MAX_FIELDS][4];
+  double fp[2][XSH_MAX_FIELDS][2];
+  double fc0[XSH_MAX_FIELDS][2];
+  int obs_field[XSH_MAX_OBS];
 };
 
 // ------------------------------------------------------------ drift forms

@@ -22,6 +22,7 @@ if numel(input) > 1
 end
 p = input{1};
 F = p.fields{1};
+if strcmp(p.gpu.deriv,'matrix'), F = 1; end      % coupled components: one descriptor
 nobs = numel(p.gpu.observe);
 of = ones(1,nobs);
 if isfield(p.gpu,'ofield'), of = p.gpu.ofield; end
@@ -81,8 +82,9 @@ for j = 1:p.ensembles(2)
   for nc = 1:p.nchk
     av = cell(1,nobs);
     for o = 1:nobs
-      k = of(o);
-      i = nnz(of(1:o) == k);                     % o's place among component k's observables
+      k = min(of(o), F);                           % coupled components: all in handle 1
+      i = o;
+      if F > 1, i = nnz(of(1:o) == k); end       % o's place among component k's observables
       col = avall{k}(:,(j-1)*p.nchk+nc);
       n = info{k}.points*info{k}.obs_sites(i);
       av{o} = reshape(col(info{k}.obs_offset(i)+(1:n)), p.d.av{o});

@@ -61,8 +61,21 @@ end
 d.boundaries = b;
 d.ranges = zeros(1,3);  d.ranges(1:nd) = p.ranges(1:nd);
 d.origins = zeros(1,3); d.origins(1:nd) = p.origins(1:nd);
-names = {'zero','kubo','gp'};
+names = {'zero','kubo','gp','matrix'};
 d.deriv = find(strcmp(names, g.deriv)) - 1;
+if strcmp(g.deriv,'matrix')                      % coupled SDE components: one descriptor
+  % da_i/dt = sum_j M(i,j) a_j + g(i) |a_i|^2 a_i + sum_k S(i,k) w_k
+  d.fields = F; d.inst = 0;
+  d.noises = field0(p,'noises'); d.inrandoms = field0(p,'inrandoms'); d.urandoms = field0(p,'urandoms');
+  d.fmat = complex(g.M);
+  d.fnl = complex(getdef(g,'g',zeros(1,F)) + zeros(1,F));
+  d.fnoise = getdef(g,'S',zeros(F,max(1,d.noises)));
+  d.fc0 = complex(getdef(g,'c0',ones(1,F)) + zeros(1,F));
+  d.fl0 = complex(zeros(1,F));
+  if isfield(g,'linear'), d.fl0 = complex(getdef(g.linear,'l0',0) + zeros(1,F)); end
+  mine = 1:numel(obs);
+  d.obs_field = of - 1;
+end
 dp = zeros(1,6);
 switch g.deriv
   case 'kubo', dp(1) = g.c;

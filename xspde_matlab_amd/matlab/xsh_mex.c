@@ -167,6 +167,31 @@ static void make_desc(const mxArray* s, xsh_desc* d) {
   memset(t, 0, sizeof t);
   vec(s, "tspec", t, XSH_MAX_OBS);
   for (int o = 0; o < XSH_MAX_OBS; ++o) d->tspec[o] = (int)t[o];
+  /* coupled SDE components (xgpudesc.m, matrix form): fields; fmat [F x F],
+     fnl, fl0, fc0 [F] complex; fnoise [F x noises] real; obs_field per observable */
+  d->fields = (int)num(s, "fields", 0);
+  if (d->fields > 1 && d->fields <= XSH_MAX_FIELDS) {
+    const int F = d->fields;
+    double m[2 * XSH_MAX_FIELDS * XSH_MAX_FIELDS] = {0};
+    cvec(s, "fmat", m, F * F); /* column-major: (i, j) at i + F j */
+    for (int i = 0; i < F; ++i)
+      for (int j = 0; j < F; ++j) {
+        d->fmat[i][j][0] = m[2 * (i + F * j)];
+        d->fmat[i][j][1] = m[2 * (i + F * j) + 1];
+      }
+    cvec(s, "fnl", &d->fnl[0][0], F);
+    cvec(s, "fl0", &d->fl0[0][0], F);
+    cvec(s, "fc0", &d->fc0[0][0], F);
+    double sn[4 * XSH_MAX_FIELDS] = {0};
+    vec(s, "fnoise", sn, 4 * F);
+    const mxArray* fn = field(s, "fnoise");
+    const int nw = fn ? (int)(mxGetNumberOfElements(fn) / F) : 0;
+    for (int i = 0; i < F; ++i)
+      for (int k = 0; k < nw && k < 4; ++k) d->fnoise[i][k] = sn[i + F * k];
+    memset(t, 0, sizeof t);
+    vec(s, "obs_field", t, XSH_MAX_OBS);
+    for (int o = 0; o < XSH_MAX_OBS; ++o) d->obs_field[o] = (int)t[o];
+  }
 }
 
 static mxArray* info_struct(const xsh_info* info) {

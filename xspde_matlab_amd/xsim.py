@@ -106,8 +106,13 @@ def preferences(p):
     return q
 
 
+def coupled(q):
+    """Several components coupled by the matrix drift form (one descriptor, SDEs)."""
+    return int(q.get("fields", 1)) > 1 and q.get("gpu", {}).get("deriv") == "matrix"
+
+
 def make_desc(q):
-    if q.get("fields", 1) != 1:
+    if q.get("fields", 1) != 1 and not coupled(q):
         raise GpuFormError("one descriptor integrates one field component (FieldsProblem splits a project)")
     g = q["gpu"]
     d = _lib.Desc()
@@ -143,6 +148,32 @@ def make_desc(q):
         d.deriv_par[0], d.deriv_par[1], d.deriv_par[2], d.deriv_par[3] = gg.real, gg.imag, u.real, u.imag
         d.deriv_par[4] = float(g.get("s", 0.0))
     lin = g.get("linear")
+    if kind == "matrix":
+        # coupled SDE components: da_i/dt = sum_j M_ij a_j + g_i |a_i|^2 a_i + sum_k S_ik w_k
+        F = int(q.get("fields", 1))
+        if F < 2 or F > _lib.XSH_MAX_FIELDS:
+            raise GpuFormError("the matrix drift form couples 2..%d components" % _lib.XSH_MAX_FIELDS)
+        d.fields = F
+        M = np.asarray(g.get("M", np.zeros((F, F))), complex).reshape(F, F)
+        gn = np.broadcast_to(np.asarray(g.get("g", 0), complex), (F,))
+        S = np.asarray(g.get("S", np.zeros((F, max(1, q["noises"])))), float).reshape(F, -1)
+        if S.shape[1] > 4 or q["noises"] > 4:
+            raise GpuFormError("the matrix drift form takes at most 4 noise components")
+        l0 = np.broadcast_to(np.asarray((lin or {}).get("l0", 0), complex), (F,))
+        c0 = np.broadcast_to(np.asarray(g.get("c0", 1.0), complex), (F,))
+        for i in range(F):
+            for j in range(F):
+                d.fmat[i][j][0], d.fmat[i][j][1] = M[i, j].real, M[i, j].imag
+            d.fnl[i][0], d.fnl[i][1] = gn[i].real, gn[i].imag
+            for k in range(S.shape[1]):
+                d.fnoise[i][k] = S[i, k]
+            d.fl0[i][0], d.fl0[i][1] = l0[i].real, l0[i].imag
+            d.fc0[i][0], d.fc0[i][1] = c0[i].real, c0[i].imag
+        d.has_linear = 1 if lin is not None else 0
+        of = [int(f) for f in (g.get("ofield") or [1] * len(g.get("observe", ["a"])))]
+        for o, f in enumerate(of[:_lib.XSH_MAX_OBS]):
+            d.obs_field[o] = f - 1
+        lin = None
     if lin is not None:
         d.has_linear = 1
         l0 = complex(lin.get("l0", 0))
@@ -157,7 +188,7 @@ def make_desc(q):
         raise GpuFormError("unknown initial form %r" % (ini,))
     d.initial = _lib.INITIAL[ini]
     if ini == "const":
-        c = complex(g.get("c0", 1.0))
+        c = complex(np.ravel(g.get("c0", 1.0))[0])
         d.init_par[0], d.init_par[1] = c.real, c.imag
     elif ini == "sech":
         d.init_par[0], d.init_par[1], d.init_par[2] = g.get("amp", 1.0), g.get("width", 1.0), g.get("eps", 0.0)
@@ -390,8 +421,9 @@ class Problem:
 
     def __init__(self, p):
         self.q = preferences(p)
-        if self.q["fields"] != 1:
-            raise GpuFormError("Problem integrates one field component; FieldsProblem several")
+        if self.q["fields"] != 1 and not coupled(self.q):
+            raise GpuFormError("Problem integrates one field component (or coupled SDE components); "
+                               "FieldsProblem several uncoupled ones")
         self.handle = _lib.Handle(make_desc(self.q))
         self.info = self.handle.info
         outs = self.q.get("output")
@@ -573,8 +605,11 @@ class FieldsProblem:
 
 
 def make_problem(p):
-    """Problem for one field component, FieldsProblem for several."""
-    return FieldsProblem(p) if int(p.get("fields", 1)) > 1 else Problem(p)
+    """Problem for one field component or coupled SDE components, FieldsProblem
+    for several uncoupled ones."""
+    if int(p.get("fields", 1)) > 1 and not coupled(p):
+        return FieldsProblem(p)
+    return Problem(p)
 
 
 def ensemble_moments(q, info, outputs, av_all, split=None):
