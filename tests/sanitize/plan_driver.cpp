@@ -152,6 +152,47 @@ int main() {
     std::memcpy(d.modes, modes, sizeof modes);
     std::snprintf(name, sizeof name, "%s 128x128 TW features", pn);
     expect(name, d, prec == XSH_FP32 ? XSH_OK : XSH_EUNSUPPORTED);
+    // coupled SDE components (the matrix drift form), 2..4 components
+    for (int F = 2; F <= XSH_MAX_FIELDS; ++F) {
+      d = base(prec, 1, 1, 1);
+      d.fields = F;
+      d.deriv = XSH_DERIV_MATRIX;
+      d.initial = XSH_INIT_CONST;
+      d.noises = F;
+      for (int i = 0; i < F; ++i) {
+        d.fmat[i][(i + 1) % F][0] = 1.0;
+        d.fmat[(i + 1) % F][i][0] = -1.0;
+        d.fnl[i][1] = 0.1;
+        d.fnoise[i][i % 4] = 0.2;
+        d.fl0[i][0] = -0.1;
+        d.fc0[i][0] = 1.0 + i;
+      }
+      d.nobserve = 3;
+      d.observe[0] = XSH_OBS_A;
+      d.observe[1] = XSH_OBS_ABS2_AVE;
+      d.observe[2] = XSH_OBS_ABS4_AVE;
+      d.obs_field[0] = F - 1;
+      d.obs_field[1] = 0;
+      d.obs_field[2] = 1;
+      std::snprintf(name, sizeof name, "%s SDE %d coupled components", pn, F);
+      expect(name, d, XSH_OK);
+    }
+    d = base(prec, 1, 1, 1);
+    d.fields = 2;
+    d.deriv = XSH_DERIV_MATRIX;
+    d.initial = XSH_INIT_CONST;
+    d.obs_field[0] = 2;
+    expect("coupled: observable of a missing component", d, XSH_EINVAL);
+    d.obs_field[0] = 0;
+    d.observe[0] = XSH_OBS_CORR;
+    expect("coupled: lattice observable", d, XSH_EUNSUPPORTED);
+    d = base(prec, 3, 64, 64);
+    d.fields = 2;
+    d.deriv = XSH_DERIV_MATRIX;
+    expect("coupled lattice components", d, XSH_EUNSUPPORTED);
+    d = base(prec, 1, 1, 1);
+    d.deriv = XSH_DERIV_MATRIX;
+    expect("matrix form with one component", d, XSH_EINVAL);
     // refusals
     d = base(prec, 3, 64, 64);
     d.tspec[0] = 1;

@@ -3,7 +3,7 @@ UndefinedBehaviorSanitizer (CPU, no device): tests/sanitize/plan_driver.cpp
 runs xsh_plan - descriptor validation, lattice derivation, kernel choice and
 every host-built table (propagators, twiddles, dense transform matrices and
 their matrix-core operand copies, boundary patches and boundfun tables, k-space
-filters, spectrum flags) - over 162 descriptors of every lattice family and
+filters, spectrum flags) - over 176 descriptors of every lattice family and
 refusal path.  Any sanitizer finding aborts the driver."""
 import os
 import subprocess
@@ -37,7 +37,7 @@ def test_host_code_under_sanitizers():
     r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=300)
     tail = "\n".join((r.stdout + r.stderr).splitlines()[-15:])
     assert r.returncode == 0, tail
-    assert "162 cases, 0 failures" in r.stdout, tail
+    assert "176 cases, 0 failures" in r.stdout, tail
     assert "ERROR: AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr, tail
 
 
