@@ -134,3 +134,54 @@ def test_oracle_sho_closed_form():
     assert err[0] < 1e-4                                # MP, dt = 0.025, order 2: 5.6e-6
     bad = dict(p, gpu=dict(p["gpu"], M=[[0, 1], [-1, 0]]))
     assert X.xsim(forms.apply_forms(bad))[0][0] > 1e-2
+
+
+class _FakeHandle:
+    """Stands in for a GPU handle: fixed per-path averages, no device."""
+
+    def __init__(self, av_all):
+        self.av_all = av_all
+
+    def run_ensemble(self, npar):
+        pass
+
+    def synchronize(self):
+        return 0.0
+
+    def nonfinite(self):
+        return False
+
+    def read_averages(self):
+        return self.av_all
+
+
+def test_fields_problem_assembles_observables_in_project_order():
+    """FieldsProblem.xensemble on fake per-component handles: observable o is
+    read from its component's handle at its place there, and the D, D^2 sums
+    equal those of ensemble_moments on the same averages taken directly."""
+    import importlib
+    import types
+    XS = importlib.import_module("xspde_matlab_amd.xsim")
+    p = _four_fields()
+    q = preferences(p)
+    subs, where = field_problems(p, q)
+    rng = np.random.default_rng(3)
+    e2, nchk, npts = q["ensembles"][1], 1 + q["checks"][0], q["points"][0]
+    parts = []
+    for sk in subs:
+        qk = preferences(sk)
+        nobs = len(sk["gpu"]["observe"])
+        ns = [161 if o in ("a", "abs2") else 1 for o in sk["gpu"]["observe"]]
+        offs = np.concatenate([[0], np.cumsum([npts * n for n in ns])])
+        info = types.SimpleNamespace(obs_offset=list(offs[:-1]), obs_sites=ns, avlen=int(offs[-1]))
+        av_all = rng.standard_normal((e2, nchk, info.avlen)) + 1j * rng.standard_normal((e2, nchk, info.avlen))
+        parts.append(types.SimpleNamespace(q=qk, info=info, handle=_FakeHandle(av_all)))
+    fp = XS.FieldsProblem.__new__(XS.FieldsProblem)
+    fp.q, fp.where, fp.parts, fp.info = q, where, parts, parts[0].info
+    fp.outputs = [lambda av, p, _n=n: av[_n] for n in range(len(where))]
+    data = fp.xensemble(1)
+    assert len(data) == 3
+    for o, (k, i) in enumerate(where):
+        part = parts[k]
+        direct = XS.ensemble_moments(part.q, part.info, [lambda av, p, _i=i: av[_i]], part.handle.av_all)
+        assert np.allclose(data[o], direct[0])
