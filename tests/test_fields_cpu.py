@@ -24,7 +24,7 @@ def test_rows_values_noises_and_observables_per_field():
     subs, where = field_problems(p, preferences(p))
     assert [s["boundaries"][0] for s in subs] == [[1, 1], [-1, -1], [1, -1], [-1, 1]]
     assert [s["boundval"][0] for s in subs] == [[1.0, 2.0], [0.5, -0.5], [0.0, 1.0], [2.0, 0.0]]
-    assert [s["noises"] for s in subs] == [1, 1, 1, 1]
+    assert [s["noises"] for s in subs] == [0, 0, 0, 1]      # only field 4's form reads noise (s = 0.1)
     assert [s["gpu"]["inst"] for s in subs] == [0, 1, 2, 3]
     assert subs[1]["gpu"]["g"] == 2j and subs[0]["gpu"]["g"] == 1j and subs[3]["gpu"]["s"] == 0.1
     # observables 1 and 3 on field 2, observable 2 on field 4; fields 1 and 3 carry a placeholder
@@ -55,7 +55,13 @@ def test_per_field_boundary_function_is_tabulated_per_field():
 def test_refusals():
     with pytest.raises(GpuFormError, match="noises"):
         p = _four_fields(noises=2)
+        p["gpu"] = dict(p["gpu"], s=0.1)                 # a noise term that reads the cells
         field_problems(p, preferences(p))
+    # cells no form reads may be any number (Peregrine.m: noises = 1, four fields)
+    p = _four_fields(noises=1)
+    p["gpu"] = dict(p["gpu"], fields=[{}, dict(g=2j), {}, {}])
+    subs, _ = field_problems(p, preferences(p))
+    assert [s["noises"] for s in subs] == [0, 0, 0, 0]
     with pytest.raises(GpuFormError, match="ofield"):
         p = _four_fields()
         p["gpu"] = dict(p["gpu"], ofield=[1, 5, 1])

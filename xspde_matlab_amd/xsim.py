@@ -501,19 +501,25 @@ def field_problems(p, q):
     bval = list(q.get("boundval") or [])
     brows = [_field_rows(bnd[d] if d < len(bnd) else None, F) for d in range(nd - 1)]
     vrows = [_field_rows(bval[d] if d < len(bval) else None, F) for d in range(nd - 1)]
-    for name in ("noises", "inrandoms", "urandoms", "knoises", "krandoms"):
-        if q[name] % F:
-            raise GpuFormError("%s = %d is not a whole number of components per field (%d fields)"
-                               % (name, q[name], F))
     bf = q.get("boundfun")
     subs, omap = [], []
     for k in range(F):
         sk = dict(p)
         sk["fields"] = 1
-        for name in ("noises", "inrandoms", "urandoms", "knoises", "krandoms"):
-            sk[name] = q[name] // F
         gk = dict(g)
         gk.update(per[k])
+        # each component's share of the noise and random cells; cells its forms do
+        # not read may be any number (Peregrine.m declares noises = 1 for 4 fields)
+        used = {"noises": gk.get("deriv") == "kubo" or (gk.get("deriv") == "gp" and gk.get("s", 0) != 0),
+                "knoises": gk.get("deriv") == "kubo" or (gk.get("deriv") == "gp" and gk.get("s", 0) != 0),
+                "inrandoms": gk.get("initial") == "coherent" or (gk.get("initial") == "sech" and gk.get("eps", 0)),
+                "krandoms": gk.get("initial") == "coherent" or (gk.get("initial") == "sech" and gk.get("eps", 0)),
+                "urandoms": gk.get("initial") == "phase"}
+        for name in ("noises", "inrandoms", "urandoms", "knoises", "krandoms"):
+            if q[name] % F and used[name]:
+                raise GpuFormError("%s = %d is not a whole number of components per field (%d fields)"
+                                   % (name, q[name], F))
+            sk[name] = q[name] // F if used[name] else 0
         mine = [o for o in range(len(obs)) if ofield[o] == k + 1]
         gk["observe"] = [obs[o] for o in mine] or ["a_ave"]
         gk["inst"] = int(g.get("inst", 0)) * F + k
