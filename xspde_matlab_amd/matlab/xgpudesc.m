@@ -85,7 +85,7 @@ d.deriv_par = dp;
 d.has_linear = double(isfield(g,'linear'));
 if d.has_linear
   L = g.linear;
-  d.lin_l0 = complex(getdef(L,'l0',0));
+  l0 = getdef(L,'l0',0); d.lin_l0 = complex(l0(1));  % (coupled components: fl0 per component)
   d.lin_l1 = complex(two(getdef(L,'l1',0)));
   d.lin_l2 = complex(two(getdef(L,'l2',0)));
 end
@@ -93,7 +93,7 @@ inames = {'const','sech','gauss','phase','coherent'};
 d.initial = find(strcmp(inames, g.initial)) - 1;
 ip = zeros(1,6);
 switch g.initial
-  case 'const',    c = getdef(g,'c0',1); ip(1:2) = [real(c), imag(c)];
+  case 'const',    c = getdef(g,'c0',1); ip(1:2) = [real(c(1)), imag(c(1))];
   case 'sech',     ip(1:4) = [getdef(g,'amp',1), getdef(g,'width',1), getdef(g,'eps',0), getdef(g,'center',0)];
   case 'gauss',    ip(1:3) = [getdef(g,'amp',1), getdef(g,'width',1), getdef(g,'width_y',0)];
   case 'phase',    ip(1) = getdef(g,'amp',1);
