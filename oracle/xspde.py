@@ -46,11 +46,13 @@ xensemble             Simulator/xensemble.m:12-108                  Batchtest #2
 xsim statistics       Simulator/xsim.m:64-188                       Batchtest rows,
                                                                     synthetic sigma test
 xsim sequences        xsim.m:33-38, xensemble.m:59-63 (transfer),   Batchtest #33
-                      xpreferences.m:27-28,197                      (Boundaries2)
+                      xpreferences.m:27-28,197                      (Boundaries2), #20
+                                                                    (SolitonDerivS)
 xcompress             Applications/SPDE/xcompress.m:18-69           Batchtest #16
 Ave / Int / D1        Applications/SPDE/Ave.m, Int.m, D1.m          Batchtest #14, #16, #21
-D2                    Applications/SPDE/D2.m:38-78                  Batchtest #33, quadratic
-                                                                    with ghost-point ends
+D2                    Applications/SPDE/D2.m:38-78                  Batchtest #33, #20,
+                                                                    quadratic with
+                                                                    ghost-point ends
 xnft                  Applications/SPDE/xnft.m                      Batchtest #16
 ====================  ===========================================  =======================
 
@@ -103,6 +105,10 @@ def xpreferences(p_in):
     p.dimensions = nd
     p.fields = int(p.get("fields", 1))
     noises = p.get("noises", None)
+    if isinstance(noises, (list, tuple)):                 # the older [x-noises, k-noises] vector
+        if len(noises) > 1 and p.get("knoises", None) is None:
+            p["knoises"] = noises[1]
+        noises = noises[0] if noises else None
     p.knoises = int(p.get("knoises", 0) or 0)
     p.krandoms = int(p.get("krandoms", 0) or 0)
     if noises is None and p.get("knoises", None) is None and p.get("unoises", None) is None:

@@ -267,3 +267,18 @@ def test_d2_matches_second_difference_with_ghost_points():
     assert np.allclose(d2[0, :, 0], 2.0, atol=1e-10)
     assert np.allclose(d2[1, 1:-1, 0], 2.0, atol=1e-10)
     assert d2[1, 0, 0] == d2[1, 1, 0] and d2[1, -1, 0] == d2[1, -2, 0]
+
+
+def test_batchtest_solitonderivs_sequence():
+    """Examples/SPDE_2D/SolitonDerivS.m (Batchtest #20, 8.570537e-08): NLS with
+    the finite-difference Laplacian D2 in the drift and no linear term, the
+    observables |a|^2 and Int |D1 a|^2, order-4 extrapolation, then a second
+    problem with doubled nonlinearity continuing from the first one's final
+    field and end time (default transfer and time origin of a sequence)."""
+    p = dict(dimensions=2, order=4, points=[101, 51], steps=10, initial=lambda v, p: sech(p.x),
+             deriv=lambda a, w, p: 1j * a * (np.conj(a) * a) + 0.5j * (X.D2(a, 2, p) - a),
+             observe=[lambda a, p: a * np.conj(a), lambda a, p: X.Int(np.abs(X.D1(a, 2, p)) ** 2, p)])
+    p1 = dict(p, deriv=lambda a, w, p: 2j * a * (np.conj(a) * a) + 0.5j * (X.D2(a, 2, p) - a))
+    err = X.xsim([p, p1])[0]
+    print("SolitonDerivS error %.7e (Batchtest 8.570537e-08)" % err[0])
+    assert 8.570537e-08 / 1.001 < err[0] < 8.570537e-08 * 1.001
