@@ -109,14 +109,15 @@ def xpreferences(p_in):
         if len(noises) > 1 and p.get("knoises", None) is None:
             p["knoises"] = noises[1]
         noises = noises[0] if noises else None
+    # which cells the project names, before the defaults below fill them in
+    given = {k: p.get(k, None) is not None for k in ("knoises", "unoises", "inrandoms", "urandoms", "krandoms")}
     p.knoises = int(p.get("knoises", 0) or 0)
     p.krandoms = int(p.get("krandoms", 0) or 0)
-    if noises is None and p.get("knoises", None) is None and p.get("unoises", None) is None:
+    if noises is None and not given["knoises"] and not given["unoises"]:
         p.noises = p.fields                                             # xpreferences.m:65-70
     else:
         p.noises = int(noises or 0)
-    if p.get("inrandoms", None) is None and p.get("urandoms", None) is None and \
-            p.get("krandoms", None) is None:
+    if not given["inrandoms"] and not given["urandoms"] and not given["krandoms"]:
         p.inrandoms = p.noises                                          # xpreferences.m:71-76
     else:
         p.inrandoms = int(p.get("inrandoms", 0) or 0)
