@@ -160,3 +160,17 @@ def test_split_averages_time_fastest():
     av = split_averages(q, info, flat)
     assert np.array_equal(av[0], a0) and np.array_equal(av[1], a1)
     assert flat[1] == a0[0, 1, 0]           # time fastest
+
+
+def test_default_noise_and_random_counts_match_the_oracle():
+    """xpreferences.m:65-76: noises default to the field count only when no
+    noise cell is named (an explicit 0 k-noises names one), inrandoms to
+    noises only when no random cell is named; the host and the oracle agree."""
+    from oracle import xspde as X
+    base = dict(dimensions=2, points=[3, 32], gpu=dict(deriv="zero", observe=["a"]))
+    for extra in ({}, dict(knoises=0), dict(knoises=2), dict(urandoms=1), dict(krandoms=0), dict(noises=2),
+                  dict(fields=2), dict(noises=2, inrandoms=1)):
+        p = dict(base, **extra)
+        q = preferences(p)
+        o = X.xpreferences(dict(p, gpu=None))
+        assert (q["noises"], q["inrandoms"]) == (o.noises, o.inrandoms), extra

@@ -64,12 +64,14 @@ def preferences(p):
     if fields < 1:
         raise GpuFormError("fields must be at least 1")
     q["fields"] = fields
+    # cells the project names (an empty one counts as not named, as [] in MATLAB)
+    given = {k: q.get(k) is not None for k in ("knoises", "unoises", "urandoms", "krandoms")}
     q["knoises"] = int(q.get("knoises") or 0)
     q["krandoms"] = int(q.get("krandoms") or 0)
-    if q.get("noises") is None and not q["knoises"]:
+    if q.get("noises") is None and not given["knoises"] and not given["unoises"]:
         q["noises"] = fields                                         # xpreferences.m:65-70
     q["noises"] = int(q.get("noises") or 0)
-    if q.get("inrandoms") is None and q.get("urandoms") is None and not q["krandoms"]:
+    if q.get("inrandoms") is None and not given["urandoms"] and not given["krandoms"]:
         q["inrandoms"] = q["noises"]                                 # xpreferences.m:71-76
     q["inrandoms"] = int(q.get("inrandoms") or 0)
     q["urandoms"] = int(q.get("urandoms") or 0)
