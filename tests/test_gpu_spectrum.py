@@ -6,7 +6,6 @@ Checked against the closed-form geometric series of a linear SDE (exact
 interaction-picture steps) and against the oracle on a seeded
 Ornstein-Uhlenbeck ensemble (Examples/SDE_1D/Equilibrium.m:12-28), fine and
 coarse runs, FP64 within 1e-10 and FP32 within 2e-5 relative."""
-import math
 
 import numpy as np
 import pytest
