@@ -656,6 +656,12 @@ def statistics(q, data):
     """Step and sampling errors of xsim.m:94-108 and the RMS summary of :144-188
     (no comparisons): returns (error vector, per-output [ndata, 3] arrays of
     mean, step error, sampling error)."""
+    er, outd = statistics_rows(q, data)
+    return error_vector(q, er), outd
+
+
+def statistics_rows(q, data):
+    """Per-output RMS rows of xsim.m:94-143 and the [ndata, 3] arrays."""
     ne = q["ensembles"][1] * q["ensembles"][2]
     nchk = 1 + q["checks"][0]
     sx = max(3, 1 + nchk)
@@ -683,12 +689,119 @@ def statistics(q, data):
         res[:, 1] = np.sqrt(np.sum(d[:, 1:nchk] ** 2, axis=1)) if nchk > 1 else 0.0
         res[:, 2] = d[:, sx - 1]
         outd.append(res)
+    return er, outd
+
+
+def error_vector(q, er):
+    """The error vector of xsim.m:174-188 from the RMS rows of every output (of
+    every problem of a sequence)."""
+    nchk = 1 + q["checks"][0]
+    sx = max(3, 1 + nchk)
+    tol = float(q.get("tol", 1e-20))
     error = np.zeros(max(6, sx + 3))
     for c in range(2, sx + 2):
         pts = 1e-99 + np.sum(er[:, c - 1] > tol)
         error[c - 1] = math.sqrt(np.sum(er[:, c - 1] ** 2) / pts)
     error[0] = math.sqrt(np.sum(error[1:sx + 1] ** 2) / (1e-99 + np.sum(error[1:sx + 1] > tol)))
-    return error, outd
+    return error
+
+
+def sequence_inputs(plist):
+    """The problems of a sequence as xpreferences.m:11-13,27-28,197 prepares
+    them: checks from the first, each default time origin where the previous
+    problem ended, the random stream of problem s keyed by s (gpu.seq), the
+    ensembles of the first for every problem (xensemble.m:25-30 loops over
+    them).  Returns the problems as handed to Problem and their preferences."""
+    ps, qs = [], []
+    origin = 0.0
+    for s, pin in enumerate(plist):
+        pk = dict(pin)
+        pk["gpu"] = dict(pk.get("gpu") or {}, seq=s)
+        if s > 0:
+            pk["checks"] = list(qs[0]["checks"])
+            if pk.get("ensembles") is not None and \
+                    [int(e) for e in _vec(pk["ensembles"], 3, 1)] != qs[0]["ensembles"]:
+                raise GpuFormError("the problems of a sequence share the ensembles of the first")
+            pk["ensembles"] = list(qs[0]["ensembles"])
+        if pk.get("origins") is None:
+            q0 = preferences(pk)
+            pk["origins"] = [origin] + list(q0["origins"][1:])
+        q = preferences(pk)
+        origin = q["origins"][0] + q["ranges"][0]
+        ps.append(pk)
+        qs.append(q)
+    return ps, qs
+
+
+def _to_cells(q, b):
+    """A path's final fields [E, sites] (x fastest) as the reference's field
+    cell [1, nx, (ny), E] (p.d.ca), for p.transfer."""
+    E = q["ensembles"][0]
+    dims = space_dims(q)
+    b = np.asarray(b).reshape(E, -1)
+    if not dims:
+        return b.T[None]
+    x = _unflatten_sites(b, 1, dims)                     # [E, nx, (ny)]
+    return np.moveaxis(x, 0, -1)[None]
+
+
+def _from_cells(q, a):
+    E = q["ensembles"][0]
+    dims = space_dims(q)
+    shape = (1,) + tuple(dims) + (E,)
+    a = np.broadcast_to(np.asarray(a, dtype=complex), shape)[0]
+    a = np.moveaxis(a, -1, 0)                              # [E, nx, (ny)]
+    if len(dims) == 2:
+        a = np.transpose(a, (0, 2, 1))                     # [E, ny, nx]: x fastest
+    return np.ascontiguousarray(a).reshape(E, -1)
+
+
+def xsim_sequence(plist):
+    """xsim(p1, p2, ...) (xsim.m:33-38, xensemble.m:25-108): per parallel
+    ensemble, sub-ensemble j and check nc the problems run in order on the GPU
+    path, each after the first starting from p.transfer(b, v, p) of the
+    previous problem's final fields b (default: b itself; v is not available on
+    the host and is passed as None); the moments of every problem as
+    xensemble accumulates them, and one error vector over the outputs of all
+    problems.  Returns (error, data per problem, preferences per problem)."""
+    t0 = time.time()
+    ps, qs = sequence_inputs(plist)
+    if any(q["fields"] != 1 for q in qs):
+        raise GpuFormError("sequences of projects with several field components are not on the GPU path")
+    probs = []
+    try:
+        for pk in ps:
+            probs.append(make_problem(pk))
+        q1 = qs[0]
+        e2, e3, nchk = q1["ensembles"][1], q1["ensembles"][2], 1 + q1["checks"][0]
+        data = [None] * len(ps)
+        for npar in range(1, e3 + 1):
+            avs = [dict() for _ in ps]
+            for j in range(1, e2 + 1):
+                for nc in range(1, nchk + 1):
+                    b = None
+                    for s, (pk, q, prob) in enumerate(zip(ps, qs, probs)):
+                        a0 = None
+                        if s > 0:
+                            tr = pk.get("transfer")
+                            prev = _to_cells(qs[s - 1], b)
+                            a0 = _from_cells(q, tr(prev, None, Params(q)) if callable(tr) else prev)
+                        av, b = prob.xpath(nc, npar, j, a0=a0, want_final=True)
+                        avs[s][(j - 1, nc - 1)] = av
+            for s, (q, prob) in enumerate(zip(qs, probs)):
+                dp = ensemble_moments(q, None, prob.outputs, None, split=lambda j, nc, _s=s: avs[_s][(j, nc)])
+                data[s] = dp if data[s] is None else [x + y for x, y in zip(data[s], dp)]
+        rows, outd = [], []
+        for q, d in zip(qs, data):
+            er, od = statistics_rows(q, d)
+            rows.append(er)
+            outd.append(od)
+        error = error_vector(q1, np.concatenate(rows, axis=0))
+        error[-1] = time.time() - t0
+    finally:
+        for prob in probs:
+            prob.close()
+    return error, outd, qs
 
 
 def xsim(p):
@@ -696,7 +809,12 @@ def xsim(p):
     comparisons).  data[o] is [ndata, 3] (mean, step error, sampling error);
     raw is raw[nc][j] with j over all ensembles(2) x ensembles(3) paths
     (xsim.m:42,56-57), or None without p.rawdata.  A non-blank p.file saves
-    error, data, input and raw to that '.mat' file (xsim.m:207-210)."""
+    error, data, input and raw to that '.mat' file (xsim.m:207-210).  A list of
+    problems is a sequence (xsim_sequence): data and input are then per problem
+    and raw is None."""
+    if isinstance(p, (list, tuple)):
+        error, outd, qs = xsim_sequence(p)
+        return error, outd, qs, None
     t0 = time.time()
     prob = make_problem(p)
     q = prob.q
