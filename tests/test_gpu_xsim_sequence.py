@@ -37,3 +37,18 @@ def test_sequence_on_the_gpu_matches_the_oracle():
             d, r = data[s][o], data_o[s][o].reshape(data[s][o].shape[0], -1)
             assert np.allclose(d[:, 0], r[:, 0], rtol=1e-9, atol=1e-12), (s, o)
             assert np.allclose(d[:, 2], r[:, 2], rtol=1e-6, atol=1e-12), (s, o)
+
+
+def test_host_xsim_on_the_gpu_reproduces_soliton():
+    """Examples/SPDE_2D/Soliton.m (Batchtest #4, 5.047219e-03 at 0.1 %) through
+    the host's own xsim end to end: the built-in NLS forms on the GPU (35
+    periodic sites on the padded FP64 DFT kernel), the host's moments and its
+    statistics with the comparison sech(x)."""
+    sech = lambda x: 1 / np.cosh(x)
+    p = dict(name="Soliton", dimensions=2, checks=0, order=2, noises=0, inrandoms=0,
+             compare=[lambda p: sech(p.x)],
+             gpu=dict(deriv="gp", g=1j, s=0.0, linear=dict(l0=-0.5j, l2=0.5j), initial="sech", amp=1.0,
+                      width=1.0, eps=0.0, observe=["a"], precision="fp64"))
+    err = xsim(p)[0]
+    print("Soliton via the host xsim on the GPU: %.7e (Batchtest 5.047219e-03)" % err[0], flush=True)
+    assert 5.047219e-03 / 1.001 < err[0] < 5.047219e-03 * 1.001

@@ -174,3 +174,38 @@ def test_default_noise_and_random_counts_match_the_oracle():
         q = preferences(p)
         o = X.xpreferences(dict(p, gpu=None))
         assert (q["noises"], q["inrandoms"]) == (o.noises, o.inrandoms), extra
+
+
+@pytest.mark.parametrize("e2,e3,checks,order", [(3, 2, 1, 0), (4, 1, 1, 2), (5, 1, 0, 0)])
+def test_host_comparisons_match_oracle(e2, e3, checks, order):
+    """p.compare (xsim.m:110-143): the difference error, the chi-squared sum
+    and the normalisation by the comparison's maximum, for a per-site output
+    and a lattice-mean output, equal the oracle's xsim_stats on the same data."""
+    cmp = [lambda p: 1.0 + 0.3 * np.cos(p.t) * np.exp(-p.x ** 2), lambda p: 2.0 + p.t]
+    p = dict(_project(e2, e3, checks, order), compare=cmp)
+    q = preferences(p)
+    nx = q["points"][1]
+    ns_list = [nx, 1]
+    info = _info(q, ns_list)
+    outs = [lambda av, p: av[0], OUTPUTS[1]]
+    nchk = 1 + checks
+    flat, arrays = _synthetic_averages(q, ns_list, e2, nchk, seed=7 + e2)
+    data = None
+    for npar in range(e3):
+        dp = ensemble_moments(q, info, outs, flat[npar])
+        data = dp if data is None else [a + b for a, b in zip(data, dp)]
+    zero = split_averages(q, info, np.zeros(info.avlen, dtype=complex))
+    shapes = [np.asarray(out(zero, q)).shape for out in outs]
+    err_h, _ = statistics(q, data, shapes)
+    po = X.xpreferences(dict(p, observe=[lambda a, p: a[0:1], lambda a, p: np.mean(a[0:1], axis=1)],
+                             output=[None, OUTPUTS[1]]))
+    dref = None
+    for npar in range(1, e3 + 1):
+        dp = X.xensemble(npar, po, path_fn=lambda p_, j, nc, n_: arrays[n_ - 1][(j, nc)])
+        dref = dp if dref is None else [a + b for a, b in zip(dref, dp)]
+    po._avshape = [(1, q["points"][0], nx), (1, q["points"][0])]
+    err_r, _ = X.xsim_stats(po, dref)
+    sx = max(3, 1 + nchk)
+    print("host %s oracle %s" % (np.array2string(err_h[:sx + 2], precision=6), np.array2string(err_r[:sx + 2], precision=6)))
+    assert err_h[sx] > 0                                   # the difference error is there
+    assert np.allclose(err_h[:sx + 2], err_r[:sx + 2], rtol=1e-12, atol=1e-300)

@@ -29,6 +29,16 @@ class _OracleProblem:
         a, av = X.run_path(self.qo, j, nc, npar, seq=self.seq, a0=a0o)
         return av, XS._from_cells(self.q, a).reshape(-1)
 
+    def xensemble(self, npar, want_raw=False):
+        q = self.q
+        avs = {(j, nc): self.xpath(nc + 1, npar, j + 1)[0]
+               for j in range(q["ensembles"][1]) for nc in range(1 + q["checks"][0])}
+        return XS.ensemble_moments(q, None, self.outputs, None, split=lambda j, nc: avs[(j, nc)])
+
+    def zero_av(self):
+        av, _ = self.xpath(1)
+        return [np.zeros_like(x) for x in av]
+
     def close(self):
         pass
 
@@ -75,3 +85,21 @@ def test_sequence_default_transfer_continues_the_field(monkeypatch):
     a, av = X.run_path(qo, 1, 1)
     second = data[1][0][:, 0].reshape(5, 16)                   # [points, x] of the second problem
     assert np.allclose(second, np.real(av[0][0, 4:, :]), rtol=1e-12, atol=1e-14)
+
+
+def _soliton():
+    """Examples/SPDE_2D/Soliton.m (Batchtest #4) in built-in forms, with its compare."""
+    sech = lambda x: 1 / np.cosh(x)
+    return dict(name="Soliton", dimensions=2, checks=0, order=2, noises=0, inrandoms=0,
+                compare=[lambda p: sech(p.x)],
+                gpu=dict(deriv="gp", g=1j, s=0.0, linear=dict(l0=-0.5j, l2=0.5j), initial="sech", amp=1.0,
+                         width=1.0, eps=0.0, observe=["a"], precision="fp64"))
+
+
+def test_host_xsim_reproduces_soliton_known_answer(monkeypatch):
+    """The host's own xsim statistics with the comparison (xsim.m:110-188) on
+    the oracle's paths give Batchtest #4's error, 5.047219e-03 within 0.1 %."""
+    monkeypatch.setattr(XS, "Problem", _OracleProblem)
+    err = XS.xsim(_soliton())[0]
+    print("Soliton via the host xsim: %.7e" % err[0])
+    assert 5.047219e-03 / 1.001 < err[0] < 5.047219e-03 * 1.001

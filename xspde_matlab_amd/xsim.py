@@ -652,20 +652,57 @@ def ensemble_moments(q, info, outputs, av_all, split=None):
     return data
 
 
-def statistics(q, data):
-    """Step and sampling errors of xsim.m:94-108 and the RMS summary of :144-188
-    (no comparisons): returns (error vector, per-output [ndata, 3] arrays of
-    mean, step error, sampling error)."""
-    er, outd = statistics_rows(q, data)
+def statistics(q, data, shapes=None):
+    """Step and sampling errors of xsim.m:94-108, the comparisons with p.compare
+    of :110-143 (when the outputs' shapes are given) and the RMS summary of
+    :144-188: returns (error vector, per-output [ndata, 3] arrays of mean, step
+    error, sampling error)."""
+    er, outd = statistics_rows(q, data, shapes)
     return error_vector(q, er), outd
 
 
-def statistics_rows(q, data):
-    """Per-output RMS rows of xsim.m:94-143 and the [ndata, 3] arrays."""
+def compare_grid(q):
+    """p.t, p.x, p.y of xgrid.m:49-56 for a compare function: the time points
+    along axis 1 and the lattice coordinates along axes 2 and 3 (x space)."""
+    nd = q["dimensions"]
+    pp = Params(q)
+    npts = q["points"][0]
+    sh = [1] * (nd + 1)
+    sh[1] = npts
+    pp["t"] = (q["origins"][0] + np.arange(npts) * q["dt"]).reshape(sh)
+    for k, name in enumerate(("x", "y")[:nd - 1]):
+        n = q["points"][k + 1]
+        s2 = [1] * (nd + 1)
+        s2[k + 2] = n
+        pp[name] = (q["origins"][k + 1] + np.arange(n) * (q["ranges"][k + 1] / max(1, n - 1))).reshape(s2)
+    return pp
+
+
+def _compare_values(q, fn, shape):
+    """p.compare{o}(p) at the output's data points, in the order of its rows
+    (the output's [lines, points, sites] flattened, sites x fastest)."""
+    dims = space_dims(q)
+    ns = int(np.prod(dims)) if dims else 1
+    c = np.asarray(fn(compare_grid(q)), dtype=float)
+    lines, npts = shape[0], shape[1]
+    if len(shape) == 3 and shape[2] == ns and ns > 1:       # per-site output
+        c = np.broadcast_to(c, (lines, npts) + tuple(dims))
+        if len(dims) == 2:
+            c = np.transpose(c, (0, 1, 3, 2))               # sites: x fastest
+        return c.reshape(-1)
+    c = np.broadcast_to(c, (lines, npts) + (1,) * len(dims))
+    return c.reshape(-1)
+
+
+def statistics_rows(q, data, shapes=None):
+    """Per-output RMS rows of xsim.m:94-143 (with the comparisons when shapes is
+    given and the project has p.compare) and the [ndata, 3] arrays."""
     ne = q["ensembles"][1] * q["ensembles"][2]
     nchk = 1 + q["checks"][0]
     sx = max(3, 1 + nchk)
     tol = float(q.get("tol", 1e-20))
+    cutoff = float(q.get("cutoff", -1e20))
+    compare = list(q.get("compare") or []) if shapes is not None else []
     er = np.zeros((len(data), max(6, nchk + 4)))
     outd = []
     for o, d in enumerate(data):
@@ -679,7 +716,22 @@ def statistics_rows(q, data):
             d[:, sx - 1] = 0
         for nc in range(2, nchk + 1):
             d[:, nc - 1] = np.abs(d[:, nc - 1] - d[:, 0])
-            er[o, nc - 1] = np.mean(d[:, nc - 1] ** 2) / mx
+            er[o, nc - 1] = np.mean(d[:, nc - 1] ** 2)
+        fn = compare[o] if o < len(compare) else None
+        if fn is not None:                                  # xsim.m:110-143
+            esum = np.sum(d[:, 1:sx] ** 2, axis=1) + 1e-20
+            ept = (esum > tol) & (np.abs(d[:, 0]) > cutoff)
+            dif0 = _compare_values(q, fn, shapes[o])
+            ept = ept & (np.abs(dif0) > cutoff)
+            mx = np.max(np.abs(dif0)) ** 2
+            if not (q.get("relerr", 1) and mx > tol):
+                mx = 1.0
+            dif = (d[:, 0] - dif0) ** 2
+            er[o, sx] = np.mean(dif) / mx
+            if ne > 1:
+                er[o, sx + 1] = np.sum(ept * (dif / esum))
+            er[o, sx + 2] = np.sum(ept)
+        er[o, 1:nchk] = er[o, 1:nchk] / mx
         er[o, sx - 1] = np.mean(d[:, sx - 1] ** 2) / mx
         seg = er[o, 1:sx + 1]
         er[o, 0] = np.sum(seg) / (1e-99 + np.sum(seg > 0))
@@ -703,7 +755,15 @@ def error_vector(q, er):
         pts = 1e-99 + np.sum(er[:, c - 1] > tol)
         error[c - 1] = math.sqrt(np.sum(er[:, c - 1] ** 2) / pts)
     error[0] = math.sqrt(np.sum(error[1:sx + 1] ** 2) / (1e-99 + np.sum(error[1:sx + 1] > tol)))
+    error[sx + 1:sx + 3] = np.sum(er[:, sx + 1:sx + 3], axis=0)   # chi-squared per point, points
+    error[sx + 1] = error[sx + 1] / (1e-20 + error[sx + 2])
     return error
+
+
+def output_shapes(prob):
+    """The shape of every output of a problem ([lines, points, (sites)])."""
+    zero = prob.zero_av()
+    return [np.asarray(out(zero, prob.q)).shape for out in prob.outputs]
 
 
 def sequence_inputs(plist):
@@ -792,8 +852,8 @@ def xsim_sequence(plist):
                 dp = ensemble_moments(q, None, prob.outputs, None, split=lambda j, nc, _s=s: avs[_s][(j, nc)])
                 data[s] = dp if data[s] is None else [x + y for x, y in zip(data[s], dp)]
         rows, outd = [], []
-        for q, d in zip(qs, data):
-            er, od = statistics_rows(q, d)
+        for q, d, prob in zip(qs, data, probs):
+            er, od = statistics_rows(q, d, output_shapes(prob))
             rows.append(er)
             outd.append(od)
         error = error_vector(q1, np.concatenate(rows, axis=0))
@@ -805,8 +865,9 @@ def xsim_sequence(plist):
 
 
 def xsim(p):
-    """[error, data, input, raw] = xsim(p) on the GPU path (xsim.m:28-188, no
-    comparisons).  data[o] is [ndata, 3] (mean, step error, sampling error);
+    """[error, data, input, raw] = xsim(p) on the GPU path (xsim.m:28-188, with
+    the comparisons of p.compare in x space).  data[o] is [ndata, 3] (mean,
+    step error, sampling error);
     raw is raw[nc][j] with j over all ensembles(2) x ensembles(3) paths
     (xsim.m:42,56-57), or None without p.rawdata.  A non-blank p.file saves
     error, data, input and raw to that '.mat' file (xsim.m:207-210).  A list of
@@ -829,7 +890,7 @@ def xsim(p):
             else:
                 dp = prob.xensemble(npar)
             data = dp if data is None else [a + b for a, b in zip(data, dp)]
-        error, outd = statistics(q, data)
+        error, outd = statistics(q, data, output_shapes(prob))
         error[-1] = time.time() - t0
         if q["file"].strip():
             from .xfile import xsave
