@@ -46,7 +46,7 @@ xensemble             Simulator/xensemble.m:12-108                  Batchtest #2
 xsim statistics       Simulator/xsim.m:64-188                       Batchtest rows,
                                                                     synthetic sigma test
 xsim sequences        xsim.m:33-38, xensemble.m:59-63 (transfer),   Batchtest #33
-                      xpreferences.m:27-28,197                      (Boundaries2), #20
+                      xpreferences.m:29-32,197                      (Boundaries2), #20
                                                                     (SolitonDerivS)
 xcompress             Applications/SPDE/xcompress.m:18-69           Batchtest #16
 Ave / Int / D1        Applications/SPDE/Ave.m, Int.m, D1.m          Batchtest #14, #16, #21
@@ -900,7 +900,7 @@ def _xsim_sequence(plist):
     for k, pin in enumerate(plist):
         q = dict(pin)
         if k > 0:
-            q["checks"] = list(ps[0].checks)                         # xpreferences.m:27-28
+            q["checks"] = list(ps[0].checks)                         # xpreferences.m:29-32
         q["_origin0"] = origin
         p = xpreferences(q)
         origin = p.origins[0] + p.ranges[0]                           # xpreferences.m:197

@@ -1,5 +1,5 @@
 """xsim of a sequence of problems on the host (xsim.m:33-38, xensemble.m:25-108,
-xpreferences.m:11-13,27-28,197) without a device: the path of every problem
+xpreferences.m:11-13,29-32,197) without a device: the path of every problem
 is the oracle's (a stand-in handle), so the product's orchestration - problem
 order per (j, nc), default and custom p.transfer, the chained time origins,
 the per-problem random streams, the moments per problem and the one error

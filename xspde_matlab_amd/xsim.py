@@ -767,7 +767,7 @@ def output_shapes(prob):
 
 
 def sequence_inputs(plist):
-    """The problems of a sequence as xpreferences.m:11-13,27-28,197 prepares
+    """The problems of a sequence as xpreferences.m:11-13,29-32,197 prepares
     them: checks from the first, each default time origin where the previous
     problem ended, the random stream of problem s keyed by s (gpu.seq), the
     ensembles of the first for every problem (xensemble.m:25-30 loops over
