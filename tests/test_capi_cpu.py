@@ -198,3 +198,17 @@ def test_time_spectra_flags():
     d.tspec[0] = 2
     rc, msg = _create(d)
     assert rc == 1 and "tspec" in msg
+
+
+def test_run_path_rejects_a_wrongly_sized_a0():
+    """The initial fields handed to xsh_run_path hold ensembles(1) x sites
+    complex values; the binding refuses any other size before the call (a
+    handle shell from xsh_plan's Info: no device is touched)."""
+    p = dict(dimensions=2, points=[5, 16], ranges=[1.0, 8.0], ensembles=[3, 1, 1], noises=0,
+             gpu=dict(deriv="zero", linear=dict(l2=0.5j), initial="gauss", observe=["abs2"]))
+    d = make_desc(preferences(p))
+    h = _lib.Handle.__new__(_lib.Handle)
+    h.desc, h.info, h._h = d, _lib.plan(d), None
+    assert h.info.nsites == 16
+    with pytest.raises(ValueError, match="ensembles\\(1\\) x sites = 48"):
+        h.run_path(1, 1, 1, a0=np.zeros((1, 16), complex))

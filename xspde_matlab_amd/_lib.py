@@ -205,6 +205,8 @@ class Handle:
         aout = np.zeros(2 * E * ns) if want_final else None
         a0c = None
         if a0 is not None:
+            if np.size(a0) != E * ns:
+                raise ValueError("a0 holds %d values; the path needs ensembles(1) x sites = %d" % (np.size(a0), E * ns))
             a0c = np.ascontiguousarray(np.stack([np.real(a0), np.imag(a0)], -1).ravel(), dtype=np.float64)
         check(load().xsh_run_path(self._h, npar, j, nc, _ptr(a0c), _ptr(aout), _ptr(av)))
         avc = av[0::2] + 1j * av[1::2]
