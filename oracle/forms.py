@@ -96,7 +96,7 @@ def make_linear(g):
         c2 = np.broadcast_to(np.asarray(l2, complex), (max(1, nd - 1),))
         L = l0 + 0j
         for d in range(1, nd):
-            L = L + c1[d - 1] * p.D[d] + c2[d - 1] * p.D[d] ** 2
+            L = L + c1[d - 1] * p.D[d + 1] + c2[d - 1] * p.D[d + 1] ** 2
         return L
     return linear
 

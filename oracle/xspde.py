@@ -12,7 +12,8 @@ Functions and the reference code they restate:
 ====================  ===========================================  =======================
 function              reference                                     pinned by
 ====================  ===========================================  =======================
-xpreferences          Simulator/xpreferences.m:42-219               Batchtest rows (below)
+xpreferences          Simulator/xpreferences.m:42-219               Batchtest rows (below),
+                      (left-out entries: xprefer.m:29-45)           default-padding test
                       Applications/SPDE/spdpreferences.m:16-72
                       Applications/SPDE/xgrid.m:18-48
 xpropfactor           Applications/SPDE/xpropfactor.m:23-58       linear-only exactness,
@@ -27,10 +28,11 @@ xpatch, da            xpatch.m:24-62, xpropfactor.m:40-51           inhomogeneou
                                                                     closed forms (DD, NN,
                                                                     DN, ND), Batchtest #42
                                                                     (time-dependent values)
-RK4                   Applications/SDE/RK4.m:7-42                   Batchtest #4, #8, #14,
-                                                                    #16, #21
+RK4                   Applications/SDE/RK4.m:7-42                   Batchtest #4, #5, #8,
+                                                                    #14, #16, #17, #21
 MP                    Applications/SDE/MP.m:10-27                   order test (dt vs dt/2),
-                                                                    Kubo closed form
+                                                                    Kubo closed form,
+                                                                    Batchtest #23, #33, #42
 xnoise / xrandom      Simulator/xnoise.m:26-60, xrandom.m:18-34     moments, KS test;
   (incl. knoises,     xift.m                                        filtered-noise covariance
   krandoms)                                                         closed form; MATLAB
@@ -49,7 +51,8 @@ xsim sequences        xsim.m:33-38, xensemble.m:59-63 (transfer),   Batchtest #3
                       xpreferences.m:29-32,197                      (Boundaries2), #20
                                                                     (SolitonDerivS)
 xcompress             Applications/SPDE/xcompress.m:18-69           Batchtest #16
-Ave / Int / D1        Applications/SPDE/Ave.m, Int.m, D1.m          Batchtest #14, #16, #21
+Ave / Int / D1        Applications/SPDE/Ave.m, Int.m, D1.m          Batchtest #14, #16, #17,
+                                                                    #21, #23
 D2                    Applications/SPDE/D2.m:38-78                  Batchtest #33, #20,
                                                                     quadratic with
                                                                     ghost-point ends
@@ -97,6 +100,17 @@ def _listify(x, n=None, default=None):
     return x
 
 
+def _prefer(x, dflt):
+    """xprefer.m:29-45: the given entries, then the default's entries for every
+    place the input leaves out (dflt is the full default list)."""
+    if x is None:
+        return list(dflt)
+    if callable(x) or not isinstance(x, (list, tuple, np.ndarray)):
+        x = [x]
+    x = list(x)
+    return x + list(dflt[len(x):])
+
+
 # ----------------------------------------------------------------- preferences
 def xpreferences(p_in):
     """Defaults, lattice, scales and grids (xpreferences.m, spdpreferences.m, xgrid.m)."""
@@ -127,14 +141,12 @@ def xpreferences(p_in):
     p.urandoms = int(p.get("urandoms", 0) or 0)
     p.ensembles = list(p.get("ensembles", [1, 1, 1])) + [1] * 3
     p.ensembles = [int(e) for e in p.ensembles[:3]]
-    p.ranges = np.asarray(_listify(p.get("ranges"), nd, 10.0), float)[:nd]
+    p.ranges = np.asarray(_prefer(p.get("ranges"), [10.0] * nd), float)[:nd]      # xpreferences.m:81
     st = p.get("steps", None)
     st = [1] * nd if st is None else _listify(st)
     p.steps = np.asarray(st + [1] * (nd - len(st)), int)[:nd]
     pts = p.get("points", None)
-    pts = [51] + [35] * (nd - 1) if pts is None else _listify(pts)
-    pts = list(pts) + [1] * (nd - len(pts))
-    pts = np.asarray(pts[:nd], int)
+    pts = np.asarray(_prefer(pts, [51] + [35] * (nd - 1))[:nd], int)             # xpreferences.m:56
     pts[1:] = 1 + (pts[1:] - 1) * p.steps[1:]                    # xpreferences.m:137
     p.points = pts
     og = p.get("origins", None)
@@ -251,11 +263,12 @@ def xpreferences(p_in):
     else:
         p.ks = [np.concatenate([ks_f[f][d] for f in range(F)], axis=0) for d in range(nd - 1)]
     names = ["x", "y", "z"]
-    for i, (xg, kg) in enumerate(zip(p.xs, p.ks)):
+    for i, (xg, kg) in enumerate(zip(p.xs[:3], p.ks[:3])):
         p[names[i]] = xg
         p["k" + names[i]] = kg
         p["D" + names[i]] = 1j * kg
-    p.D = [None] + [1j * kg for kg in p.ks]
+    p.D = [None, None] + [1j * kg for kg in p.ks]       # p.D{d} = i k_d, d = 2.. (xpropfactor.m:30-33)
+    p.r = [None, None] + list(p.xs)                     # p.r{d}, d = 2..: x grids (xgrid.m:19-21)
     p.xc = [p.origins[0] + p.dx[0] * np.arange(pts[0])] + \
            [p.origins[d] + np.arange(pts[d]) * dx[d] for d in range(1, nd)]
     p.kc = [None] + [(np.arange(pts[d]) - pts[d] // 2) * p.dk[d] for d in range(1, nd)]
@@ -266,15 +279,17 @@ def xpreferences(p_in):
     obs = p.get("observe", None)
     p.observe = _listify(obs) if obs is not None else [lambda a, p: a]
     nobs = len(p.observe)
-    p.transforms = [np.asarray(_listify(t, nd, 0))[:nd] for t in
-                    _listify(p.get("transforms", None), nobs, 0)]
+    # cells of per-output settings: entries left out take the default
+    # (xprefer.m:29-45; transforms zeros, xpreferences.m:240)
+    tr = p.get("transforms", None)
+    tr = [tr] if tr is not None and not isinstance(tr, (list, tuple)) else tr
+    p.transforms = [np.asarray(_prefer(t, [0] * nd)[:nd]) for t in _prefer(tr, [0] * nobs)]
     outs = p.get("output", None)
-    p.output = _listify(outs, nobs, None) if outs is not None else [None] * nobs
+    p.output = _prefer(outs, [None] * nobs)
     p.output = [o if o is not None else (lambda av, p, _n=n: av[_n])
                 for n, o in enumerate(p.output)]
-    p.compare = _listify(p.get("compare", None), len(p.output), None)
-    ax = p.get("axes", None)
-    p.axes = _listify(ax, len(p.output), None) if ax is not None else [None] * len(p.output)
+    p.compare = _prefer(p.get("compare", None), [None] * len(p.output))          # xpreferences.m:363
+    p.axes = _prefer(p.get("axes", None), [None] * len(p.output))
     p.cutoff = float(p.get("cutoff", -1e20))
     # time-frequency grid of the spectra (xpreferences.m:199-203) and the
     # spectrum switch (xpreferences.m:99,246)
@@ -752,10 +767,10 @@ def _grid_for_output(p, n, shape, kspace):
         grids.append(ax.reshape(sh))
     q.t = grids[0]
     names = ["x", "y", "z"]
-    for i in range(1, nd):
+    for i in range(1, min(nd, 4)):
         q[names[i - 1]] = grids[i]
         q["k" + names[i - 1]] = grids[i]
-    q.r = grids
+    q.r = [None] + grids                                # p.r{1} = t, p.r{d} (xgrid.m:52-55)
     return q, full
 
 

@@ -80,11 +80,12 @@ def preferences(p):
     steps = [int(s) for s in _vec(q.get("steps"), nd, 1)]
     q["steps"] = steps
     pts = q.get("points")
-    pts = [51] + [35] * (nd - 1) if pts is None else _vec(pts, nd, 1)
+    pts = [51] + [35] * (nd - 1) if pts is None else _vec(pts, nd, 35)             # xpreferences.m:56
     pts = [int(pts[0])] + [1 + (int(pts[d]) - 1) * steps[d] for d in range(1, nd)]   # :137
     q["points"] = pts
     og = q.get("origins")
-    q["origins"] = [0.0] + [-r / 2 for r in q["ranges"][1:]] if og is None else [float(o) for o in og]
+    og = [0.0] if og is None else [float(o) for o in np.ravel(og)]
+    q["origins"] = (og + [-r / 2 for r in q["ranges"][len(og):]])[:nd]        # xpreferences.m:83
     ch = q.get("checks")
     ch = [1] + [0] * (nd - 1) if ch is None else _vec(ch, nd, 0)
     if any(ch[1:]):
@@ -663,7 +664,8 @@ def statistics(q, data, shapes=None):
 
 def compare_grid(q):
     """p.t, p.x, p.y of xgrid.m:49-56 for a compare function: the time points
-    along axis 1 and the lattice coordinates along axes 2 and 3 (x space)."""
+    along axis 1 and the lattice coordinates along axes 2 and 3 (x space);
+    also as p.r, indexed as the reference's p.r{d} (index 0 unused)."""
     nd = q["dimensions"]
     pp = Params(q)
     npts = q["points"][0]
@@ -675,6 +677,7 @@ def compare_grid(q):
         s2 = [1] * (nd + 1)
         s2[k + 2] = n
         pp[name] = (q["origins"][k + 1] + np.arange(n) * (q["ranges"][k + 1] / max(1, n - 1))).reshape(s2)
+    pp["r"] = [None, pp["t"]] + [pp[name] for name in ("x", "y")[:nd - 1]]   # p.r{1} = t, p.r{d}
     return pp
 
 
