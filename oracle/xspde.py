@@ -29,10 +29,10 @@ xpatch, da            xpatch.m:24-62, xpropfactor.m:40-51           inhomogeneou
                                                                     DN, ND), Batchtest #42
                                                                     (time-dependent values)
 RK4                   Applications/SDE/RK4.m:7-42                   Batchtest #4, #5, #8,
-                                                                    #14, #16, #17, #21
+                                                                    #14, #16-#19, #21, #23
 MP                    Applications/SDE/MP.m:10-27                   order test (dt vs dt/2),
                                                                     Kubo closed form,
-                                                                    Batchtest #23, #33, #42
+                                                                    Batchtest #33, #42
 xnoise / xrandom      Simulator/xnoise.m:26-60, xrandom.m:18-34     moments, KS test;
   (incl. knoises,     xift.m                                        filtered-noise covariance
   krandoms)                                                         closed form; MATLAB
@@ -51,7 +51,7 @@ xsim sequences        xsim.m:33-38, xensemble.m:59-63 (transfer),   Batchtest #3
                       xpreferences.m:29-32,197                      (Boundaries2), #20
                                                                     (SolitonDerivS)
 xcompress             Applications/SPDE/xcompress.m:18-69           Batchtest #16
-Ave / Int / D1        Applications/SPDE/Ave.m, Int.m, D1.m          Batchtest #14, #16, #17,
+Ave / Int / D1        Applications/SPDE/Ave.m, Int.m, D1.m          Batchtest #14, #16-#18,
                                                                     #21, #23
 D2                    Applications/SPDE/D2.m:38-78                  Batchtest #33, #20,
                                                                     quadratic with
@@ -756,7 +756,9 @@ def _grid_for_output(p, n, shape, kspace):
     full = len(shape) - 2 == nd - 1
     axes = [p.xc[0]]
     for d in range(1, nd):
-        if kspace[d]:
+        if full and shape[d + 1] == 1 < p.points[d]:
+            axes.append(p.xc[d][:1])                            # an axis the output integrated out
+        elif kspace[d]:
             axes.append(np.fft.fftshift(np.fft.fftfreq(p.points[d], 1.0 / p.points[d])) * p.dk[d])
         else:
             axes.append(p.xc[d])
@@ -940,16 +942,21 @@ def _probe_shapes(p):
     q.ensembles = [1, 1, 1]
     a = np.zeros((p.fields,) + p.space + (1,), complex)
     av = []
+    obs = []
     for o in range(len(p.observe)):
         d = xdata(a, o, q)
         av.append(np.zeros((d.shape[0], int(p.points[0]), d.shape[1]), complex))
+        raw = np.asarray(p.observe[o](a, q))                     # [lines, space_o..., E]
+        obs.append(tuple(raw.shape[1:-1]) if raw.ndim == p.dimensions + 1 else ())
     shapes = []
-    for out in p.output:
+    for n, out in enumerate(p.output):
         r = np.asarray(out(av, p))
         lines, nt = r.shape[0], r.shape[1]
         nsp = int(np.prod(r.shape[2:]))
         if nsp == p.nspace and p.dimensions > 1:
             shapes.append((lines, nt) + p.space)
+        elif n < len(obs) and 1 < nsp == int(np.prod(obs[n] or (0,))):
+            shapes.append((lines, nt) + obs[n])                  # some axes integrated out (size 1)
         else:
             shapes.append((lines, nt))
     return shapes
@@ -967,10 +974,11 @@ def Int(o, *args):
     for transformed fields); a space dimension with step 0 is not integrated."""
     p = args[-1]
     dx = p.dx if len(args) == 1 else np.asarray(args[0], float)
+    partial = bool(np.any(np.asarray(dx[1:p.dimensions]) <= 0))  # MATLAB's sum keeps the other axes
     r = o
     for d in range(p.dimensions - 1, 0, -1):       # last axis first: lower indices stay valid
         if dx[d] > 0:
-            r = np.sum(r, axis=d) * dx[d]
+            r = np.sum(r, axis=d, keepdims=partial) * dx[d]
     return r
 
 

@@ -112,10 +112,29 @@ def gaussianav4d():                 # Examples/SPDE_nD/GaussianAv4D.m:12-42, Bat
                 compare=[g4, None, None, lambda p: 1], axes=[None, None, None, [0, -1, -1, -1, -1]])
 
 
+def gaussianav6d():                 # Examples/SPDE_nD/GaussianAv6D.m:10-42, Batchtest #18
+    """6D diffraction on 9^6 sites: transforms of axes 4-7 only, integrals over
+    some axes (zero steps keep the others: a [1, 1, 1, 9, 9, 1, 1] output) and
+    over mixed x/k steps, compressed axes, compares for outputs 2-4."""
+    def g6ft(p):
+        sig = 1 + (p.t / 10) ** 2
+        return sig ** -1 * np.exp(-(p.r[2] ** 2 + p.r[3] ** 2) / sig)
+    ii = lambda a, p: a * np.conj(a)
+    return dict(dimensions=7, points=[2, 9, 9, 9, 9, 9, 9], ranges=[10, 7, 7, 7, 7, 7, 7],
+                initial=lambda v, p: np.exp(-0.5 * sum(p.r[d] ** 2 for d in range(2, 8))),
+                observe=[ii, lambda a, p: X.Int(ii(a, p), p.dx * np.array([0, 1, 1, 0, 0, 1, 1]), p), ii,
+                         lambda a, p: X.Int(ii(a, p), np.concatenate([p.dx[:3], p.dk[3:7]]), p)],
+                linear=lambda p: 1j * 0.05 * sum(p.D[d] ** 2 for d in range(2, 8)),
+                transforms=[0, 0, [0, 0, 0, 1, 1, 1, 1], [0, 0, 0, 1, 1, 1, 1]],
+                axes=[[-1, -1, -1, -1, 0, 0, 0], [0, -1, -1, -1, -1, -1, -1], [0, 0, 0, -1, -1, -1, -1]],
+                compare=[None, lambda p: np.pi ** 2 * (1 + (p.t / 10) ** 2) ** -1, g6ft, lambda p: np.pi ** 3])
+
+
 def gpevortex2d():                  # Examples/SPDE_3D/GPEvortex2D.m:6-35, Batchtest #23
-    """A rotating 2D condensate in imaginary time: MP (noises default to the
-    field count), 40 x 40 periodic, order 4, a drift that renormalises every
-    step (normda: Int over the lattice) with the rotation term x D1_y - y D1_x."""
+    """A rotating 2D condensate in imaginary time: RK4 (one trajectory,
+    xpreferences.m:101-105), 40 x 40 periodic, order 4, a drift that
+    renormalises every step (normda: Int over the lattice) with the rotation
+    term x D1_y - y D1_x."""
     g, om = 200.0, 0.6
     V = lambda p: 0.35 * (p.x ** 2 + p.y ** 2)
     L = lambda a, p: 1j * (p.x * X.D1(a, 3, p) - p.y * X.D1(a, 2, p))
@@ -132,7 +151,8 @@ def gpevortex2d():                  # Examples/SPDE_3D/GPEvortex2D.m:6-35, Batch
 
 ROWS = [(soliton, 5.047219e-03), (characteristic, 9.654854e-05), (solitonav, 1.410144e-02),
         (gaussianav2d, 4.813746e-07), (gaussiankinetic, 4.049843e-06), (solitonderivn, 2.134464e-06),
-        (gpevortex2d, 1.772071e-03), (gaussian3d, 1.682525e-11), (gaussianav4d, 2.444070e-07)]
+        (gpevortex2d, 1.772071e-03), (gaussian3d, 1.682525e-11), (gaussianav4d, 2.444070e-07),
+        (gaussianav6d, 6.243503e-05)]
 
 
 @pytest.mark.parametrize("make,expected", ROWS, ids=[r[0].__name__ for r in ROWS])
