@@ -389,3 +389,29 @@ def test_left_out_entries_take_the_defaults():
     h = preferences(dict(dimensions=3, ranges=[5.0], points=[11, 16], origins=[0.0, -1.0], gpu=dict(deriv="zero")))
     assert h["ranges"] == [5.0, 10.0, 10.0] and h["points"] == [11, 16, 35]
     assert h["origins"] == [0.0, -1.0, -5.0]
+
+
+def gpe2(seed=0, noise=0.01):       # Examples/SPDE_2D/GPE2.m:12-31, Batchtest #12
+    V = lambda p: 0.1 * p.x ** 2
+
+    def deriv(a, w, p):
+        rho = np.conj(a[0:1]) * a[0:1] + np.conj(a[1:2]) * a[1:2]
+        return -1j * a * (np.concatenate([V(p), 2 * V(p)]) + rho) + noise * w
+    return dict(dimensions=2, fields=2, order=2, points=[101, 64], ranges=[20, 20], ensembles=[6], steps=4,
+                seed=seed, initial=lambda w, p: np.concatenate([sech(p.x + 5), sech(p.x - 5)]) + 0j,
+                deriv=deriv, linear=lambda p: np.concatenate([0.5j * p.Dx ** 2, 0.5j * p.Dx ** 2]),
+                observe=[lambda a, p: a[0:1] * np.conj(a[0:1]), lambda a, p: a[1:2] * np.conj(a[1:2])])
+
+
+def test_batchtest_gpe2_statistically():
+    """GPE2 #12 (6.585455e-03): two components coupled through their shared
+    density, one propagator row per component, MP with weak noise.  Its step
+    error depends on MATLAB's stream; the expected value lies inside the spread
+    of 28 seeds of the counter stream, which the noiseless project (3 %
+    below it) does not reach."""
+    e = 6.585455e-03
+    v = np.array([X.xsim(gpe2(s))[0][0] for s in range(28)])
+    print("GPE2 over 28 seeds: %.4f .. %.4f of the expected value" % (v.min() / e, v.max() / e))
+    assert v.min() < e < v.max()
+    quiet = X.xsim(gpe2(0, noise=0.0))[0][0]
+    assert quiet < v.min() and abs(quiet / e - 1) > 0.02
